@@ -1,0 +1,137 @@
+/*
+ * kmc_oracle.h -- CPU restatement of the MulSKIPS KMC event engine.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine
+ * in mulskips_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product path never calls it.
+ *
+ * It restates, in plain C and FP64, the reference's serial BKL engine with
+ * the same data layout (packed LattCoo words, LattInd, the three particle
+ * lists with swap-with-last deletion, the FP64 binary sum tree), the same
+ * random-number consumption order (ran3 + the fixed numberslist table for
+ * RunType 'T') and the same floating-point expressions, so that a deck run
+ * through it reproduces the reference's xyz output byte for byte.
+ *
+ * Citations are relative to the reference repository root
+ * (mulskips-source/...).  Each function in kmc_oracle.c names the file:line
+ * range it restates.
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   - ran3, tree, Get_Prob, Deposition, Evaporation, absorption, desorption,
+ *     FIND_NN, FIND_SN, IF_Arm_Choice, the initial builders used by the
+ *     regression decks and the xyz writer are pinned by the 8 valid
+ *     regression fixtures (regression-tests/input-and-references-files):
+ *     both I00000000.xyz and I00000001.xyz byte-identical, plus the final
+ *     clock/site/counter lines of the logs.
+ *   - FIND_NN/FIND_SN are additionally pinned by closed forms (rotation by
+ *     +-120 deg about a bond, point inversion, mirror reflection) over every
+ *     pair of the 32 bond vectors.
+ *   - ran3 is pinned by its closed integer recurrence and by the Knuth
+ *     subtractive-generator identity checked in the tests.
+ *   - LA-mode rates (Get_Prob.f:127-149) are pinned by the closed form only
+ *     (the reference LA fixture is stale against the current source):
+ *     "parity pinned by closed form, not by a reference run".
+ */
+#ifndef MSK_KMC_ORACLE_H
+#define MSK_KMC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_sim orc_sim;
+
+/* One executed event, as recorded by orc_step/orc_main.  This is the event
+ * log format consumed by the engine's replay entry point (mskg_replay). */
+typedef struct {
+  int64_t iter;       /* 1-based iteration number (main.f Iter)            */
+  int32_t site[3];    /* grid coordinates of the picked MC particle        */
+  int32_t itrans;     /* Ind_Event at pick time: 1..3 deposit species,
+                         4 evaporate, 5 desorb, 6.. absorb coverage (k-5) */
+  int32_t arm;        /* deposition at Coor 1: 1 arm-chair, 0 zig-zag;
+                         -1 when no stacking choice was made               */
+  int32_t coor;       /* coordination of the site before the event        */
+  int32_t pad;
+  double  rtot;       /* Tree(1) at pick time (total rate R)              */
+  double  time;       /* clock after the event (Time += 1/R)              */
+} orc_event;
+
+/* Box sizes are run-time here; the reference fixes them at compile time
+ * (mulskips-source/modules/defsystem.f:27). */
+orc_sim *orc_create(int len_x, int len_y, int len_z);
+void     orc_destroy(orc_sim *s);
+const char *orc_last_error(const orc_sim *s);
+
+/* RunType 'T' replays this table of uniforms (PickTreeEvent.f:72-1079). */
+int orc_set_numberslist(orc_sim *s, const double *values, int n);
+
+/* Parse a start.dat (main.f:83-212 + SetProbability.f:21-143), set up the
+ * tree (AllocateArrays.f) and run the initial builder chosen by InitSt
+ * (main.f:224-294).  Supported builders: F, FS, C, S, I, FF.
+ * Returns 0 on success. */
+int orc_load_deck(orc_sim *s, const char *start_dat_path);
+
+/* The whole program main.f:296-379: frame 0 output, the KMC loop with its
+ * output cadence and the final frame.  When outdir is non-NULL the
+ * I????????.xyz (and _d.xyz) frames are written there.  When log is
+ * non-NULL, up to log_cap events are recorded (log_n receives the count).
+ * stop_reason: 0 limit reached, 1 "few MC particles", 2 error. */
+int orc_main(orc_sim *s, const char *outdir, orc_event *log, int64_t log_cap,
+             int64_t *log_n, int *stop_reason);
+
+/* One iteration of main.f:304-371 without frame output.  Returns 0 after an
+ * executed event, 1 for the early stop (NumAdAtom<=20), 2 on error. */
+int orc_step(orc_sim *s, orc_event *ev);
+
+/* Write the main xyz frame of WriteMolMolXYZFile.f (and its SiC
+ * wrong-site side effect on LattCoo) exactly as the reference does. */
+int orc_write_frame(orc_sim *s, const char *xyz_path, const char *d_path,
+                    int64_t iter);
+
+/* ---- state access (for building device states and comparing) --------- */
+int32_t *orc_lattcoo(orc_sim *s);   /* (LenX, LenY, LenZ+1), x fastest   */
+int32_t *orc_lattind(orc_sim *s);
+void orc_dims(const orc_sim *s, int *lx, int *ly, int *lz);
+void orc_counts(const orc_sim *s, int32_t *n_ad, int32_t *n_at, int32_t *n_vd);
+/* Copy list i (0=MC, 1=bulk, 2=void) out: xyz[3n], nn[12n] (Fortran
+ * NextNXYZ(3,4) order), ev[n] and prob[n] only for MC (may be NULL). */
+int orc_get_list(const orc_sim *s, int which, int32_t *xyz, int32_t *nn,
+                 int32_t *ev, double *prob);
+double  orc_time(const orc_sim *s);
+double  orc_total_rate(const orc_sim *s);   /* Tree(1) */
+int64_t orc_iter(const orc_sim *s);
+void orc_count_crystal(const orc_sim *s, int32_t cc[3], int32_t cov[3]);
+int  orc_ncrystal(const orc_sim *s, int *ncov);
+/* Rate tables as parsed (PTransE(3,0:3,...,0:3) Fortran order, PTransD(3,3),
+ * PTransAbs(3,0:3), PTransDes(3,0:3)); any pointer may be NULL. */
+void orc_tables(const orc_sim *s, double *pe, double *pd, double *pabs,
+                double *pdes, double *ptrans_zig, int32_t list_crystal[3],
+                int32_t list_cov[3]);
+
+/* Get_Prob (Get_Prob.f:54-331) evaluated on the current state for the MC
+ * particle at xyz, without consuming random numbers and without side
+ * effects: returns the event category (0 none, 1 adsorption on an empty
+ * site [species/coverage drawn at execution], 2 placeholder deposition
+ * (rate 1e-8), 4 evaporation, 5 desorption, 6 isolated-atom evaporation
+ * (rate 1e5*R)) and the FP64 rate. */
+int orc_pure_rate(orc_sim *s, const int32_t xyz[3], int32_t *category,
+                  double *rate);
+
+/* FIND_NN.f / FIND_SN.f restated (FP64 Rodrigues rotation / inversion /
+ * translation then NINT), exported for the bond-code table proof. */
+void orc_find_nn(const int32_t s1[3], const int32_t s2[3], const int32_t sc[3],
+                 int32_t out[2][3], int lx, int ly);
+void orc_find_sn(const int32_t site[3], const int32_t sitec[3],
+                 const int32_t sn[3][3], int32_t zig[3][3], int32_t arm[3][3],
+                 int lx, int ly);
+
+/* ran3 (random.f:22-69) as a stand-alone generator for tests. */
+typedef struct { int32_t ma[56]; int32_t inext, inextp, iff; } orc_ran3;
+double orc_ran3_next(orc_ran3 *r, int32_t *idum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
