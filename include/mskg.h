@@ -1,0 +1,159 @@
+/*
+ * mskg.h -- C ABI of the MI355X KMC event engine for MulSKIPS.
+ *
+ * The engine replaces the reference's KMC loop (mulskips-source/main.f:
+ * 304-371) and the routines it drives: rate evaluation (mcevent/Get_Prob.f,
+ * Get_Prob_Ini.f), event selection (mcevent/PickTreeEvent.f, UpdateTree.f,
+ * random.f) and event execution (mcevent/deposition.f, Evaporation.f,
+ * absorption.f, desorption.f, Add.f, Erase.f, IF_Arm_Choice.f,
+ * lattice/FIND_NN.f, FIND_SN.f).  The Fortran host keeps parsing start.dat,
+ * building the initial lattice (initial/*.f) and writing frames
+ * (inpout/*.f); it hands the lattice to the engine with mskg_load_state,
+ * advances it with mskg_run up to each output boundary and takes it back with
+ * mskg_store_state.  Binding stubs for iso_c_binding are in
+ * mulskips_amd/fortran/mskg_bindings.f90.
+ *
+ * Conventions: plain C types only; every entry point returns 0 on success
+ * and a nonzero MSKG_E* code on failure (the reference STOPs instead);
+ * mskg_last_error() gives the message.  One engine owns one GPU stream and
+ * its device memory; calls on one engine must be serialised by the caller,
+ * exactly as the reference is single-threaded.  Grid coordinates are the
+ * reference's 0-based (x, y, z) with 0<=x<len_x, 0<=y<len_y, 0<=z<=len_z;
+ * lattice arrays are (len_x, len_y, len_z+1) with x fastest, the layout of
+ * LattCoo (modules/defsystem.f:49).
+ */
+#ifndef MSKG_H
+#define MSKG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MSKG_OK = 0,
+  MSKG_EINVAL = 1,      /* bad argument or configuration                        */
+  MSKG_ESTATE = 2,      /* lattice state inconsistent (the reference would STOP) */
+  MSKG_ECAPACITY = 3,   /* a caller-provided buffer or a tile list is too small  */
+  MSKG_EDEVICE = 4,     /* HIP runtime error or no usable GPU                    */
+  MSKG_ENOMEM = 5
+};
+
+/* stop_reason values of mskg_run (main.f:307-310, 361-366) */
+enum { MSKG_STOP_LIMIT = 0, MSKG_STOP_FEW_MC = 1, MSKG_STOP_NO_EVENTS = 3 };
+
+typedef struct mskg_engine mskg_engine;
+
+/* Everything the reference keeps in MODULE DefSystem / Definitions that the
+ * engine needs (defsystem.f:27-68, definitions.f:21-46, main.f:83-212). */
+typedef struct {
+  int32_t len_x, len_y, len_z;          /* LenX, LenY, LenZ; len_x, len_y multiples of 12 */
+  int32_t n_crystal, n_cov;             /* NCrystal, NCov                                */
+  int32_t list_crystal[3], list_cov[3]; /* ListCrystal, ListCov (atomic numbers)         */
+  const double *ptrans_e;   /* PTransE(3,0:3,0:3,0:3,0:3,0:3,0:3), Fortran order, 12288 values */
+  const double *ptrans_d;   /* PTransD(3,1:3), 9 values                           */
+  const double *ptrans_abs; /* PTransAbs(3,0:3), 12 values (may be NULL if n_cov=0) */
+  const double *ptrans_des; /* PTransDes(3,0:3), 12 values (may be NULL if n_cov=0) */
+  double ptrans_zig;        /* PtransZig                                          */
+  int32_t la_mode;          /* 1: laser-annealing rates from a per-site T (Get_Prob.f:127-149) */
+  double tm, p0, sigma, tflex;
+  uint64_t seed;            /* Philox-4x32-10 key for this replica                */
+  int32_t tile;             /* synchronous-sublattice tile edge (grid units, even,
+                               >=16, len_x/tile and len_y/tile even); 0 = choose  */
+  int32_t tile_capacity;    /* max MC particles per tile held on chip; 0 = 1024   */
+  double events_per_pass;   /* window tau = this / (largest tile rate); 0 = 1.0   */
+  int32_t reserved[8];
+} mskg_config;
+
+/* One logged event, as written by the reference loop instrumented at
+ * main.f:312-357 (or by the CPU oracle): the picked site, Ind_Event and, for a
+ * deposition at Coor 1, the IF_Arm_Choice outcome. */
+typedef struct {
+  int64_t iter;
+  int32_t site[3];
+  int32_t itrans;   /* 1..3 deposit species, 4 evaporate, 5 desorb, 6.. absorb ListCov(k-5) */
+  int32_t arm;      /* 1 arm-chair, 0 zig-zag, -1 not applicable */
+  int32_t coor;
+  double rtot;      /* Tree(1) before the pick */
+  double time;      /* clock after the event   */
+} mskg_event;
+
+/* Creates the engine on the current HIP device and builds the geometry
+ * tables.  Replaces AllocateArrays.f:24-45 and the table set-up of
+ * SetProbability.f:21-143 (the host still parses start.dat). */
+int mskg_create(const mskg_config *cfg, mskg_engine **out);
+void mskg_destroy(mskg_engine *e);
+const char *mskg_last_error(const mskg_engine *e);
+
+/* Uploads the host lattice: LattCoo (len_x*len_y*(len_z+1) words, bit layout
+ * of defsystem.f:87-91) and the three particle lists with their NextNXYZ
+ * (3x4 ints per record, Fortran order), ListAdAtom's Ind_Event/ProbTrans.
+ * Replaces nothing in the reference (its lists are module globals); it is
+ * the hand-over point after the initial builders (main.f:298). */
+int mskg_load_state(mskg_engine *e, const int32_t *latt_coo,
+                    int32_t n_ad, const int32_t *ad_xyz, const int32_t *ad_nn,
+                    const int32_t *ad_event, const double *ad_prob,
+                    int32_t n_bulk, const int32_t *bulk_xyz, const int32_t *bulk_nn,
+                    int32_t n_void, const int32_t *void_xyz, const int32_t *void_nn,
+                    double kmc_time);
+
+/* Per-site temperature in K for laser-annealing mode, (len_x, len_y,
+ * len_z+1) like LattCoo's T bits (SetT.f:25-50). */
+int mskg_set_temperature(mskg_engine *e, const int32_t *temp_k);
+
+/* Rate-update sweep: re-evaluates Get_Prob (Get_Prob.f:54-331) for every
+ * MC particle of the lattice from the current state.  */
+int mskg_refresh_rates(mskg_engine *e);
+
+/* Advances the lattice with the parallel synchronous-sublattice KMC until
+ * max_events events have executed or the clock passes max_time (whichever
+ * first; the check is made between sweeps).  Replaces main.f:304-371. */
+int mskg_run(mskg_engine *e, int64_t max_events, double max_time,
+             int64_t *events_done, double *kmc_time, int32_t *stop_reason);
+
+/* Applies a logged event sequence in order with the reference's update rules
+ * (no random numbers are drawn: species and stacking come from the log).
+ * clock_mode 0 advances the clock by 1/rtot per event (main.f:313). */
+int mskg_replay(mskg_engine *e, const mskg_event *log, int64_t n, int32_t clock_mode);
+
+/* Downloads the lattice back into LattCoo and the three lists (ordered by
+ * grid position; capacities in records).  Ind_Event of an adsorption site is
+ * reported as 1; ProbTrans is the FP32 rate widened to FP64. */
+int mskg_store_state(mskg_engine *e, int32_t *latt_coo,
+                     int32_t cap_ad, int32_t *n_ad, int32_t *ad_xyz, int32_t *ad_nn,
+                     int32_t *ad_event, double *ad_prob,
+                     int32_t cap_bulk, int32_t *n_bulk, int32_t *bulk_xyz, int32_t *bulk_nn,
+                     int32_t cap_void, int32_t *n_void, int32_t *void_xyz, int32_t *void_nn);
+
+/* Event class and FP32 rate of n sites (class codes: 0 none, 1 adsorption,
+ * 2 placeholder deposition, 4 evaporation, 5 desorption, 6 isolated atom). */
+int mskg_site_rates(mskg_engine *e, int64_t n, const int32_t *xyz, int32_t *cls, float *rate);
+
+/* Counters since load: executed events by reference event type
+ * (counts[0..2] deposition of species 1..3, [3] evaporation, [4] desorption,
+ * [5..7] absorption of ListCov 1..3), the KMC clock, and list sizes
+ * (NumAdAtom, NumAtoms, NumVoids of defsystem.f:36). */
+int mskg_counters(mskg_engine *e, int64_t counts[8], double *kmc_time,
+                  int64_t *n_adatom, int64_t *n_atoms, int64_t *n_voids);
+
+/* Timing of the last mskg_run / mskg_refresh_rates (device milliseconds,
+ * measured with HIP events on the engine's stream) and the number of lattice
+ * points the last rate sweep read. */
+int mskg_last_timing(mskg_engine *e, double *run_ms, double *sweep_ms, int64_t *sweep_points,
+                     int64_t *sweeps_done);
+
+/* Host evaluation of the engine's FP64 geometry -- the same source the
+ * kernels run -- for checking it against the reference formulas without a
+ * GPU.  FIND_NN (lattice/FIND_NN.f:25-77): out = the two new neighbours of
+ * sc given s1 and s2.  FIND_SN (lattice/FIND_SN.f:26-84): zig/arm = the
+ * zig-zag and arm-chair candidates (3x3, row per candidate). */
+int mskg_geometry_find_nn(const int32_t s1[3], const int32_t s2[3], const int32_t sc[3], int32_t len_x,
+                          int32_t len_y, int32_t out[6]);
+int mskg_geometry_find_sn(const int32_t site[3], const int32_t sitec[3], const int32_t sn[9], int32_t len_x,
+                          int32_t len_y, int32_t zig[9], int32_t arm[9]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,184 @@
+"""start.dat input decks: parser and writer.
+
+Restates the reference's input grammar: the positional, list-directed header
+read by mulskips-source/main.f:83-212 and the rate blocks read by
+mcevent/SetProbability.f:21-143, in the form pymulskips writes them
+(pymulskips/setuprun.py:292-389).  The engine needs only the rate tables
+and species lists (mskg_config); the rest of the header drives the host.
+"""
+import dataclasses
+import re
+
+import numpy as np
+
+# Number of evaporation lines per (NCrystal, NCov), SetProbability.f:47-75.
+MAX_ELINES = {(1, 0): 3, (2, 0): 18, (3, 0): 57, (1, 1): 9, (2, 1): 50, (3, 1): 150,
+              (1, 2): 19, (2, 2): 100, (3, 2): 288, (1, 3): 34, (2, 3): 172, (3, 3): 480}
+
+
+@dataclasses.dataclass
+class Deck:
+    n_crystal: int
+    n_cov: int
+    list_crystal: list
+    list_cov: list
+    init_st: str
+    lens: list                 # Len1..Len4 (or the LA block for 'LA')
+    ptrans_zig: float
+    sf: float
+    exit_strategy: str         # 'Iter' or 'Time'
+    max_iter: int = 0
+    out_molmol: int = 0
+    max_time: float = 0.0
+    out_time: float = 0.0
+    run_type: str = "R"
+    idum: int = 1
+    save_final_state: str = "F"
+    save_coo: str = "F"
+    alloy_fraction: float = 0.0
+    evap: list = dataclasses.field(default_factory=list)  # (indsp, digits-string, value)
+    depo: list = dataclasses.field(default_factory=list)  # (indsp, coor, value)
+    absn: list = dataclasses.field(default_factory=list)  # (indsp, coor, value)
+    deso: list = dataclasses.field(default_factory=list)
+    extra_lines: dict = dataclasses.field(default_factory=dict)
+
+    # ---- rate tables as the engine takes them (Fortran order) -----------
+    def tables(self):
+        pe = np.zeros(3 * 4 ** 6)
+        nd = self.n_crystal + self.n_cov
+        for indsp, digits, value in self.evap:
+            elind = int(digits)
+            ind = [0] * 6
+            for j in range(1, self.n_crystal + 1):
+                ind[j - 1] = (elind // 10 ** (nd - j)) % 10
+            for j in range(self.n_crystal + 1, nd + 1):
+                ind[3 + j - self.n_crystal - 1] = (elind // 10 ** (nd - j)) % 10
+            flat = (indsp - 1) + 3 * (ind[0] + 4 * (ind[1] + 4 * (ind[2] + 4 * (ind[3] + 4 * (ind[4] + 4 * ind[5])))))
+            pe[flat] = value
+        pd = np.zeros(9)
+        for indsp, coor, value in self.depo:
+            pd[(indsp - 1) + 3 * (coor - 1)] = value
+        pabs = np.zeros(12)
+        for indsp, coor, value in self.absn:
+            pabs[(indsp - self.n_crystal - 1) + 3 * coor] = value
+        pdes = np.zeros(12)
+        for indsp, coor, value in self.deso:
+            pdes[(indsp - self.n_crystal - 1) + 3 * coor] = value
+        lc = (list(self.list_crystal) + [0, 0, 0])[:3]
+        lcov = (list(self.list_cov) + [0, 0, 0])[:3]
+        return dict(ptrans_e=pe, ptrans_d=pd, ptrans_abs=pabs, ptrans_des=pdes, ptrans_zig=self.ptrans_zig,
+                    n_crystal=self.n_crystal, n_cov=self.n_cov, list_crystal=np.array(lc, np.int32),
+                    list_cov=np.array(lcov, np.int32))
+
+
+def _fields(line):
+    """Tokens of a list-directed record (the reference reads only the leading
+    items it needs; everything after them, comments included, is skipped)."""
+    return re.split(r"[\s,]+", line.strip())
+
+
+def _real(tok):
+    return float(tok.replace("d", "e").replace("D", "e"))
+
+
+def read_start_dat(path):
+    with open(path) as f:
+        lines = f.read().splitlines()
+    it = iter(lines)
+    nxt = lambda: next(it)  # noqa: E731
+    t = _fields(nxt())
+    ncr, ncov = int(t[0]), int(t[1])
+    if not (1 <= ncr <= 3) or not (0 <= ncov <= 3):
+        raise ValueError("NCrystal must be 1..3 and NCov 0..3")
+    lc = [int(v) for v in _fields(nxt())[:ncr]]
+    lcov = [int(v) for v in _fields(nxt())[:ncov]] if ncov else []
+    init_st = _fields(nxt())[0][:2]
+    extra = {}
+    if init_st == "FF":
+        lens = [int(v) for v in _fields(nxt())[:4]]
+    elif init_st == "ST":
+        t = _fields(nxt())
+        lens = [int(t[0]), int(t[1]), int(t[2]), _real(t[3]), _real(t[4]), _real(t[5])]
+    elif init_st in ("IN", "SH"):
+        extra["cadfilename"] = nxt()
+        lens = []
+    elif init_st == "LA":
+        extra["cadfilename"] = nxt()
+        extra["tempfilename"] = nxt()
+        lens = [_real(_fields(nxt())[0]) for _ in range(6)]
+        extra["homogeneous"] = _fields(nxt())[0]
+    else:
+        lens = [int(v) for v in _fields(nxt())[:3]]
+    alloy = 0.0
+    if lc[:2] == [14, 32] and (ncr == 2 or (ncr == 3 and lc[2] == 5)):
+        alloy = _real(_fields(nxt())[0])  # main.f:142-148
+    zig = _real(_fields(nxt())[0])
+    sf = _real(_fields(nxt())[0])
+    exit_s = _fields(nxt())[0][:4]
+    d = Deck(ncr, ncov, lc, lcov, init_st, lens, zig, sf, exit_s, alloy_fraction=alloy, extra_lines=extra)
+    if exit_s == "Iter":
+        d.max_iter = int(_fields(nxt())[0])
+        d.out_molmol = int(_fields(nxt())[0])
+    elif exit_s == "Time":
+        d.max_time = _real(_fields(nxt())[0])
+        d.out_time = _real(_fields(nxt())[0])
+    d.run_type = _fields(nxt())[0][:2]
+    d.idum = int(_fields(nxt())[0])
+    d.save_final_state = _fields(nxt())[0][:2]
+    if d.save_final_state == "T":
+        extra["restartfilename"] = nxt()
+    d.save_coo = _fields(nxt())[0][:2]
+    if d.save_coo == "T":
+        extra["coofilename"] = nxt()
+    for _ in range(MAX_ELINES[(ncr, ncov)]):
+        t = _fields(nxt())
+        indsp = int(t[0])
+        if indsp > ncr:
+            raise ValueError("1st index in evaporation block must be <= NCrystal")
+        d.evap.append((indsp, t[1], _real(t[2])))
+    for _ in range(3 * ncr):
+        t = _fields(nxt())
+        d.depo.append((int(t[0]), int(t[1]), _real(t[2])))
+    for blk in (d.absn, d.deso):
+        for _ in range(3 * ncov):
+            t = _fields(nxt())
+            blk.append((int(t[0]), int(t[1]), _real(t[2])))
+    return d
+
+
+def write_start_dat(d, path):
+    """Write a deck in the grammar read_start_dat (and main.f) accepts."""
+    out = [f"{d.n_crystal} {d.n_cov} ! NCrystal, NCov",
+           " ".join(str(v) for v in d.list_crystal) + " ! crystal species"]
+    if d.n_cov:
+        out.append(" ".join(str(v) for v in d.list_cov) + " ! coverage species")
+    out.append(f"{d.init_st} ! InitSt")
+    if d.init_st in ("IN", "SH"):
+        out.append(d.extra_lines["cadfilename"])
+    elif d.init_st == "LA":
+        out += [d.extra_lines["cadfilename"], d.extra_lines["tempfilename"]]
+        out += [repr(float(v)) for v in d.lens[:6]]
+        out.append(d.extra_lines.get("homogeneous", "F"))
+    else:
+        out.append(" ".join(str(v) for v in d.lens) + " ! Len1 Len2 Len3 [Len4]")
+    if d.list_crystal[:2] == [14, 32]:
+        out.append(repr(float(d.alloy_fraction)) + " ! alloy fraction")
+    out.append(repr(float(d.ptrans_zig)) + " ! PtransZig")
+    out.append(repr(float(d.sf)) + " ! KMC_sf")
+    out.append(f"{d.exit_strategy} ! ExitStrategy")
+    if d.exit_strategy == "Iter":
+        out += [f"{d.max_iter} ! MaxIter", f"{d.out_molmol} ! OutMolMol"]
+    else:
+        out += [f"{repr(float(d.max_time))} ! MaxTime", f"{repr(float(d.out_time))} ! OutTime"]
+    out += [f"{d.run_type} ! RunType", f"{d.idum} ! IDUM", f"{d.save_final_state} ! SaveFinalState"]
+    if d.save_final_state == "T":
+        out.append(d.extra_lines["restartfilename"])
+    out.append(f"{d.save_coo} ! SaveCoo")
+    if d.save_coo == "T":
+        out.append(d.extra_lines["coofilename"])
+    out += [f"{a} {b} {repr(float(v))}" for a, b, v in d.evap]
+    out += [f"{a} {b} {repr(float(v))}" for a, b, v in d.depo]
+    out += [f"{a} {b} {repr(float(v))}" for a, b, v in d.absn]
+    out += [f"{a} {b} {repr(float(v))}" for a, b, v in d.deso]
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")

@@ -1,0 +1,188 @@
+"""Host-side mirror of the reference's KMC loop, driving the HIP engine.
+
+The reference's Fortran host (mulskips-source/main.f) parses start.dat,
+builds the lattice, then loops PickTreeEvent / Deposition / Evaporation /
+absorption / desorption (main.f:304-371).  Here the lattice is handed to the
+GPU engine with `load_state`, advanced with `run` (or `replay` of a recorded
+event log) and fetched back with `store_state`, through the C ABI of
+include/mskg.h.  Errors the reference turns into STOP become exceptions.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, EVENT_DTYPE, ptr
+
+CLASS_NAMES = {0: "none", 1: "adsorb", 2: "placeholder", 4: "evaporate", 5: "desorb", 6: "isolated"}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mskg error {code}: {msg}")
+        self.code = code
+
+
+class Engine:
+    """One lattice resident on the current GPU.
+
+    tables: dict with ptrans_e (12288, Fortran order of PTransE), ptrans_d (9),
+    ptrans_abs (12), ptrans_des (12), ptrans_zig, n_crystal, n_cov,
+    list_crystal, list_cov -- the quantities SetProbability.f reads from
+    start.dat.
+    """
+
+    def __init__(self, box, tables, seed=1, tile=0, tile_capacity=0, events_per_pass=0.0, la=None):
+        self.L = _lib.lib()
+        lx, ly, lz = box
+        self.box = (int(lx), int(ly), int(lz))
+        self._arrays = {k: np.ascontiguousarray(tables[k], np.float64)
+                        for k in ("ptrans_e", "ptrans_d", "ptrans_abs", "ptrans_des")}
+        cfg = Config()
+        cfg.len_x, cfg.len_y, cfg.len_z = self.box
+        cfg.n_crystal = int(tables["n_crystal"])
+        cfg.n_cov = int(tables["n_cov"])
+        for i in range(3):
+            cfg.list_crystal[i] = int(tables["list_crystal"][i])
+            cfg.list_cov[i] = int(tables["list_cov"][i])
+        cfg.ptrans_e = ptr(self._arrays["ptrans_e"])
+        cfg.ptrans_d = ptr(self._arrays["ptrans_d"])
+        cfg.ptrans_abs = ptr(self._arrays["ptrans_abs"])
+        cfg.ptrans_des = ptr(self._arrays["ptrans_des"])
+        cfg.ptrans_zig = float(tables["ptrans_zig"])
+        if la:
+            cfg.la_mode = 1
+            cfg.tm, cfg.p0, cfg.sigma, cfg.tflex = (float(la[k]) for k in ("tm", "p0", "sigma", "tflex"))
+        cfg.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        cfg.tile = int(tile)
+        cfg.tile_capacity = int(tile_capacity)
+        cfg.events_per_pass = float(events_per_pass)
+        self._cfg = cfg
+        h = C.c_void_p()
+        rc = self.L.mskg_create(C.byref(cfg), C.byref(h))
+        self.h = h
+        if rc:
+            msg = self.L.mskg_last_error(h).decode() if h else "allocation failed"
+            self.close()
+            raise EngineError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mskg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise EngineError(rc, self.L.mskg_last_error(self.h).decode())
+
+    def load_state(self, lattcoo, mc, bulk, void, kmc_time=0.0):
+        """lattcoo: int32 array (LenZ+1, LenY, LenX); mc/bulk/void: dicts with
+        'xyz' (n,3) and 'nn' (n,4,3) int32 arrays (NextNXYZ), mc also
+        'event' (Ind_Event) and 'prob' (ProbTrans)."""
+        lat = np.ascontiguousarray(lattcoo, np.int32)
+        lx, ly, lz = self.box
+        assert lat.shape == (lz + 1, ly, lx), lat.shape
+
+        def arr(d, k, dt):
+            return np.ascontiguousarray(d[k], dt) if len(d["xyz"]) else np.zeros((1,) + np.shape(d[k])[1:], dt)
+
+        keep = []
+        args = []
+        for d, extra in ((mc, True), (bulk, False), (void, False)):
+            n = len(d["xyz"])
+            xyz, nn = arr(d, "xyz", np.int32), arr(d, "nn", np.int32)
+            keep += [xyz, nn]
+            args += [n, ptr(xyz), ptr(nn)]
+            if extra:
+                ev, pr = arr(d, "event", np.int32), arr(d, "prob", np.float64)
+                keep += [ev, pr]
+                args += [ptr(ev), ptr(pr)]
+        self._check(self.L.mskg_load_state(self.h, ptr(lat), *args, float(kmc_time)))
+
+    def set_temperature(self, temp_k):
+        t = np.ascontiguousarray(temp_k, np.int32)
+        self._check(self.L.mskg_set_temperature(self.h, ptr(t)))
+
+    def refresh_rates(self):
+        self._check(self.L.mskg_refresh_rates(self.h))
+
+    def run(self, max_events, max_time=float("inf")):
+        n = C.c_int64()
+        t = C.c_double()
+        s = C.c_int32()
+        mt = float(max_time) if np.isfinite(max_time) else 1e300
+        self._check(self.L.mskg_run(self.h, int(max_events), mt, C.byref(n), C.byref(t), C.byref(s)))
+        return n.value, t.value, s.value
+
+    def replay(self, log, clock_mode=0):
+        log = np.ascontiguousarray(log, EVENT_DTYPE)
+        self._check(self.L.mskg_replay(self.h, ptr(log), len(log), int(clock_mode)))
+
+    def store_state(self, lattcoo=None, capacity=None):
+        lx, ly, lz = self.box
+        lat = np.zeros((lz + 1, ly, lx), np.int32) if lattcoo is None else np.ascontiguousarray(lattcoo, np.int32)
+        na, nb, nv = C.c_int32(), C.c_int32(), C.c_int32()
+        z = None
+        self.L.mskg_store_state(self.h, None, 0, C.byref(na), z, z, z, z, 0, C.byref(nb), z, z, 0, C.byref(nv), z, z)
+        cap = capacity or (na.value, nb.value, nv.value)
+        mc = {"xyz": np.zeros((max(cap[0], 1), 3), np.int32), "nn": np.zeros((max(cap[0], 1), 4, 3), np.int32),
+              "event": np.zeros(max(cap[0], 1), np.int32), "prob": np.zeros(max(cap[0], 1), np.float64)}
+        bulk = {"xyz": np.zeros((max(cap[1], 1), 3), np.int32), "nn": np.zeros((max(cap[1], 1), 4, 3), np.int32)}
+        void = {"xyz": np.zeros((max(cap[2], 1), 3), np.int32), "nn": np.zeros((max(cap[2], 1), 4, 3), np.int32)}
+        self._check(self.L.mskg_store_state(
+            self.h, ptr(lat), cap[0], C.byref(na), ptr(mc["xyz"]), ptr(mc["nn"]), ptr(mc["event"]), ptr(mc["prob"]),
+            cap[1], C.byref(nb), ptr(bulk["xyz"]), ptr(bulk["nn"]), cap[2], C.byref(nv), ptr(void["xyz"]),
+            ptr(void["nn"])))
+        for d, n in ((mc, na.value), (bulk, nb.value), (void, nv.value)):
+            for k in list(d):
+                d[k] = d[k][:n]
+        return lat, mc, bulk, void
+
+    def site_rates(self, xyz):
+        a = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        cls = np.zeros(len(a), np.int32)
+        rate = np.zeros(len(a), np.float32)
+        self._check(self.L.mskg_site_rates(self.h, len(a), ptr(a), ptr(cls), ptr(rate)))
+        return cls, rate
+
+    def counters(self, lists=True):
+        c = np.zeros(8, np.int64)
+        t = C.c_double()
+        na, nb, nv = C.c_int64(), C.c_int64(), C.c_int64()
+        if lists:
+            self._check(self.L.mskg_counters(self.h, ptr(c), C.byref(t), C.byref(na), C.byref(nb), C.byref(nv)))
+            return c, t.value, (na.value, nb.value, nv.value)
+        self._check(self.L.mskg_counters(self.h, ptr(c), C.byref(t), None, None, None))
+        return c, t.value, None
+
+    def last_timing(self):
+        r, s = C.c_double(), C.c_double()
+        n, k = C.c_int64(), C.c_int64()
+        self._check(self.L.mskg_last_timing(self.h, C.byref(r), C.byref(s), C.byref(n), C.byref(k)))
+        return {"run_ms": r.value, "sweep_ms": s.value, "sweep_points": n.value, "sweeps": k.value}
+
+
+def find_nn(s1, s2, sc, lx, ly):
+    """The engine's FIND_NN geometry evaluated on the host (no GPU needed)."""
+    L = _lib.lib()
+    a, b, c = (np.ascontiguousarray(v, np.int32) for v in (s1, s2, sc))
+    out = np.zeros(6, np.int32)
+    L.mskg_geometry_find_nn(ptr(a), ptr(b), ptr(c), int(lx), int(ly), ptr(out))
+    return out.reshape(2, 3)
+
+
+def find_sn(site, sitec, sn, lx, ly):
+    """The engine's FIND_SN geometry evaluated on the host (no GPU needed)."""
+    L = _lib.lib()
+    a, c = np.ascontiguousarray(site, np.int32), np.ascontiguousarray(sitec, np.int32)
+    n = np.ascontiguousarray(sn, np.int32).reshape(9)
+    zig = np.zeros(9, np.int32)
+    arm = np.zeros(9, np.int32)
+    L.mskg_geometry_find_sn(ptr(a), ptr(c), ptr(n), int(lx), int(ly), ptr(zig), ptr(arm))
+    return zig.reshape(3, 3), arm.reshape(3, 3)

@@ -68,6 +68,7 @@ struct orc_sim {
   int32_t count_crystal[3], count_crystal_old[3], count_cov[3];
   int last_arm;
   int64_t oob_reads;
+  int64_t nonbond_arm, nonbond_nn;  /* geometry results that are not one of the 32 bond vectors */
   /* error handling: the reference STOPs; we unwind to the entry point */
   jmp_buf jb; int have_jb;
   char err[512];
@@ -380,6 +381,15 @@ void orc_find_sn(const int32_t site[3], const int32_t sitec_in[3], const int32_t
   }
 }
 
+static int is_bond_disp(const orc_sim *s, const int32_t a[3], const int32_t b[3]) {
+  int d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  if (d[0] > s->lx / 2) d[0] -= s->lx;
+  if (d[0] < -s->lx / 2) d[0] += s->lx;
+  if (d[1] > s->ly / 2) d[1] -= s->ly;
+  if (d[1] < -s->ly / 2) d[1] += s->ly;
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2] == 27;
+}
+
 /* IF_Arm_Choice.f:21-56 */
 static int if_arm_choice(orc_sim *s, int32_t zig[3][3], int32_t arm[3][3]) {
   int carm = 0, czig = 0;
@@ -630,7 +640,10 @@ static void deposition(orc_sim *s, int32_t ind_void, int iatom) {
     orc_find_sn(site, sitec, (const int32_t(*)[3])sn, zig, arm, s->lx, s->ly);
     int is_arm = if_arm_choice(s, zig, arm);
     s->last_arm = is_arm;
-    for (int i = 0; i < 3; i++) cp3(s->ad[ind_void].nn[i + 1], is_arm ? arm[i] : zig[i]);
+    for (int i = 0; i < 3; i++) {
+      cp3(s->ad[ind_void].nn[i + 1], is_arm ? arm[i] : zig[i]);
+      if (!is_bond_disp(s, site, is_arm ? arm[i] : zig[i])) s->nonbond_arm++;
+    }
   }
   memcpy(nn_site, s->ad[ind_void].nn, sizeof nn_site);
   for (int i = 0; i < 4; i++) { /* :78-235 */
@@ -663,6 +676,7 @@ static void deposition(orc_sim *s, int32_t ind_void, int iatom) {
           coornn = 2;
           cp3(nnn[1], site);
           orc_find_nn(nnn[0], nnn[1], p, ns, s->lx, s->ly);
+          if (!is_bond_disp(s, p, ns[0]) || !is_bond_disp(s, p, ns[1])) s->nonbond_nn++;
           for (int j = 2; j < 4; j++) {
             cp3(nnn[j], ns[j - 2]);
             int32_t w3 = LC(s, nnn[j]);
@@ -1578,7 +1592,7 @@ int orc_step(orc_sim *s, orc_event *ev) {
     ev->coor = ibits(LC(s, ev->site), POS_COOR, LEN_COOR);
   }
   do_execute(s, ind, itrans);
-  if (ev) { ev->arm = s->last_arm; ev->time = s->time; ev->pad = 0; }
+  if (ev) { ev->arm = s->last_arm; ev->time = s->time; }
   s->iter++;
   s->have_jb = 0;
   return 0;
@@ -1625,7 +1639,7 @@ int orc_main(orc_sim *s, const char *outdir, orc_event *log, int64_t log_cap, in
       ev->coor = ibits(LC(s, ev->site), POS_COOR, LEN_COOR);
     }
     do_execute(s, ind, itrans);
-    if (ev) { ev->arm = s->last_arm; ev->time = s->time; ev->pad = 0; nlog++; }
+    if (ev) { ev->arm = s->last_arm; ev->time = s->time; nlog++; }
     s->iter++;
     if (is_iter ? (s->iter > s->max_iter) : (s->time > s->max_time)) break;
   }
@@ -1673,6 +1687,12 @@ int orc_set_numberslist(orc_sim *s, const double *v, int n) {
   memcpy(s->numbers, v, sizeof(double) * (size_t)n);
   s->n_numbers = n;
   return 0;
+}
+
+void orc_anomalies(const orc_sim *s, int64_t out[3]) {
+  out[0] = s->nonbond_arm;
+  out[1] = s->nonbond_nn;
+  out[2] = s->oob_reads;
 }
 
 int32_t *orc_lattcoo(orc_sim *s) { return s->latt; }

@@ -53,7 +53,6 @@ typedef struct {
   int32_t arm;        /* deposition at Coor 1: 1 arm-chair, 0 zig-zag;
                          -1 when no stacking choice was made               */
   int32_t coor;       /* coordination of the site before the event        */
-  int32_t pad;
   double  rtot;       /* Tree(1) at pick time (total rate R)              */
   double  time;       /* clock after the event (Time += 1/R)              */
 } orc_event;
@@ -126,6 +125,11 @@ void orc_find_nn(const int32_t s1[3], const int32_t s2[3], const int32_t sc[3],
 void orc_find_sn(const int32_t site[3], const int32_t sitec[3],
                  const int32_t sn[3][3], int32_t zig[3][3], int32_t arm[3][3],
                  int lx, int ly);
+
+/* Geometry anomalies seen so far: [0] arm-chair neighbours placed at a
+ * displacement that is not one of the 32 bond vectors, [1] the same for
+ * FIND_NN results, [2] reads of LattCoo outside the box. */
+void orc_anomalies(const orc_sim *s, int64_t out[3]);
 
 /* ran3 (random.f:22-69) as a stand-alone generator for tests. */
 typedef struct { int32_t ma[56]; int32_t inext, inextp, iff; } orc_ran3;

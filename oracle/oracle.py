@@ -1,0 +1,220 @@
+"""ctypes wrapper of the C oracle (kmc_oracle.c) -- test infrastructure only."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libkmc_oracle.so")
+CLI = os.path.join(HERE, "kmc_oracle_cli")
+NUMBERSLIST = os.path.join(os.path.dirname(HERE), "tests", "golden", "numberslist.txt")
+
+EVENT_DTYPE = np.dtype([("iter", "<i8"), ("site", "<i4", (3,)), ("itrans", "<i4"), ("arm", "<i4"),
+                        ("coor", "<i4"), ("rtot", "<f8"), ("time", "<f8")])
+assert EVENT_DTYPE.itemsize == 48
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def numberslist():
+    return np.loadtxt(NUMBERSLIST, dtype=np.float64)
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P, I, D, I64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [I, I, I]
+        L.orc_destroy.argtypes = [P]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_last_error.argtypes = [P]
+        L.orc_set_numberslist.argtypes = [P, P, I]
+        L.orc_load_deck.argtypes = [P, C.c_char_p]
+        L.orc_main.argtypes = [P, C.c_char_p, P, I64, P, P]
+        L.orc_step.argtypes = [P, P]
+        L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
+        L.orc_lattcoo.restype = C.POINTER(C.c_int32)
+        L.orc_lattcoo.argtypes = [P]
+        L.orc_dims.argtypes = [P, P, P, P]
+        L.orc_counts.argtypes = [P, P, P, P]
+        L.orc_get_list.argtypes = [P, I, P, P, P, P]
+        L.orc_time.restype = D
+        L.orc_time.argtypes = [P]
+        L.orc_total_rate.restype = D
+        L.orc_total_rate.argtypes = [P]
+        L.orc_iter.restype = I64
+        L.orc_iter.argtypes = [P]
+        L.orc_count_crystal.argtypes = [P, P, P]
+        L.orc_ncrystal.argtypes = [P, P]
+        L.orc_tables.argtypes = [P, P, P, P, P, P, P, P]
+        L.orc_pure_rate.argtypes = [P, P, P, P]
+        L.orc_find_nn.argtypes = [P, P, P, P, I, I]
+        L.orc_find_sn.argtypes = [P, P, P, P, P, I, I]
+        L.orc_anomalies.argtypes = [P, P]
+        L.orc_ran3_next.restype = D
+        L.orc_ran3_next.argtypes = [P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """One reference-faithful serial KMC simulation on the CPU."""
+
+    def __init__(self, lx, ly, lz):
+        self.L = _load()
+        self.h = self.L.orc_create(lx, ly, lz)
+        if not self.h:
+            raise MemoryError("orc_create failed")
+        self.dims = (lx, ly, lz)
+        nl = numberslist()
+        self.L.orc_set_numberslist(self.h, _p(nl), len(nl))
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what}: {self.L.orc_last_error(self.h).decode()}")
+
+    def load_deck(self, path):
+        self._check(self.L.orc_load_deck(self.h, os.fsencode(path)), "orc_load_deck")
+
+    def run_main(self, outdir=None, log_cap=0):
+        log = np.zeros(max(log_cap, 1), EVENT_DTYPE)
+        n = C.c_int64(0)
+        stop = C.c_int(0)
+        rc = self.L.orc_main(self.h, os.fsencode(outdir) if outdir else None, _p(log) if log_cap else None,
+                             log_cap, C.byref(n), C.byref(stop))
+        self._check(rc, "orc_main")
+        return log[:n.value] if log_cap else None, stop.value
+
+    def step(self, n, record=True):
+        """Run n iterations of the KMC loop; returns the event log."""
+        log = np.zeros(n, EVENT_DTYPE)
+        done = 0
+        for k in range(n):
+            rc = self.L.orc_step(self.h, _p(log[k:k + 1]) if record else None)
+            if rc == 1:
+                break
+            self._check(rc, "orc_step")
+            done += 1
+        return log[:done]
+
+    def lattcoo(self):
+        lx, ly, lz = self.dims
+        ptr = self.L.orc_lattcoo(self.h)
+        return np.ctypeslib.as_array(ptr, shape=(lz + 1, ly, lx)).copy()
+
+    def counts(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self.L.orc_counts(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def lists(self):
+        """(mc, bulk, void) as dicts of numpy arrays, in list order."""
+        na, nb, nv = self.counts()
+        out = []
+        for which, n in ((0, na), (1, nb), (2, nv)):
+            xyz = np.zeros((max(n, 1), 3), np.int32)
+            nn = np.zeros((max(n, 1), 4, 3), np.int32)
+            ev = np.zeros(max(n, 1), np.int32)
+            pr = np.zeros(max(n, 1), np.float64)
+            self.L.orc_get_list(self.h, which, _p(xyz), _p(nn), _p(ev), _p(pr))
+            d = {"xyz": xyz[:n], "nn": nn[:n]}
+            if which == 0:
+                d["event"] = ev[:n]
+                d["prob"] = pr[:n]
+            out.append(d)
+        return out
+
+    def anomalies(self):
+        a = np.zeros(3, np.int64)
+        self.L.orc_anomalies(self.h, _p(a))
+        return {"nonbond_arm": int(a[0]), "nonbond_nn": int(a[1]), "oob_reads": int(a[2])}
+
+    def time(self):
+        return self.L.orc_time(self.h)
+
+    def total_rate(self):
+        return self.L.orc_total_rate(self.h)
+
+    def iteration(self):
+        return self.L.orc_iter(self.h)
+
+    def count_crystal(self):
+        cc = np.zeros(3, np.int32)
+        cv = np.zeros(3, np.int32)
+        self.L.orc_count_crystal(self.h, _p(cc), _p(cv))
+        return cc, cv
+
+    def tables(self):
+        pe = np.zeros(3 * 4096)
+        pd = np.zeros(9)
+        pabs = np.zeros(12)
+        pdes = np.zeros(12)
+        pz = C.c_double()
+        lc = np.zeros(3, np.int32)
+        lcov = np.zeros(3, np.int32)
+        self.L.orc_tables(self.h, _p(pe), _p(pd), _p(pabs), _p(pdes), C.byref(pz), _p(lc), _p(lcov))
+        ncov = C.c_int()
+        ncr = self.L.orc_ncrystal(self.h, C.byref(ncov))
+        return dict(ptrans_e=pe, ptrans_d=pd, ptrans_abs=pabs, ptrans_des=pdes, ptrans_zig=pz.value,
+                    list_crystal=lc, list_cov=lcov, n_crystal=ncr, n_cov=ncov.value)
+
+    def pure_rate(self, xyz):
+        cat = C.c_int32()
+        r = C.c_double()
+        a = np.asarray(xyz, np.int32)
+        self._check(self.L.orc_pure_rate(self.h, _p(a), C.byref(cat), C.byref(r)), "orc_pure_rate")
+        return cat.value, r.value
+
+    def write_frame(self, xyz_path, d_path=None, it=0):
+        self._check(self.L.orc_write_frame(self.h, os.fsencode(xyz_path), os.fsencode(d_path) if d_path else None,
+                                           it), "orc_write_frame")
+
+
+def find_nn(s1, s2, sc, lx, ly):
+    L = _load()
+    a, b, c = (np.asarray(v, np.int32) for v in (s1, s2, sc))
+    out = np.zeros((2, 3), np.int32)
+    L.orc_find_nn(_p(a), _p(b), _p(c), _p(out), lx, ly)
+    return out
+
+
+def find_sn(site, sitec, sn, lx, ly):
+    L = _load()
+    a, b = np.asarray(site, np.int32), np.asarray(sitec, np.int32)
+    s = np.ascontiguousarray(sn, np.int32)
+    zig = np.zeros((3, 3), np.int32)
+    arm = np.zeros((3, 3), np.int32)
+    L.orc_find_sn(_p(a), _p(b), _p(s), _p(zig), _p(arm), lx, ly)
+    return zig, arm
+
+
+def ran3_sequence(idum, n):
+    L = _load()
+    st = np.zeros(59, np.int32)  # orc_ran3: ma[56], inext, inextp, iff
+    d = C.c_int32(idum)
+    return np.array([L.orc_ran3_next(_p(st), C.byref(d)) for _ in range(n)])

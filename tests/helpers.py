@@ -1,0 +1,61 @@
+"""Shared test helpers: small decks, oracle <-> engine state plumbing."""
+import os
+
+import numpy as np
+
+from mulskips_amd import deck as deckmod
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+REGRESSION = os.path.join(GOLDEN, "regression")
+DECKS = ["cube", "cube-zigzag", "sphere", "surface", "inverted-pyramid", "finfet", "Si-SL", "CVD-Si-SiH2"]
+
+
+def regression_deck(name):
+    return os.path.join(REGRESSION, f"start-{name}.dat")
+
+
+def small_deck(tmpdir, base, lens, run_type="R", idum=9117116, max_iter=10 ** 9, ptrans_zig=None):
+    """A regression deck with its shape lengths changed so it fits a small box."""
+    d = deckmod.read_start_dat(regression_deck(base))
+    d.lens = list(lens)
+    d.run_type = run_type
+    d.idum = idum
+    if d.exit_strategy == "Time":
+        d.exit_strategy, d.max_iter, d.out_molmol = "Iter", max_iter, max_iter
+    else:
+        d.max_iter = d.out_molmol = max_iter
+    if ptrans_zig is not None:
+        d.ptrans_zig = ptrans_zig
+    path = os.path.join(str(tmpdir), f"start-{base}-small.dat")
+    deckmod.write_start_dat(d, path)
+    return path, d
+
+
+# (base deck, box, lens): small versions of the regression shapes
+SMALL = {
+    "cube": ("cube", (144, 144, 144), (72, 72, 72)),
+    "cube-zigzag": ("cube-zigzag", (144, 144, 144), (72, 72, 72)),
+    "sphere": ("sphere", (144, 144, 144), (96, 96, 96)),
+    "surface": ("surface", (96, 96, 192), (72, 72, 72)),
+    "inverted-pyramid": ("inverted-pyramid", (144, 144, 144), (48, 60, 60)),
+    "Si-SL": ("Si-SL", (96, 96, 192), (96, 120, 120)),
+    "finfet": ("finfet", (96, 96, 240), (72, 96, 168, 48)),
+    "CVD-Si-SiH2": ("CVD-Si-SiH2", (96, 96, 192), (72, 120, 120)),
+}
+
+
+def oracle_state(o):
+    mc, bulk, void = o.lists()
+    return o.lattcoo(), mc, bulk, void
+
+
+def parity_mask(box):
+    lx, ly, lz = box
+    z, y, x = np.meshgrid(np.arange(lz + 1), np.arange(ly), np.arange(lx), indexing="ij")
+    return (((x ^ y) & 1) == 0) & (((x ^ z) & 1) == 0)
+
+
+def list_map(d):
+    """{site tuple: neighbour tuple} of a list dict."""
+    return {tuple(int(v) for v in s): tuple(int(v) for v in n.reshape(-1)) for s, n in zip(d["xyz"], d["nn"])}
