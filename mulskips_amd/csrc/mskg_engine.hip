@@ -267,7 +267,9 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
   ctr.z = (uint32_t)(st->sweeps >> 32) ^ ((uint32_t)P.colour << 29);
   ctr.w = 0;
   int err = 0;
-  for (;;) {
+  // every wave ends its pass: the window tau bounds the expected number of
+  // events, and this cap bounds the worst case
+  for (int guard = 0; guard < (1 << 16); guard++) {
     double part = 0.0;
     for (int j = lane; j < n; j += WAVE) part += (double)lr[j];
     const double R = wave_sum(part);
