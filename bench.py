@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark: KMC events/s of the MI355X engine on one GPU.
+
+Workload (the large single-replica SiC configuration): a 3C-SiC (001)
+substrate of 40 monolayers (z <= 120 grid units, frozen) in a
+3072 x 3072 x 3072 super-lattice grid, i.e. 512^3 diamond sites, Si and C on
+alternating fcc sublattices; evaporation 1e13*exp(-E/kT) with E the sum of
+Si-Si 0.6 / Si-C 1.1 / C-C 0.9 eV pair energies over the reciprocal first
+neighbours; Si and C deposition at 2 and 1.8 ML/s; T = 1600 K.  The lattice
+is built on the device, rates live in HBM as FP32, the clock and rate sums
+are FP64.  A step is one mskg_run of --events-per-step events through the
+synchronous-sublattice engine (8 colours, 24^3-grid-unit tiles, window
+policy 1: one event per active tile and pass on average).
+
+The JSON line also carries
+* roofline: the rate-update sweep kernel (Get_Prob over every lattice point,
+  21 algorithmic bytes per point: the 16-byte site record read, the FP32
+  rate and the class byte written), timed with HIP events on the engine's
+  stream, against the MI355X HBM peak of 8 TB/s;
+* cpu_baseline: the CPU oracle (the reference engine restated in C, serial,
+  1 core) on the same model at a 480 x 480 x 240 grid for a bounded time.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+SWEEP_BYTES_PER_POINT = 21
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--events-per-step", type=float, default=2e7)
+    ap.add_argument("--box", type=int, nargs=3, default=[3072, 3072, 3072])
+    ap.add_argument("--substrate-ml", type=int, default=40)
+    ap.add_argument("--tile", type=int, default=24)
+    ap.add_argument("--tau-policy", type=int, default=1)
+    ap.add_argument("--events-per-pass", type=float, default=1.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(tables, substrate_len, seconds):
+    """The oracle (C restatement of the reference engine, serial, one core) on
+    the same model and substrate at a 480 x 480 x 240 grid, for about
+    `seconds` of wall time after its initialisation."""
+    import tempfile
+
+    from mulskips_amd import deck as deckmod
+    from oracle.oracle import Oracle
+    box = (480, 480, 240)
+    with tempfile.TemporaryDirectory() as tmp:
+        d = deckmod.deck_from_tables(tables, "F", [substrate_len, 120, 120])
+        path = os.path.join(tmp, "start.dat")
+        deckmod.write_start_dat(d, path)
+        o = Oracle(*box)
+        o.load_deck(path)
+        o.set_frozen(substrate_len + 1)
+        o.step(20000, record=False)  # leave the initial transient
+        n = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            k = len(o.step(100000, record=False))
+            n += k
+            if k < 100000:
+                break
+        dt = time.perf_counter() - t0
+        o.close()
+    return {"value": n / dt, "unit": "events/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (reference engine restated in C, serial BKL, no 98 KB table copy), same SiC model "
+                      f"and frozen 40-ML substrate on a 480x480x240 grid, {n} events in {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported (this engine is a single-GPU build; "
+              "see DESIGN.md for how the lattice would shard)", file=sys.stderr)
+        return 2
+    import torch  # device selection and synchronisation only
+
+    from mulskips_amd import models
+    from mulskips_amd.engine import Engine
+    torch.cuda.init()
+    box = tuple(a.box)
+    substrate = a.substrate_ml * 3 - (a.substrate_ml * 3) % 24
+    tables = models.sic_c2_tables()
+    eng = Engine(box, tables, seed=20261202, tile=a.tile, events_per_pass=a.events_per_pass,
+                 tau_policy=a.tau_policy, frozen_below_z=substrate + 1)
+    eng.build_slab(1, substrate)
+    eps = int(a.events_per_step)
+    for _ in range(a.warmup):
+        eng.run(eps)
+    torch.cuda.synchronize()
+    total = 0
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        n, t_kmc, stop = eng.run(eps)
+        total += n
+        dev_ms += eng.last_timing()["run_ms"]
+        if stop:
+            break
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    counts, t_kmc, _ = eng.counters(lists=False)
+    # the rate-update sweep, timed with HIP events on the engine's stream
+    sweeps = []
+    for _ in range(5):
+        eng.refresh_rates()
+        tm = eng.last_timing()
+        sweeps.append(tm["sweep_ms"])
+    sweep_ms = sorted(sweeps)[len(sweeps) // 2]
+    npts = tm["sweep_points"]
+    achieved = npts * SWEEP_BYTES_PER_POINT / (sweep_ms * 1e-3) / 1e9
+    out = {
+        "metric": "KMC events/s at 512^3 SiC (1 GPU)",
+        "value": total / wall,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 rates, fp64 clock/sums, int8 sites",
+        "data": "synthetic (device-built SiC slab)",
+        "config": {"workload": "3C-SiC (001) growth, 512^3 diamond sites (3072^3 super-lattice grid), "
+                               "40-ML frozen substrate, pair energies Si-Si 0.6 / Si-C 1.1 / C-C 0.9 eV, "
+                               "fluxes 2 / 1.8 ML/s, T=1600 K",
+                   "box": list(box), "tile": a.tile, "colours": 8, "tau_policy": a.tau_policy,
+                   "events_per_pass": a.events_per_pass, "events_per_step": eps},
+        "roofline": {"kernel": "k_rate_sweep (rate-update sweep)", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "points": npts, "bytes_per_point": SWEEP_BYTES_PER_POINT, "ms": sweep_ms},
+        "engine": {"device_ms": dev_ms, "events": total, "kmc_time_s": t_kmc,
+                   "counts": [int(c) for c in counts]},
+    }
+    eng.close()
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(tables, substrate, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -62,8 +62,14 @@ typedef struct {
   int32_t tile;             /* synchronous-sublattice tile edge (grid units, even,
                                >=16, len_x/tile and len_y/tile even); 0 = choose  */
   int32_t tile_capacity;    /* max MC particles per tile held on chip; 0 = 1024   */
-  double events_per_pass;   /* window tau = this / (largest tile rate); 0 = 1.0   */
-  int32_t reserved[8];
+  double events_per_pass;   /* k of the window policy below; 0 = 1.0                */
+  int32_t tau_policy;       /* 0: tau = k / (largest tile rate), every tile does at most
+                               about k events per colour pass; 1: tau = k * (tiles with MC
+                               particles) / (total rate), k events per tile on average */
+  int32_t frozen_below_z;   /* engine option without a reference counterpart: atoms with
+                               z < frozen_below_z never evaporate (a frozen substrate);
+                               0 = off, the reference behaviour */
+  int32_t reserved[6];
 } mskg_config;
 
 /* One logged event, as written by the reference loop instrumented at
@@ -98,6 +104,15 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt_coo,
                     int32_t n_void, const int32_t *void_xyz, const int32_t *void_nn,
                     double kmc_time);
 
+/* Device-side initial state, without a host LattCoo: the flat (001) slab
+ * of initial/SetSi.f:25-220 (mode 0: Si on both fcc sublattices, wall
+ * layers z=0..5 and len_z-6..len_z-1) or initial/SetSiC3C.f:25-198
+ * (mode 1: Si on sublattice A, C on B, no walls) of height len (reduced to
+ * a multiple of 24 by the caller, as main.f:239,248 do).  The result equals
+ * those builders followed by mskg_load_state, except that the species of an
+ * empty site is drawn when it is filled rather than when the slab is built. */
+int mskg_build_slab(mskg_engine *e, int32_t mode, int32_t len);
+
 /* Per-site temperature in K for laser-annealing mode, (len_x, len_y,
  * len_z+1) like LattCoo's T bits (SetT.f:25-50). */
 int mskg_set_temperature(mskg_engine *e, const int32_t *temp_k);
@@ -111,6 +126,14 @@ int mskg_refresh_rates(mskg_engine *e);
  * first; the check is made between sweeps).  Replaces main.f:304-371. */
 int mskg_run(mskg_engine *e, int64_t max_events, double max_time,
              int64_t *events_done, double *kmc_time, int32_t *stop_reason);
+
+/* Record every event mskg_run executes (capacity events; 0 switches the log
+ * off).  The order is a valid serial order of the parallel run: events of
+ * one tile are in execution order, and events of different tiles of the same
+ * colour pass touch disjoint sites, so they commute.  mskg_get_event_log
+ * copies up to cap events and reports how many were recorded. */
+int mskg_set_event_log(mskg_engine *e, int64_t capacity);
+int mskg_get_event_log(mskg_engine *e, mskg_event *out, int64_t cap, int64_t *n);
 
 /* Applies a logged event sequence in order with the reference's update rules
  * (no random numbers are drawn: species and stacking come from the log).

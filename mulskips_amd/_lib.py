@@ -17,6 +17,7 @@ EXPORTS = [
     "mskg_create", "mskg_destroy", "mskg_last_error", "mskg_load_state", "mskg_set_temperature",
     "mskg_refresh_rates", "mskg_run", "mskg_replay", "mskg_store_state", "mskg_site_rates",
     "mskg_counters", "mskg_last_timing", "mskg_geometry_find_nn", "mskg_geometry_find_sn",
+    "mskg_build_slab", "mskg_set_event_log", "mskg_get_event_log",
 ]
 
 
@@ -30,7 +31,8 @@ class Config(C.Structure):
         ("ptrans_des", C.c_void_p), ("ptrans_zig", C.c_double),
         ("la_mode", C.c_int32), ("tm", C.c_double), ("p0", C.c_double), ("sigma", C.c_double),
         ("tflex", C.c_double), ("seed", C.c_uint64), ("tile", C.c_int32), ("tile_capacity", C.c_int32),
-        ("events_per_pass", C.c_double), ("reserved", C.c_int32 * 8),
+        ("events_per_pass", C.c_double), ("tau_policy", C.c_int32),
+        ("frozen_below_z", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -65,6 +67,9 @@ def lib():
     L.mskg_site_rates.argtypes = [P, I64, P, P, P]
     L.mskg_counters.argtypes = [P, P, P, P, P, P]
     L.mskg_last_timing.argtypes = [P, P, P, P, P]
+    L.mskg_build_slab.argtypes = [P, I32, I32]
+    L.mskg_set_event_log.argtypes = [P, I64]
+    L.mskg_get_event_log.argtypes = [P, P, I64, P]
     L.mskg_geometry_find_nn.argtypes = [P, P, P, I32, I32, P]
     L.mskg_geometry_find_sn.argtypes = [P, P, P, I32, I32, P, P]
     _lib = L

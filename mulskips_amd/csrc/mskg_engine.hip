@@ -6,7 +6,8 @@
 //                  packed words + neighbour gathers); the rate-update sweep.
 //   k_replay       applies a logged event sequence with the reference's
 //                  update rules (one wave; events are strictly ordered).
-//   k_tile_scan    per-tile rate totals and active flags (start of a run).
+//   k_build_lists  per-tile candidate lists from the classes (after an upload,
+//                  a rate sweep or a replay); k_tile_rates: their rate totals.
 //   k_pass         one colour of the synchronous-sublattice sweep: each
 //                  workgroup (one 64-wide wave) runs rejection-free KMC inside
 //                  one tile for the window tau, selecting events with a
@@ -34,6 +35,15 @@ namespace {
 
 constexpr int WAVE = 64;
 
+// Per-sweep accumulators, striped over slots so that thousands of tile
+// waves do not serialise on one address; k_sweep_end folds them.
+constexpr int NSLOT = 128;
+struct Slot {
+  unsigned long long rmax;  // bit pattern of the largest (non-negative) tile rate
+  double rsum;
+  unsigned long long n, tiles, events, counts[8];
+};
+
 // Device-resident run state (one per engine).
 struct RunState {
   double tau;            // current window
@@ -41,9 +51,14 @@ struct RunState {
   double rtot;           // total rate estimate (isolated-atom rule)
   double rmax;           // largest tile rate of the last sweep
   double events_per_pass;
+  int tau_policy;        // 0: tau = k / max tile rate; 1: tau = k * active tiles / total rate
   unsigned long long events;
   unsigned long long counts[8];
-  unsigned long long sweeps;
+  unsigned long long sweeps;      // sweep counter (RNG stream position)
+  unsigned long long sweeps_run;  // sweeps that advanced the clock
+  long long n_mc;                 // MC particles at the end of the last sweep
+  unsigned long long log_n;
+  Slot slot[NSLOT];
   int err;
   int err_site[3];
   int stop;
@@ -52,9 +67,8 @@ struct RunState {
 struct PassParams {
   Lattice L;
   RunState *st;
-  double *tile_r;
-  const int32_t *tiles;  // tiles of this colour
-  int ntiles;
+  mskg_event *log;          // optional event log (valid serialisation of the run)
+  unsigned long long log_cap;
   int ntz;
   int cap;
   uint32_t key0, key1;
@@ -73,6 +87,24 @@ __device__ inline double wave_incl_scan(double v, int lane) {
     if (lane >= off) v += t;
   }
   return v;
+}
+
+__device__ inline void slot_add(RunState *st, int b, double r, unsigned long long n, unsigned long long nev,
+                                const unsigned long long *cnt) {
+  Slot &sl = st->slot[b & (NSLOT - 1)];
+  if (n) {
+    const unsigned long long rb = (unsigned long long)__double_as_longlong(r);
+    if (rb > sl.rmax) atomicMax(&sl.rmax, rb);
+    atomicAdd(&sl.rsum, r);
+    atomicAdd(&sl.n, n);
+    atomicAdd(&sl.tiles, 1ull);
+  }
+  if (nev) {
+    atomicAdd(&sl.events, nev);
+    if (cnt)
+      for (int q = 0; q < 8; q++)
+        if (cnt[q]) atomicAdd(&sl.counts[q], cnt[q]);
+  }
 }
 
 struct TileGeom {
@@ -137,15 +169,21 @@ __global__ void __launch_bounds__(256) k_rate_sweep(Lattice L, double rtot, int 
 // ---- event-log replay -------------------------------------------------------
 __global__ void __launch_bounds__(WAVE) k_replay(Lattice L, const mskg_event *log, int64_t n, int clock_mode,
                                                  RunState *st) {
-  if (threadIdx.x != 0) return;
+  __shared__ EvScratch sc;
+  __shared__ int s_bad;
+  const int lane = threadIdx.x;
   double clock = st->clock;
   for (int64_t k = 0; k < n; k++) {
     const mskg_event ev = log[k];
-    P3 s = at(L, ev.site[0], ev.site[1], ev.site[2]);
-    if (s.i < 0 || L.cls[s.i] == 0) {
-      st->err = ERR_NOT_MC;
-      st->err_site[0] = s.x; st->err_site[1] = s.y; st->err_site[2] = s.z;
-      st->events = (unsigned long long)k;
+    const P3 s = at(L, ev.site[0], ev.site[1], ev.site[2]);
+    if (lane == 0) s_bad = (s.i < 0 || L.cls[s.i] == 0) ? 1 : 0;
+    __syncthreads();
+    if (s_bad) {
+      if (lane == 0) {
+        st->err = ERR_NOT_MC;
+        st->err_site[0] = s.x; st->err_site[1] = s.y; st->err_site[2] = s.z;
+        st->events = (unsigned long long)k;
+      }
       return;
     }
     if (clock_mode == 0) clock = clock + 1.0 / ev.rtot;
@@ -153,95 +191,147 @@ __global__ void __launch_bounds__(WAVE) k_replay(Lattice L, const mskg_event *lo
     ch.arm = ev.arm;
     ch.u_arm = 0.0;
     EvResult res;
-    execute(L, ev.rtot, s, ev.itrans, ch, res);
-    if (res.err) {
-      st->err = res.err;
-      st->err_site[0] = s.x; st->err_site[1] = s.y; st->err_site[2] = s.z;
-      st->events = (unsigned long long)k;
+    execute(L, ev.rtot, s, ev.itrans, ch, res, sc, lane);
+    if (lane == 0) s_bad = res.err;
+    __syncthreads();
+    if (s_bad) {
+      if (lane == 0) {
+        st->err = res.err;
+        st->err_site[0] = s.x; st->err_site[1] = s.y; st->err_site[2] = s.z;
+        st->events = (unsigned long long)k;
+      }
       return;
     }
-    int slot = ev.itrans <= 3 ? ev.itrans - 1 : (ev.itrans == 4 ? 3 : (ev.itrans == 5 ? 4 : ev.itrans - 1));
-    if (slot >= 0 && slot < 8) st->counts[slot]++;
+    if (lane == 0) {
+      int slot = ev.itrans <= 3 ? ev.itrans - 1 : (ev.itrans == 4 ? 3 : (ev.itrans == 5 ? 4 : ev.itrans - 1));
+      if (slot >= 0 && slot < 8) st->counts[slot]++;
+    }
   }
-  st->events += (unsigned long long)n;
-  st->clock = clock;
+  if (lane == 0) {
+    st->events += (unsigned long long)n;
+    st->clock = clock;
+  }
 }
 
 // ---- synchronous sublattice -------------------------------------------------
-__global__ void __launch_bounds__(256) k_tile_scan(Lattice L, int ntiles, int ntz, double *tile_r) {
-  // one wave per tile
+// Build every tile's candidate list from the classes (after a state upload,
+// a rate sweep or a replay).  One wave per tile.
+__global__ void __launch_bounds__(256) k_build_lists(Lattice L, int ntiles) {
   int lane = threadIdx.x & (WAVE - 1);
   int t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   if (t >= ntiles) return;
   TileGeom g = tile_geom(t, L.tile, L.ntx, L.nty, L.box.lz);
-  double acc = 0.0;
-  int any = 0;
-  for (int k = lane; k < g.npt; k += WAVE) {
-    P3 p = tile_point(L.box, g, L.tile, k);
-    uint8_t c = L.cls[p.i];
-    if (c) {
-      any = 1;
-      if (c != CLS_ISOLATED) acc += (double)L.rate[p.i];
+  int n = 0;
+  for (int k0 = 0; k0 < g.npt; k0 += WAVE) {
+    int k = k0 + lane;
+    bool on = false;
+    if (k < g.npt) {
+      P3 p = tile_point(L.box, g, L.tile, k);
+      on = L.cls[p.i] != 0;
+      L.inlist[p.i] = on ? 1 : 0;
     }
+    unsigned long long m = __ballot(on);
+    int slot = n + __popcll(m & ((1ull << lane) - 1ull));
+    if (on) {
+      if (slot < L.capg) L.tl_idx[(int64_t)t * L.capg + slot] = k;
+      else *L.list_err = ERR_TILE_LIST;
+    }
+    n += __popcll(m);
+  }
+  if (lane == 0) {
+    L.tl_n[t] = n;
+    if (n) widen_window(L, tile_colour(g.tx, g.ty, g.tz), g.tz);
+  }
+}
+
+// Window of tile layers from lists that are already built.
+__global__ void __launch_bounds__(256) k_list_window(Lattice L, int ntiles) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  if (L.tl_n[t] > 0) {
+    const int tx = t % L.ntx, ty = (t / L.ntx) % L.nty, tz = t / (L.ntx * L.nty);
+    widen_window(L, tile_colour(tx, ty, tz), tz);
+  }
+}
+
+// Rate total (without isolated atoms) and population of every tile's list.
+__global__ void __launch_bounds__(256) k_tile_rates(Lattice L, int ntiles, RunState *st) {
+  int lane = threadIdx.x & (WAVE - 1);
+  int t = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+  if (t >= ntiles) return;
+  TileGeom g = tile_geom(t, L.tile, L.ntx, L.nty, L.box.lz);
+  const int m = min(L.tl_n[t], L.capg);
+  double acc = 0.0;
+  int cnt = 0;
+  for (int j = lane; j < m; j += WAVE) {
+    P3 p = tile_point(L.box, g, L.tile, L.tl_idx[(int64_t)t * L.capg + j]);
+    uint8_t c = L.cls[p.i];
+    if (c) cnt++;
+    if (c && c != CLS_ISOLATED) acc += (double)L.rate[p.i];
   }
   acc = wave_sum(acc);
-  any = __any(any);
-  if (lane == 0) {
-    tile_r[t] = acc;
-    L.tile_active[t] = any ? 1 : 0;
-  }
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, WAVE);
+  if (lane == 0 && cnt) slot_add(st, t, acc, (unsigned long long)cnt, 0, nullptr);
 }
 
 __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
   extern __shared__ unsigned char smem[];
   const int lane = threadIdx.x;
-  const Lattice &L = P.L;
+  Lattice L = P.L;
   const int cap = P.cap;
   int32_t *lk = (int32_t *)smem;                  // tile-local point offsets
   float *lr = (float *)(lk + cap);                // rates
   uint8_t *lc = (uint8_t *)(lr + cap);            // classes
-  __shared__ int s_n, s_sel, s_stop, s_nt;
+  __shared__ int s_n, s_stop, s_nt;
   __shared__ int s_tk[5];
   __shared__ float s_tr[5];
   __shared__ uint8_t s_tc[5];
+  __shared__ EvScratch sc;
 
-  if (blockIdx.x >= P.ntiles) return;
-  const int t = P.tiles[blockIdx.x];
   RunState *st = P.st;
   const double tau = st->tau;
-  if (!(tau > 0.0) || st->err) return;
-  if (!L.tile_active[t]) {
-    if (lane == 0) P.tile_r[t] = 0.0;
-    return;
-  }
+  if (!(tau > 0.0) || st->err || st->stop) return;
+  // tiles of this colour inside the window of active tile layers
+  const int c0 = P.colour & 1, c1 = (P.colour >> 1) & 1, c2 = (P.colour >> 2) & 1;
+  const int zlo = max(L.win[4 * P.colour], 0), zhi = min(L.win[4 * P.colour + 1], P.ntz - 1);
+  if (zlo > zhi) return;  // no tile of this colour holds candidates
+  const int zfirst = zlo + ((c2 - zlo) & 1);
+  const int nlayers = zfirst <= zhi ? (zhi - zfirst) / 2 + 1 : 0;
+  const int hx = L.ntx / 2, per_layer = hx * (L.nty / 2);
+  const int nwin = nlayers * per_layer;
+  for (int wi = blockIdx.x; wi < nwin; wi += gridDim.x) {
+  const int layer = wi / per_layer, rr = wi - layer * per_layer;
+  const int t = ((zfirst + 2 * layer) * L.nty + 2 * (rr / hx) + c1) * L.ntx + 2 * (rr % hx) + c0;
+  const int n_in = L.tl_n[t];
+  if (n_in == 0) continue;
+  L.cur_tile = t;
   TileGeom g = tile_geom(t, L.tile, L.ntx, L.nty, L.box.lz);
 
-  // 1. collect the tile's MC particles into LDS (ballot + prefix compaction)
+  // 1. the tile's candidate list -> LDS, dropping entries no longer active
+  const int m_in = min(n_in, L.capg);
   int n = 0;
-  bool overflow = false;
-  for (int k0 = 0; k0 < g.npt; k0 += WAVE) {
-    int k = k0 + lane;
+  for (int j0 = 0; j0 < m_in; j0 += WAVE) {
+    int j = j0 + lane;
     uint8_t c = 0;
     float r = 0.0f;
-    if (k < g.npt) {
+    int k = 0;
+    if (j < m_in) {
+      k = L.tl_idx[(int64_t)t * L.capg + j];
       P3 p = tile_point(L.box, g, L.tile, k);
       c = L.cls[p.i];
       if (c) r = L.rate[p.i];
+      else L.inlist[p.i] = 0;
     }
     unsigned long long m = __ballot(c != 0);
-    int pre = __popcll(m & ((1ull << lane) - 1ull));
-    if (c) {
-      int slot = n + pre;
-      if (slot < cap) {
-        lk[slot] = k;
-        lr[slot] = r;
-        lc[slot] = c;
-      }
+    int slot = n + __popcll(m & ((1ull << lane) - 1ull));
+    if (c && slot < cap) {
+      lk[slot] = k;
+      lr[slot] = r;
+      lc[slot] = c;
     }
     n += __popcll(m);
-    if (n > cap) { overflow = true; break; }
   }
-  if (overflow) {
+  if (n > cap) {
     if (lane == 0) {
       st->err = ERR_CAPACITY;
       P3 p = tile_point(L.box, g, L.tile, 0);
@@ -249,13 +339,7 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
     }
     return;
   }
-  if (n == 0) {
-    if (lane == 0) {
-      L.tile_active[t] = 0;
-      P.tile_r[t] = 0.0;
-    }
-    return;
-  }
+  __syncthreads();
 
   // 2. rejection-free KMC inside the tile for the window tau
   const double rtot = st->rtot;
@@ -299,30 +383,48 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
     }
     if (sel < 0) sel = last_nz;  // rounding at the top end of the scan
     if (sel < 0) break;
-    // execute on one lane
-    if (lane == 0) {
-      P3 s = tile_point(L.box, g, L.tile, lk[sel]);
-      U4 d2 = philox4x32_10(ctr, P.key0, P.key1);
+    {
+      const P3 s = tile_point(L.box, g, L.tile, lk[sel]);
+      const U4 d2 = philox4x32_10(ctr, P.key0, P.key1);
       Choice ch;
       ch.arm = -1;
       ch.u_arm = u01_open(d2.z, d2.w);
-      int itrans = pick_itrans(L, s, lc[sel], u01_open(d2.x, d2.y));
+      const int itrans = pick_itrans(L, s, lc[sel], u01_open(d2.x, d2.y));
+      const int coor0 = s_coor(L.site[s.i]);
+      __syncthreads();  // every lane has read the site before it changes
       EvResult res;
-      execute(L, rtot, s, itrans, ch, res);
-      err = res.err;
-      int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
-      if (slot >= 0 && slot < 8) cnt[slot]++;
-      int nt = 0;
-      for (int q = 0; q < res.ntouched; q++) {
-        int kl = tile_local(g, L.tile, res.touched[q]);
-        if (kl < 0) continue;
-        s_tk[nt] = kl;
-        s_tr[nt] = L.rate[res.touched[q].i];
-        s_tc[nt] = L.cls[res.touched[q].i];
-        nt++;
+      execute(L, rtot, s, itrans, ch, res, sc, lane);
+      __syncthreads();  // neighbour updates by lanes 1..3 are visible to lane 0
+      if (lane == 0) {
+        err = res.err;
+        if (P.log) {
+          const unsigned long long li = atomicAdd(&st->log_n, 1ull);
+          if (li < P.log_cap) {
+            mskg_event ev;
+            ev.iter = (int64_t)li + 1;
+            ev.site[0] = s.x; ev.site[1] = s.y; ev.site[2] = s.z;
+            ev.itrans = itrans;
+            ev.arm = res.arm;
+            ev.coor = coor0;
+            ev.rtot = rtot;
+            ev.time = st->clock + tloc;
+            P.log[li] = ev;
+          }
+        }
+        int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
+        if (slot >= 0 && slot < 8) cnt[slot]++;
+        int nt = 0;
+        for (int q = 0; q < res.ntouched; q++) {
+          int kl = tile_local(g, L.tile, res.touched[q]);
+          if (kl < 0) continue;
+          s_tk[nt] = kl;
+          s_tr[nt] = L.rate[res.touched[q].i];
+          s_tc[nt] = L.cls[res.touched[q].i];
+          nt++;
+        }
+        s_nt = nt;
+        s_stop = err;
       }
-      s_nt = nt;
-      s_stop = err;
     }
     ctr.w++;
     nev++;
@@ -365,22 +467,26 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
     }
     if (err) break;
   }
-  // 3. tile totals for the next window; counters
+  // 3. write the list back (active entries only), tile totals, counters
   double part = 0.0;
-  int anyc = 0;
-  for (int j = lane; j < n; j += WAVE) {
-    if (lc[j]) anyc = 1;
-    if (lc[j] != CLS_ISOLATED) part += (double)lr[j];
+  int nout = 0;
+  for (int j0 = 0; j0 < n; j0 += WAVE) {
+    int j = j0 + lane;
+    bool keep = j < n && lc[j] != 0;
+    if (j < n && !keep) L.inlist[tile_point(L.box, g, L.tile, lk[j]).i] = 0;
+    if (keep && lc[j] != CLS_ISOLATED) part += (double)lr[j];
+    unsigned long long m = __ballot(keep);
+    int slot = nout + __popcll(m & ((1ull << lane) - 1ull));
+    if (keep) L.tl_idx[(int64_t)t * L.capg + slot] = lk[j];
+    nout += __popcll(m);
   }
   const double R = wave_sum(part);
-  anyc = __any(anyc);
   if (lane == 0) {
-    P.tile_r[t] = R;
-    if (!anyc) L.tile_active[t] = 0;
-    if (nev) {
-      atomicAdd(&st->events, nev);
-      for (int q = 0; q < 8; q++)
-        if (cnt[q]) atomicAdd(&st->counts[q], cnt[q]);
+    L.tl_n[t] = nout;
+    slot_add(st, t, R, (unsigned long long)nout, nev, cnt);
+    if (nout) {
+      if (g.tz < L.win[4 * P.colour + 2]) atomic_min_i32(&L.win[4 * P.colour + 2], g.tz);
+      if (g.tz > L.win[4 * P.colour + 3]) atomic_max_i32(&L.win[4 * P.colour + 3], g.tz);
     }
     if (err) {
       st->err = err;
@@ -388,36 +494,132 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
       st->err_site[0] = p.x; st->err_site[1] = p.y; st->err_site[2] = p.z;
     }
   }
+  if (err) return;
+  __syncthreads();
+  }  // window loop
 }
 
 // After the eight colour passes: the clock advances by the window just used
-// (operator splitting of the master equation over the colours), and the next
-// window is events_per_pass / (largest tile rate).
-__global__ void __launch_bounds__(1024) k_sweep_end(RunState *st, const double *tile_r, int ntiles) {
-  __shared__ double smax[1024], ssum[1024];
-  double m = 0.0, s = 0.0;
-  for (int t = threadIdx.x; t < ntiles; t += blockDim.x) {
-    double r = tile_r[t];
-    m = fmax(m, r);
-    s += r;
-  }
-  smax[threadIdx.x] = m;
-  ssum[threadIdx.x] = s;
+// (operator splitting of the master equation over the colours), the next
+// window is events_per_pass / (largest tile rate), and the reference's
+// early stop applies when at most 20 MC particles remain (main.f:307-310).
+__global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, const int *list_err) {
+  __shared__ double srsum[NSLOT];
+  __shared__ unsigned long long srmax[NSLOT], sn[NSLOT], stl[NSLOT], sev[NSLOT], scnt[8][NSLOT];
+  const int i = threadIdx.x;
+  Slot &sl = st->slot[i];
+  srsum[i] = sl.rsum; srmax[i] = sl.rmax; sn[i] = sl.n; stl[i] = sl.tiles; sev[i] = sl.events;
+  for (int q = 0; q < 8; q++) scnt[q][i] = sl.counts[q];
+  sl.rsum = 0.0; sl.rmax = 0ull; sl.n = 0ull; sl.tiles = 0ull; sl.events = 0ull;
+  for (int q = 0; q < 8; q++) sl.counts[q] = 0ull;
   __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + off]);
-      ssum[threadIdx.x] += ssum[threadIdx.x + off];
+  for (int off = NSLOT / 2; off > 0; off >>= 1) {
+    if (i < off) {
+      srsum[i] += srsum[i + off];
+      srmax[i] = srmax[i] > srmax[i + off] ? srmax[i] : srmax[i + off];
+      sn[i] += sn[i + off];
+      stl[i] += stl[i + off];
+      sev[i] += sev[i + off];
+      for (int q = 0; q < 8; q++) scnt[q][i] += scnt[q][i + off];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    if (st->tau > 0.0 && !st->err) st->clock += st->tau;
-    st->sweeps++;
-    st->rmax = smax[0];
-    if (ssum[0] > 0.0) st->rtot = ssum[0];
-    st->tau = smax[0] > 0.0 ? st->events_per_pass / smax[0] : 0.0;
-    if (!(smax[0] > 0.0)) st->stop = MSKG_STOP_NO_EVENTS;
+  if (i != 0) return;
+  const double rmax = __longlong_as_double((long long)srmax[0]);
+  const double rsum = srsum[0];
+  const long long nmc = (long long)sn[0];
+  st->events += sev[0];
+  for (int q = 0; q < 8; q++) st->counts[q] += scnt[q][0];
+  // the passes of the next sweep visit the layers that held candidates
+  for (int c = 0; c < 8; c++) {
+    win[4 * c] = win[4 * c + 2];
+    win[4 * c + 1] = win[4 * c + 3];
+    win[4 * c + 2] = 0x7fffffff;
+    win[4 * c + 3] = -1;
+  }
+  if (*list_err && !st->err) st->err = *list_err;
+  if (st->tau > 0.0 && !st->err && !st->stop) {
+    st->clock += st->tau;
+    st->sweeps_run++;
+  }
+  st->sweeps++;
+  st->rmax = rmax;
+  st->n_mc = nmc;
+  if (rsum > 0.0) st->rtot = rsum;
+  if (st->tau_policy == 1)
+    st->tau = rsum > 0.0 ? st->events_per_pass * (double)stl[0] / rsum : 0.0;
+  else
+    st->tau = rmax > 0.0 ? st->events_per_pass / rmax : 0.0;
+  if (!st->stop) {
+    if (nmc <= 20) st->stop = MSKG_STOP_FEW_MC;
+    else if (!(rmax > 0.0)) st->stop = MSKG_STOP_NO_EVENTS;
+  }
+}
+
+// ---- device-side initial state ----------------------------------------------
+// Diamond sites of the reference grid: x, y, z multiples of 3 whose thirds
+// have equal parity; (x+y+z)/3 = 0 (mod 4) is fcc sublattice A (bonds
+// JumpDia(:,1:4)), = 3 (mod 4) sublattice B (JumpDia(:,5:8))
+// (initial/SetSi.f:36-45, modules/defsystem.f:112-113).
+__device__ inline int dia_type(int x, int y, int z) {
+  if (x % 3 || y % 3 || z % 3) return 0;
+  int x1 = x / 3, y1 = y / 3, z1 = z / 3;
+  if ((x1 & 1) != (y1 & 1) || (x1 & 1) != (z1 & 1)) return 0;
+  int m = (x1 + y1 + z1) & 3;
+  return m == 0 ? 1 : (m == 3 ? 2 : 0);
+}
+__constant__ int c_jump_dia[8][3] = {{3, 3, 3}, {3, -3, -3}, {-3, -3, 3}, {-3, 3, -3},
+                                     {3, 3, -3}, {3, -3, 3}, {-3, -3, -3}, {-3, 3, 3}};
+
+// The flat (001) slab of SetSi.f:25-220 (sic=0: Si on both sublattices,
+// wall layers z=0..5 and LenZ-6..LenZ-1) or SetSiC3C.f:25-198 (sic=1: Si on
+// A, C on B, no walls): bulk below len-3, the top atomic layer as MC
+// particles with PTransE(s,2,0,...), the empty sites above with
+// coordination 2 and their deposition rate.  Every point is written.
+__global__ void __launch_bounds__(256) k_build_slab(Lattice L, int len, int sic) {
+  const Tables &t = *L.tab;
+  const int64_t n = L.box.npts;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    pxyz(L.box, p, x, y, z);
+    Site s;
+    for (int a = 0; a < 4; a++) s.d[a][0] = s.d[a][1] = s.d[a][2] = 0;
+    s.kc = 0; s.valid = 0; s.temp = 0;
+    int cls = CLS_NONE;
+    double r = 0.0;
+    const bool wall = !sic && ((z >= 0 && z <= 5) || (z >= L.box.lz - 6 && z <= L.box.lz - 1));
+    const int ty = dia_type(x, y, z);
+    if (wall) {
+      s.kc = KIND_WALL;
+    } else if (ty && z <= len) {
+      const int sp = (sic && ty == 2) ? 2 : 1;
+      NL l;
+      for (int i = 0; i < 4; i++) {
+        const int *j = c_jump_dia[(ty == 1 ? 0 : 4) + i];
+        l.x[i] = x + j[0]; l.y[i] = y + j[1]; l.z[i] = z + j[2];
+        if (l.x[i] >= L.box.lx) l.x[i] -= L.box.lx;
+        if (l.x[i] < 0) l.x[i] += L.box.lx;
+        if (l.y[i] >= L.box.ly) l.y[i] -= L.box.ly;
+        if (l.y[i] < 0) l.y[i] += L.box.ly;
+      }
+      set_list(L.box, s, x, y, z, l);
+      if (z <= len - 4) {
+        s.kc = (uint8_t)(sp | (4 << 3));
+      } else if (z <= len - 1) {
+        s.kc = (uint8_t)(sp | (2 << 3));
+        if (z >= t.frozen_below_z) {
+          cls = CLS_EVAP;
+          r = PEv(t, sp, 2, 0, 0, 0, 0, 0);
+        }
+      } else {
+        s.kc = (uint8_t)(KIND_EMPTY | (2 << 3));
+        cls = CLS_ADSORB;
+        r = pd_of(t, 1, 2, 0) + pd_of(t, 2, 2, 0) + pd_of(t, 3, 2, 0) + t.pabs[0 + 6] + t.pabs[1 + 6] + t.pabs[2 + 6];
+      }
+    }
+    L.site[p] = s;
+    L.rate[p] = cls ? (float)r : 0.0f;
+    L.cls[p] = (uint8_t)cls;
   }
 }
 
@@ -442,15 +644,23 @@ struct mskg_engine {
   Site *d_site = nullptr;
   float *d_rate = nullptr;
   uint8_t *d_cls = nullptr;
-  uint8_t *d_tile_active = nullptr;
-  double *d_tile_r = nullptr;
-  int32_t *d_tiles = nullptr;  // concatenated per-colour tile lists
+  uint8_t *d_inlist = nullptr;
+  int32_t *d_tl_idx = nullptr;
+  int32_t *d_tl_n = nullptr;
+  int32_t *d_tile_n = nullptr;
+  int *d_list_err = nullptr;
+  int *d_win = nullptr;
   Geometry *d_geo = nullptr;
   Tables *d_tab = nullptr;
   RunState *d_st = nullptr;
   int *d_err = nullptr;
-  int tile = 0, ntx = 0, nty = 0, ntz = 0, ntiles = 0, cap = 1024;
-  std::vector<int> colour_off, colour_n;
+  int tile = 0, ntx = 0, nty = 0, ntz = 0, ntiles = 0, cap = 1024, capg = 512;
+  bool lists_dirty = true;
+  int pass_grid = 4096;  // workgroups per colour pass (they stride over the window)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  mskg_event *d_log = nullptr;
+  int64_t log_cap = 0;
   double run_ms = 0, sweep_ms = 0;
   int64_t sweep_points = 0, sweeps_done = 0;
   bool loaded = false;
@@ -487,7 +697,13 @@ static Lattice make_lattice(mskg_engine *e) {
   L.site = e->d_site;
   L.rate = e->d_rate;
   L.cls = e->d_cls;
-  L.tile_active = e->d_tile_active;
+  L.inlist = e->d_inlist;
+  L.tl_idx = e->d_tl_idx;
+  L.tl_n = e->d_tl_n;
+  L.list_err = e->d_list_err;
+  L.win = e->d_win;
+  L.capg = e->capg;
+  L.cur_tile = -1;
   L.box = e->box;
   L.geo = e->d_geo;
   L.tab = e->d_tab;
@@ -585,6 +801,7 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   e->tab.tm = cfg->tm;
   e->tab.sigma = cfg->sigma;
   e->tab.tflex = cfg->tflex;
+  e->tab.frozen_below_z = cfg->frozen_below_z;
   build_geometry(e->geo);
   e->tile = choose_tile(cfg->len_x, cfg->len_y, cfg->tile);
   if (e->tile < 0)
@@ -593,6 +810,7 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
                  "number of tiles along x and y",
                  cfg->len_x, cfg->len_y, cfg->tile);
   e->cap = cfg->tile_capacity > 0 ? cfg->tile_capacity : 1024;
+  e->capg = e->cap;
   e->ntx = cfg->len_x / e->tile;
   e->nty = cfg->len_y / e->tile;
   e->ntz = (cfg->len_z + 1 + e->tile - 1) / e->tile;
@@ -607,9 +825,11 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   HIPCHK(e, hipMalloc(&e->d_site, n * sizeof(Site)));
   HIPCHK(e, hipMalloc(&e->d_rate, n * sizeof(float)));
   HIPCHK(e, hipMalloc(&e->d_cls, n * sizeof(uint8_t)));
-  HIPCHK(e, hipMalloc(&e->d_tile_active, e->ntiles));
-  HIPCHK(e, hipMalloc(&e->d_tile_r, e->ntiles * sizeof(double)));
-  HIPCHK(e, hipMalloc(&e->d_tiles, e->ntiles * sizeof(int32_t)));
+  HIPCHK(e, hipMalloc(&e->d_inlist, n));
+  HIPCHK(e, hipMalloc(&e->d_tl_idx, (size_t)e->ntiles * e->capg * sizeof(int32_t)));
+  HIPCHK(e, hipMalloc(&e->d_tl_n, e->ntiles * sizeof(int32_t)));
+  HIPCHK(e, hipMalloc(&e->d_list_err, sizeof(int)));
+  HIPCHK(e, hipMalloc(&e->d_win, 32 * sizeof(int)));
   HIPCHK(e, hipMalloc(&e->d_geo, sizeof(Geometry)));
   HIPCHK(e, hipMalloc(&e->d_tab, sizeof(Tables)));
   HIPCHK(e, hipMalloc(&e->d_st, sizeof(RunState)));
@@ -619,21 +839,9 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   HIPCHK(e, hipMemsetAsync(e->d_site, 0, n * sizeof(Site), e->stream));
   HIPCHK(e, hipMemsetAsync(e->d_rate, 0, n * sizeof(float), e->stream));
   HIPCHK(e, hipMemsetAsync(e->d_cls, 0, n, e->stream));
-  HIPCHK(e, hipMemsetAsync(e->d_tile_active, 0, e->ntiles, e->stream));
-  HIPCHK(e, hipMemsetAsync(e->d_tile_r, 0, e->ntiles * sizeof(double), e->stream));
-  // tile lists per colour (colour = parity of the tile coordinates)
-  std::vector<int32_t> tl;
-  e->colour_off.assign(8, 0);
-  e->colour_n.assign(8, 0);
-  for (int c = 0; c < 8; c++) {
-    e->colour_off[c] = (int)tl.size();
-    for (int t = 0; t < e->ntiles; t++) {
-      int tx = t % e->ntx, ty = (t / e->ntx) % e->nty, tz = t / (e->ntx * e->nty);
-      if (((tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2)) == c) tl.push_back(t);
-    }
-    e->colour_n[c] = (int)tl.size() - e->colour_off[c];
-  }
-  HIPCHK(e, hipMemcpyAsync(e->d_tiles, tl.data(), tl.size() * sizeof(int32_t), hipMemcpyHostToDevice, e->stream));
+  HIPCHK(e, hipMemsetAsync(e->d_inlist, 0, n, e->stream));
+  HIPCHK(e, hipMemsetAsync(e->d_tl_n, 0, e->ntiles * sizeof(int32_t), e->stream));
+  HIPCHK(e, hipMemsetAsync(e->d_list_err, 0, sizeof(int), e->stream));
   RunState st;
   memset(&st, 0, sizeof st);
   HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
@@ -645,8 +853,10 @@ void mskg_destroy(mskg_engine *e) {
   if (!e) return;
   if (e->stream) hipStreamSynchronize(e->stream);
   hipFree(e->d_site); hipFree(e->d_rate); hipFree(e->d_cls);
-  hipFree(e->d_tile_active); hipFree(e->d_tile_r); hipFree(e->d_tiles);
-  hipFree(e->d_geo); hipFree(e->d_tab); hipFree(e->d_st); hipFree(e->d_err);
+  if (e->graph_exec) hipGraphExecDestroy(e->graph_exec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  hipFree(e->d_inlist); hipFree(e->d_tl_idx); hipFree(e->d_tl_n); hipFree(e->d_list_err); hipFree(e->d_win);
+  hipFree(e->d_geo); hipFree(e->d_tab); hipFree(e->d_st); hipFree(e->d_err); hipFree(e->d_log);
   if (e->ev0) hipEventDestroy(e->ev0);
   if (e->ev1) hipEventDestroy(e->ev1);
   if (e->stream) hipStreamDestroy(e->stream);
@@ -725,6 +935,7 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt, int32_t n_ad, const int
   HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
   e->loaded = true;
+  e->lists_dirty = true;
   return MSKG_OK;
 }
 
@@ -755,7 +966,8 @@ int mskg_refresh_rates(mskg_engine *e) {
   if (rc) return rc;
   HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
   Lattice L = make_lattice(e);
-  L.tile_active = nullptr;
+  L.inlist = nullptr;
+  e->lists_dirty = true;
   int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
   HIPCHK(e, hipEventRecord(e->ev0, e->stream));
   hipLaunchKernelGGL(k_rate_sweep, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err);
@@ -783,7 +995,8 @@ int mskg_replay(mskg_engine *e, const mskg_event *log, int64_t n, int32_t clock_
     return fail_(e, MSKG_EDEVICE, "replay upload: %s", hipGetErrorString(ce));
   }
   Lattice L = make_lattice(e);
-  L.tile_active = nullptr;
+  L.inlist = nullptr;
+  e->lists_dirty = true;
   hipLaunchKernelGGL(k_replay, dim3(1), dim3(WAVE), 0, e->stream, L, d_log, n, clock_mode, e->d_st);
   hipError_t le = hipGetLastError();
   RunState st;
@@ -807,55 +1020,78 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
   if (rc) return rc;
   if (st.err) return fail_(e, MSKG_ESTATE, "engine is in an error state: %s", err_name(st.err));
   const unsigned long long ev_start = st.events;
-  // tile totals and active flags from the current rates, then the first window
-  hipLaunchKernelGGL(k_tile_scan, dim3((e->ntiles + 3) / 4), dim3(256), 0, e->stream, L, e->ntiles, e->ntz,
-                     e->d_tile_r);
-  HIPCHK(e, hipGetLastError());
+  const int tblocks = (e->ntiles + 3) / 4;
+  {
+    static int w0[32];
+    for (int c = 0; c < 8; c++) {
+      w0[4 * c] = w0[4 * c + 2] = 0x7fffffff;
+      w0[4 * c + 1] = w0[4 * c + 3] = -1;
+    }
+    HIPCHK(e, hipMemcpyAsync(e->d_win, w0, sizeof w0, hipMemcpyHostToDevice, e->stream));
+  }
+  if (e->lists_dirty) {
+    HIPCHK(e, hipMemsetAsync(e->d_list_err, 0, sizeof(int), e->stream));
+    hipLaunchKernelGGL(k_build_lists, dim3(tblocks), dim3(256), 0, e->stream, L, e->ntiles);
+    HIPCHK(e, hipGetLastError());
+    e->lists_dirty = false;
+  } else {
+    hipLaunchKernelGGL(k_list_window, dim3(tblocks), dim3(256), 0, e->stream, L, e->ntiles);
+    HIPCHK(e, hipGetLastError());
+  }
   {
     RunState s0 = st;
     s0.tau = 0.0;  // k_sweep_end with tau = 0 only sets the next window
     s0.stop = 0;
+    memset(s0.slot, 0, sizeof s0.slot);
+    s0.tau_policy = e->cfg.tau_policy;
     s0.events_per_pass = e->cfg.events_per_pass > 0 ? e->cfg.events_per_pass : 1.0;
     HIPCHK(e, hipMemcpyAsync(e->d_st, &s0, sizeof s0, hipMemcpyHostToDevice, e->stream));
   }
-  hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(1024), 0, e->stream, e->d_st, e->d_tile_r, e->ntiles);
+  // tile totals of the current rates, then the first window
+  hipLaunchKernelGGL(k_tile_rates, dim3(tblocks), dim3(256), 0, e->stream, L, e->ntiles, e->d_st);
+  HIPCHK(e, hipGetLastError());
+  hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err);
   HIPCHK(e, hipGetLastError());
   const size_t shm = (size_t)e->cap * (4 + 4 + 1) + 16;
+  // one sweep = eight colour passes + the window/clock update, captured once
+  // as a graph and replayed
+  if (!e->graph_exec) {
+    const int grid = std::max(1, std::min(e->ntiles / 8 + 1, e->pass_grid));
+    HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    for (int c = 0; c < 8; c++) {
+      PassParams P;
+      P.L = L;
+      P.st = e->d_st;
+      P.log = e->d_log;
+      P.log_cap = (unsigned long long)e->log_cap;
+      P.ntz = e->ntz;
+      P.cap = e->cap;
+      P.key0 = (uint32_t)e->cfg.seed;
+      P.key1 = (uint32_t)(e->cfg.seed >> 32);
+      P.colour = c;
+      hipLaunchKernelGGL(k_pass, dim3(grid), dim3(WAVE), shm, e->stream, P);
+    }
+    hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err);
+    HIPCHK(e, hipStreamEndCapture(e->stream, &e->graph));
+    HIPCHK(e, hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+  }
   int32_t stop = MSKG_STOP_LIMIT;
   HIPCHK(e, hipEventRecord(e->ev0, e->stream));
   int64_t sweeps = 0;
   int batch = 1;
   for (;;) {
-    for (int sb = 0; sb < batch; sb++) {
-      for (int c = 0; c < 8; c++) {
-        if (e->colour_n[c] == 0) continue;
-        PassParams P;
-        P.L = L;
-        P.st = e->d_st;
-        P.tile_r = e->d_tile_r;
-        P.tiles = e->d_tiles + e->colour_off[c];
-        P.ntiles = e->colour_n[c];
-        P.ntz = e->ntz;
-        P.cap = e->cap;
-        P.key0 = (uint32_t)e->cfg.seed;
-        P.key1 = (uint32_t)(e->cfg.seed >> 32);
-        P.colour = c;
-        hipLaunchKernelGGL(k_pass, dim3(P.ntiles), dim3(WAVE), shm, e->stream, P);
-      }
-      hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(1024), 0, e->stream, e->d_st, e->d_tile_r, e->ntiles);
-    }
-    HIPCHK(e, hipGetLastError());
+    for (int sb = 0; sb < batch; sb++) HIPCHK(e, hipGraphLaunch(e->graph_exec, e->stream));
     sweeps += batch;
     rc = read_state(e, st);
     if (rc) return rc;
     if (st.err) break;
     int64_t done = (int64_t)(st.events - ev_start);
-    if (st.stop == MSKG_STOP_NO_EVENTS) { stop = MSKG_STOP_NO_EVENTS; break; }
+    if (st.stop) { stop = st.stop; break; }
     if (done >= max_events || st.clock >= max_time) break;
     // size the next batch of sweeps from the observed events per sweep
     double per = (double)done / (double)sweeps;
     int64_t want = per > 0 ? (int64_t)((double)(max_events - done) / per * 0.9) : 16;
-    batch = (int)std::max<int64_t>(1, std::min<int64_t>(want, 256));
+    batch = (int)std::max<int64_t>(1, std::min<int64_t>(want, 512));
   }
   HIPCHK(e, hipEventRecord(e->ev1, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
@@ -993,6 +1229,68 @@ int mskg_last_timing(mskg_engine *e, double *run_ms, double *sweep_ms, int64_t *
   if (sweep_ms) *sweep_ms = e->sweep_ms;
   if (sweep_points) *sweep_points = e->sweep_points;
   if (sweeps_done) *sweeps_done = e->sweeps_done;
+  return MSKG_OK;
+}
+
+int mskg_build_slab(mskg_engine *e, int32_t mode, int32_t len) {
+  if (!e || !e->d_site) return MSKG_EINVAL;
+  if (mode != 0 && mode != 1) return fail_(e, MSKG_EINVAL, "mode must be 0 (SetSi) or 1 (SetSiC3C)");
+  if (len < 6 || len > e->box.lz - (mode == 0 ? 7 : 1) || len % 3)
+    return fail_(e, MSKG_EINVAL, "slab height %d does not fit the box height %d", len, e->box.lz);
+  Lattice L = make_lattice(e);
+  int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(k_build_slab, dim3(blocks), dim3(256), 0, e->stream, L, len, mode);
+  HIPCHK(e, hipGetLastError());
+  // the total rate of the initial population seeds the isolated-atom rule
+  RunState st;
+  memset(&st, 0, sizeof st);
+  st.events_per_pass = e->cfg.events_per_pass > 0 ? e->cfg.events_per_pass : 1.0;
+  const double nsurf = (double)e->box.lx * e->box.ly / 36.0;  // 2 atoms per (6x6) surface cell per layer
+  const Tables &t = e->tab;
+  st.rtot = nsurf * (mode ? 0.5 * (t.pe[0 + 3 * 2] + t.pe[1 + 3 * 2]) : t.pe[0 + 3 * 2]) +
+            nsurf * (t.pd[0 + 3] + t.pd[1 + 3] + t.pd[2 + 3]);
+  HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(e, hipStreamSynchronize(e->stream));
+  e->loaded = true;
+  e->lists_dirty = true;
+  return MSKG_OK;
+}
+
+int mskg_set_event_log(mskg_engine *e, int64_t capacity) {
+  if (!e || capacity < 0) return MSKG_EINVAL;
+  if (e->d_log) { HIPCHK(e, hipFree(e->d_log)); e->d_log = nullptr; }
+  e->log_cap = 0;
+  if (capacity > 0) {
+    HIPCHK(e, hipMalloc(&e->d_log, capacity * sizeof(mskg_event)));
+    e->log_cap = capacity;
+  }
+  if (e->graph_exec) { hipGraphExecDestroy(e->graph_exec); e->graph_exec = nullptr; }
+  if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
+  if (e->loaded) {
+    RunState st;
+    int rc = read_state(e, st);
+    if (rc) return rc;
+    st.log_n = 0;
+    HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+  }
+  return MSKG_OK;
+}
+
+int mskg_get_event_log(mskg_engine *e, mskg_event *out, int64_t cap, int64_t *n) {
+  if (!e || !n) return MSKG_EINVAL;
+  RunState st;
+  int rc = read_state(e, st);
+  if (rc) return rc;
+  *n = (int64_t)st.log_n;
+  if (st.log_n > (unsigned long long)e->log_cap)
+    return fail_(e, MSKG_ECAPACITY, "event log overflowed: %lld events, capacity %lld", (long long)st.log_n,
+                 (long long)e->log_cap);
+  int64_t m = std::min<int64_t>(*n, cap);
+  if (m > 0 && out) {
+    HIPCHK(e, hipMemcpyAsync(out, e->d_log, m * sizeof(mskg_event), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+  }
   return MSKG_OK;
 }
 

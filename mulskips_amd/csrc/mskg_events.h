@@ -21,18 +21,28 @@ enum : int {
   ERR_NOT_MC = 3,     // replayed event on a site that is not an MC particle
   ERR_COVERAGE = 4,   // coverage index not in ListCov
   ERR_CAPACITY = 5,   // tile candidate list overflow
-  ERR_TEMPERATURE = 6 // zero temperature in LA mode
+  ERR_TEMPERATURE = 6, // zero temperature in LA mode
+  ERR_TILE_LIST = 7    // a tile's candidate list overflowed
 };
 
+// Per-tile candidate lists (synchronous-sublattice mode): every MC particle
+// of a tile is listed once (inlist[p] = 1) as its tile-local offset; entries
+// whose class has dropped to 0 are removed when the tile is next processed.
 struct Lattice {
   Site *site;
   float *rate;
   uint8_t *cls;
-  uint8_t *tile_active;   // may be null
+  uint8_t *inlist;        // may be null (no list maintenance)
+  int32_t *tl_idx;        // [ntiles][capg] tile-local offsets
+  int32_t *tl_n;          // [ntiles] entries used
+  int *list_err;          // set on overflow
+  int *win;               // [8][4] per colour: tile-layer window lo, hi (read by that colour's
+                          // pass), next lo, next hi (collected for the next sweep)
   Box box;
   const Geometry *geo;
   const Tables *tab;
-  int tile, ntx, nty;     // tile edge in grid units and tile counts (for tile_active)
+  int tile, ntx, nty, capg;
+  int cur_tile;           // the tile whose pass is running: its list lives on chip
 };
 
 struct P3 {
@@ -68,16 +78,66 @@ MSKG_HD bool recip(const NL &l, const P3 &a) {
   return false;
 }
 
-MSKG_HD void mark_tile(const Lattice &L, const P3 &p) {
-  if (!L.tile_active || p.i < 0) return;
-  int tx = p.x / L.tile, ty = p.y / L.tile, tz = p.z / L.tile;
-  L.tile_active[((int64_t)tz * L.nty + ty) * L.ntx + tx] = 1;
+MSKG_HD int tile_of(const Lattice &L, const P3 &p) {
+  return ((p.z / L.tile) * L.nty + p.y / L.tile) * L.ntx + p.x / L.tile;
+}
+MSKG_HD int tile_offset(const Lattice &L, const P3 &p) {
+  const int half = L.tile / 2, z0 = (p.z / L.tile) * L.tile;
+  return ((p.z - z0) * half + ((p.y % L.tile) >> 1)) * half + ((p.x % L.tile) >> 1);
+}
+MSKG_HD int atomic_add_i32(int32_t *a, int v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return atomicAdd(a, v);
+#else
+  int o = *a;
+  *a += v;
+  return o;
+#endif
+}
+MSKG_HD void atomic_min_i32(int *a, int v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  atomicMin(a, v);
+#else
+  if (v < *a) *a = v;
+#endif
+}
+MSKG_HD void atomic_max_i32(int *a, int v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  atomicMax(a, v);
+#else
+  if (v > *a) *a = v;
+#endif
+}
+MSKG_HD int tile_colour(int tx, int ty, int tz) { return (tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2); }
+// A tile layer that holds candidates joins the window of layers its colour's
+// pass visits.  A colour's window is only widened by passes of the other
+// colours (same-colour tiles never touch each other), so it is constant
+// while its own pass runs.
+MSKG_HD void widen_window(const Lattice &L, int colour, int tz) {
+  if (!L.win) return;
+  int *w = L.win + 4 * colour;
+  // read first: the window rarely changes, and atomics on one word serialise
+  if (tz < w[0]) atomic_min_i32(&w[0], tz);
+  if (tz > w[1]) atomic_max_i32(&w[1], tz);
+  if (tz < w[2]) atomic_min_i32(&w[2], tz);
+  if (tz > w[3]) atomic_max_i32(&w[3], tz);
+}
+// A site that becomes an MC particle joins its tile's list (once).
+MSKG_HD void enlist(const Lattice &L, const P3 &p) {
+  if (!L.inlist || L.inlist[p.i]) return;
+  L.inlist[p.i] = 1;
+  const int t = tile_of(L, p);
+  if (t == L.cur_tile) return;  // the running tile keeps its list on chip
+  widen_window(L, tile_colour(p.x / L.tile, p.y / L.tile, p.z / L.tile), p.z / L.tile);
+  const int slot = atomic_add_i32(&L.tl_n[t], 1);
+  if (slot < L.capg) L.tl_idx[(int64_t)t * L.capg + slot] = tile_offset(L, p);
+  else *L.list_err = ERR_TILE_LIST;
 }
 MSKG_HD void set_rate(const Lattice &L, const P3 &p, int cls, double rate) {
   if (p.i < 0) return;
   L.rate[p.i] = cls ? (float)rate : 0.0f;
   L.cls[p.i] = (uint8_t)cls;
-  if (cls) mark_tile(L, p);
+  if (cls) enlist(L, p);
 }
 
 // ---- rates ---------------------------------------------------------------
@@ -139,10 +199,19 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
     return;
   }
   // evaporation, :215-324
+  if (p.z < t.frozen_below_z) {  // frozen substrate (engine option, no reference counterpart)
+    cls = CLS_NONE;
+    rate = 0.0;
+    return;
+  }
   int n_si = 0, n_c = 0, n_b = 0, n_wall = 0, covnn[3] = {0, 0, 0};
+  P3 qs[4];
+  Site sqs[4];
+  for (int i = 0; i < 4; i++) qs[i] = entry(L, nn, i);
+  for (int i = 0; i < 4; i++) sqs[i] = rd(L, qs[i].i);  // four independent gathers in flight
   for (int i = 0; i < 4; i++) {
-    P3 q = entry(L, nn, i);
-    Site sq = rd(L, q.i);
+    const P3 &q = qs[i];
+    const Site &sq = sqs[i];
     int kq = s_kind(sq);
     if (kind_occupied(kq)) {
       if (kq != KIND_WALL) {
@@ -205,272 +274,432 @@ MSKG_HD bool put_list(const Lattice &L, Site &s, const P3 &p, const NL &l, EvRes
   return true;
 }
 
-// Deposition, mcevent/deposition.f:24-251.
-MSKG_HD void deposition(const Lattice &L, double rtot, const P3 &s, int species, const Choice &ch, EvResult &res) {
-  const Geometry &g = *L.geo;
-  const int lx = L.box.lx, ly = L.box.ly;
-  Site ss = rd(L, s.i);
-  int coor = s_coor(ss), coor_old = coor;
-  s_set_kind(ss, species);
-  wr(L, s.i, ss);
-  res.arm = -1;
-  NL nn_site = list_of(L, s, ss);
-  if (coor == 1) {  // :49-74, FIND_SN + IF_Arm_Choice
-    P3 c = entry(L, nn_site, 0);
-    Site sc = rd(L, c.i);
-    NL lc = list_of(L, c, sc);
-    int sn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, j = 0;
-    for (int i = 0; i < 4; i++)
-      if (!same(lc, i, s)) {
-        if (j < 3) { sn[j][0] = lc.x[i]; sn[j][1] = lc.y[i]; sn[j][2] = lc.z[i]; }
-        j++;
-      }
-    int site[3] = {s.x, s.y, s.z}, cc[3] = {c.x, c.y, c.z}, zig[3][3], arm[3][3];
-    find_sn(g, lx, ly, site, cc, sn, zig, arm);
-    int carm = 0, czig = 0;  // IF_Arm_Choice.f:35-54
-    for (int k = 0; k < 3; k++) {
-      carm += s_coor(rd(L, at(L, arm[k][0], arm[k][1], arm[k][2]).i));
-      czig += s_coor(rd(L, at(L, zig[k][0], zig[k][1], zig[k][2]).i));
-    }
-    int is_arm;
-    if (ch.arm >= 0) is_arm = ch.arm;
-    else if (czig > carm) is_arm = 0;
-    else if (czig == carm) is_arm = (ch.u_arm <= L.tab->ptrans_zig) ? 0 : 1;
-    else is_arm = 1;
-    res.arm = is_arm;
-    for (int k = 0; k < 3; k++) {
-      nn_site.x[k + 1] = is_arm ? arm[k][0] : zig[k][0];
-      nn_site.y[k + 1] = is_arm ? arm[k][1] : zig[k][1];
-      nn_site.z[k + 1] = is_arm ? arm[k][2] : zig[k][2];
-    }
-    if (!put_list(L, ss, s, nn_site, res)) return;
-    wr(L, s.i, ss);
-  }
-  res.ntouched = 0;
-  res.touched[res.ntouched++] = s;
-  for (int i = 0; i < 4; i++) {  // :78-235
-    P3 p = entry(L, nn_site, i);
-    Site sp = rd(L, p.i);
-    res.touched[res.ntouched++] = p;
-    int kind = s_kind(sp), coornn = s_coor(sp);
-    if (!kind_occupied(kind)) {
-      if (coornn == 0) {
-        coornn = 1;
-        s_set_coor(sp, coornn);
-        NL nnn;
-        for (int j = 0; j < 4; j++) nnn.x[j] = nnn.y[j] = nnn.z[j] = 0;
-        nnn.x[0] = s.x; nnn.y[0] = s.y; nnn.z[0] = s.z;
-        int cls = 0;
-        double r = 0.0;
-        get_prob(L, rtot, p, sp, coornn, nnn, cls, r, res.err);
-        if (cls && !put_list(L, sp, p, nnn, res)) return;  // AddMC stores the list
-        wr(L, p.i, sp);
-        set_rate(L, p, cls, r);
-      } else if (coornn == 1) {
-        NL nnn = list_of(L, p, sp);
-        int dx = nnn.x[0] - s.x, dy = nnn.y[0] - s.y, dz = nnn.z[0] - s.z;
-        dx = dx < 0 ? -dx : dx;
-        dx = dx < lx - dx ? dx : lx - dx;
-        dy = dy < 0 ? -dy : dy;
-        dy = dy < ly - dy ? dy : ly - dy;
-        if (dx * dx + dy * dy + dz * dz == 72) {
-          coornn = 2;
-          nnn.x[1] = s.x; nnn.y[1] = s.y; nnn.z[1] = s.z;
-          int a1[3] = {nnn.x[0], nnn.y[0], nnn.z[0]}, a2[3] = {s.x, s.y, s.z}, pc[3] = {p.x, p.y, p.z}, ns[2][3];
-          find_nn(g, lx, ly, a1, a2, pc, ns);
-          for (int j = 2; j < 4; j++) {
-            nnn.x[j] = ns[j - 2][0]; nnn.y[j] = ns[j - 2][1]; nnn.z[j] = ns[j - 2][2];
-            P3 q = entry(L, nnn, j);
-            Site sq = rd(L, q.i);
-            int kq = s_kind(sq);
-            if (kind_occupied(kq) && s_coor(sq) <= 3 && kq != KIND_WALL) {
-              if (recip(list_of(L, q, sq), p)) coornn++;
-            }
-          }
-          s_set_coor(sp, coornn);
-          if (!put_list(L, sp, p, nnn, res)) return;
-          wr(L, p.i, sp);
-          refresh(L, rtot, p, sp, coornn, nnn, res.err);
-        }
-      } else if (coornn == 2 || coornn == 3) {
-        NL nnn = list_of(L, p, sp);
-        if (recip(nnn, s)) {
-          coornn++;
-          s_set_coor(sp, coornn);
-          wr(L, p.i, sp);
-          refresh(L, rtot, p, sp, coornn, nnn, res.err);
-        }
-      }
-    } else if (kind != KIND_WALL) {
-      if (coornn <= 3) {
-        NL nnn = list_of(L, p, sp);
-        if (recip(nnn, s)) {
-          coornn++;
-          s_set_coor(sp, coornn);
-          wr(L, p.i, sp);
-          refresh(L, rtot, p, sp, coornn, nnn, res.err);
-          if (coor_old == 1 && i >= 1) coor++;
-        }
-      }
-    } else {
-      coor++;  // wall neighbour
-    }
-  }
-  ss = rd(L, s.i);
-  if (coor != coor_old) {
-    s_set_coor(ss, coor);
-    wr(L, s.i, ss);
-  }
-  refresh(L, rtot, s, ss, coor, nn_site, res.err);
+// ---- event execution -------------------------------------------------------
+// The whole workgroup (one 64-wide wave) executes an event together: lane 0
+// updates the site itself, lanes 0..3 update its four neighbours at once.
+// The reference visits the neighbours one after the other, but each
+// neighbour's update writes only that neighbour's record and reads only
+// records the other neighbour updates do not write (two neighbours of a
+// site are never neighbours of each other), so the concurrent updates give
+// the sequential result.  When two list entries resolve to the same point
+// the neighbours are visited in order by lane 0 instead.
+struct EvScratch {
+  NL nn;            // the site's neighbour list as the update loop uses it
+  P3 pts[4];
+  int coor_old;
+  int inc[4];       // per-neighbour change of the site's own coordination
+  int err[4];
+  int unfixed[4];   // evaporation: neighbour i was unfixed (Coor 2 -> 1)
+  int found[4];     //   ... and found its reciprocal partner (NBuff) at found_pos[i]
+  int found_pos[4][3];
+  int seq;          // visit the neighbours in order on lane 0
+  int abort;
+};
+
+#define MSKG_SYNC() __syncthreads()
+
+__device__ inline void prefetch4(const Lattice &L, const NL &nn, P3 q[4], Site sq[4]) {
+  for (int i = 0; i < 4; i++) q[i] = entry(L, nn, i);
+  for (int i = 0; i < 4; i++) sq[i] = rd(L, q[i].i);
 }
 
-// The part of Evaporation.f:82-128 / 225-259 that unfixes a list when a
-// site drops to coordination 1: keep only the last occupied reciprocal
-// neighbour (NBuff) in slot 1.  nbuff persists across calls within one
-// evaporation exactly like the reference's local variable.
-MSKG_HD void unfix(const Lattice &L, const P3 &p, const NL &old, NL &nl, int nbuff[3]) {
+__device__ inline void resolve_neighbours(const Lattice &L, EvScratch &sc, int n) {
+  int seq = 0;
+  for (int i = 0; i < 4; i++) sc.pts[i] = entry(L, sc.nn, i);
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++)
+      if (sc.pts[i].i >= 0 && sc.pts[i].i == sc.pts[j].i) seq = 1;
+  sc.seq = seq;
+  for (int i = 0; i < 4; i++) {
+    sc.inc[i] = 0;
+    sc.err[i] = 0;
+    sc.unfixed[i] = 0;
+    sc.found[i] = 0;
+  }
+}
+
+__device__ inline void touched_list(const P3 &s, const EvScratch &sc, int n, EvResult &res) {
+  res.ntouched = 0;
+  res.touched[res.ntouched++] = s;
+  for (int i = 0; i < n; i++) res.touched[res.ntouched++] = sc.pts[i];
+}
+
+// deposition.f:78-235, the update of neighbour i of the new atom at s.
+__device__ void dep_neighbour(const Lattice &L, double rtot, const P3 &s, int i, EvScratch &sc, int &err) {
+  const Geometry &g = *L.geo;
+  const int lx = L.box.lx, ly = L.box.ly;
+  const P3 p = sc.pts[i];
+  Site sp = rd(L, p.i);
+  int kind = s_kind(sp), coornn = s_coor(sp);
+  if (!kind_occupied(kind)) {
+    if (coornn == 0) {
+      coornn = 1;
+      s_set_coor(sp, coornn);
+      NL nnn;
+      for (int j = 0; j < 4; j++) nnn.x[j] = nnn.y[j] = nnn.z[j] = 0;
+      nnn.x[0] = s.x; nnn.y[0] = s.y; nnn.z[0] = s.z;
+      int cls = 0;
+      double r = 0.0;
+      get_prob(L, rtot, p, sp, coornn, nnn, cls, r, err);
+      if (cls && !set_list(L.box, sp, p.x, p.y, p.z, nnn)) { err = ERR_GEOM; return; }  // AddMC stores the list
+      wr(L, p.i, sp);
+      set_rate(L, p, cls, r);
+    } else if (coornn == 1) {
+      NL nnn = list_of(L, p, sp);
+      int dx = nnn.x[0] - s.x, dy = nnn.y[0] - s.y, dz = nnn.z[0] - s.z;
+      dx = dx < 0 ? -dx : dx;
+      dx = dx < lx - dx ? dx : lx - dx;
+      dy = dy < 0 ? -dy : dy;
+      dy = dy < ly - dy ? dy : ly - dy;
+      if (dx * dx + dy * dy + dz * dz == 72) {
+        coornn = 2;
+        nnn.x[1] = s.x; nnn.y[1] = s.y; nnn.z[1] = s.z;
+        int a1[3] = {nnn.x[0], nnn.y[0], nnn.z[0]}, a2[3] = {s.x, s.y, s.z}, pc[3] = {p.x, p.y, p.z}, ns[2][3];
+        find_nn(g, lx, ly, a1, a2, pc, ns);
+        P3 q[2];
+        Site sq[2];
+        for (int j = 2; j < 4; j++) {
+          nnn.x[j] = ns[j - 2][0]; nnn.y[j] = ns[j - 2][1]; nnn.z[j] = ns[j - 2][2];
+          q[j - 2] = entry(L, nnn, j);
+        }
+        sq[0] = rd(L, q[0].i);
+        sq[1] = rd(L, q[1].i);
+        for (int j = 0; j < 2; j++) {
+          int kq = s_kind(sq[j]);
+          if (kind_occupied(kq) && s_coor(sq[j]) <= 3 && kq != KIND_WALL) {
+            if (recip(list_of(L, q[j], sq[j]), p)) coornn++;
+          }
+        }
+        s_set_coor(sp, coornn);
+        if (!set_list(L.box, sp, p.x, p.y, p.z, nnn)) { err = ERR_GEOM; return; }
+        wr(L, p.i, sp);
+        refresh(L, rtot, p, sp, coornn, nnn, err);
+      }
+    } else if (coornn == 2 || coornn == 3) {
+      NL nnn = list_of(L, p, sp);
+      if (recip(nnn, s)) {
+        coornn++;
+        s_set_coor(sp, coornn);
+        wr(L, p.i, sp);
+        refresh(L, rtot, p, sp, coornn, nnn, err);
+      }
+    }
+  } else if (kind != KIND_WALL) {
+    if (coornn <= 3) {
+      NL nnn = list_of(L, p, sp);
+      if (recip(nnn, s)) {
+        coornn++;
+        s_set_coor(sp, coornn);
+        wr(L, p.i, sp);
+        refresh(L, rtot, p, sp, coornn, nnn, err);
+        if (sc.coor_old == 1 && i >= 1) sc.inc[i] = 1;
+      }
+    }
+  } else {
+    sc.inc[i] = 1;  // wall neighbour
+  }
+}
+
+// Deposition, mcevent/deposition.f:24-251.
+__device__ void deposition(const Lattice &L, double rtot, const P3 &s, int species, const Choice &ch, EvResult &res,
+                           EvScratch &sc, int lane) {
+  const Geometry &g = *L.geo;
+  if (lane == 0) {
+    sc.abort = 0;
+    Site ss = rd(L, s.i);
+    int coor = s_coor(ss);
+    sc.coor_old = coor;
+    s_set_kind(ss, species);
+    wr(L, s.i, ss);
+    res.arm = -1;
+    NL nn_site = list_of(L, s, ss);
+    if (coor == 1) {  // :49-74, FIND_SN + IF_Arm_Choice
+      P3 c = entry(L, nn_site, 0);
+      Site scr = rd(L, c.i);
+      NL lc = list_of(L, c, scr);
+      int sn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, j = 0;
+      for (int i = 0; i < 4; i++)
+        if (!same(lc, i, s)) {
+          if (j < 3) { sn[j][0] = lc.x[i]; sn[j][1] = lc.y[i]; sn[j][2] = lc.z[i]; }
+          j++;
+        }
+      int site[3] = {s.x, s.y, s.z}, cc[3] = {c.x, c.y, c.z}, zig[3][3], arm[3][3];
+      find_sn(g, L.box.lx, L.box.ly, site, cc, sn, zig, arm);
+      Site sa[3], sz[3];
+      for (int k = 0; k < 3; k++) {
+        sa[k] = rd(L, at(L, arm[k][0], arm[k][1], arm[k][2]).i);
+        sz[k] = rd(L, at(L, zig[k][0], zig[k][1], zig[k][2]).i);
+      }
+      int carm = 0, czig = 0;  // IF_Arm_Choice.f:35-54
+      for (int k = 0; k < 3; k++) {
+        carm += s_coor(sa[k]);
+        czig += s_coor(sz[k]);
+      }
+      int is_arm;
+      if (ch.arm >= 0) is_arm = ch.arm;
+      else if (czig > carm) is_arm = 0;
+      else if (czig == carm) is_arm = (ch.u_arm <= L.tab->ptrans_zig) ? 0 : 1;
+      else is_arm = 1;
+      res.arm = is_arm;
+      for (int k = 0; k < 3; k++) {
+        nn_site.x[k + 1] = is_arm ? arm[k][0] : zig[k][0];
+        nn_site.y[k + 1] = is_arm ? arm[k][1] : zig[k][1];
+        nn_site.z[k + 1] = is_arm ? arm[k][2] : zig[k][2];
+      }
+      if (!set_list(L.box, ss, s.x, s.y, s.z, nn_site)) sc.abort = ERR_GEOM;
+      else wr(L, s.i, ss);
+    }
+    sc.nn = nn_site;
+    resolve_neighbours(L, sc, 4);
+    touched_list(s, sc, 4, res);
+  }
+  MSKG_SYNC();
+  if (sc.abort) {
+    if (lane == 0) res.err = sc.abort;
+    return;
+  }
+  if (sc.seq) {
+    if (lane == 0)
+      for (int i = 0; i < 4; i++) dep_neighbour(L, rtot, s, i, sc, sc.err[i]);
+  } else if (lane < 4) {
+    dep_neighbour(L, rtot, s, lane, sc, sc.err[lane]);
+  }
+  MSKG_SYNC();
+  if (lane == 0) {
+    int coor = sc.coor_old;
+    for (int i = 0; i < 4; i++) {
+      coor += sc.inc[i];
+      if (sc.err[i]) res.err = sc.err[i];
+    }
+    Site ss = rd(L, s.i);
+    if (coor != sc.coor_old) {
+      s_set_coor(ss, coor);
+      wr(L, s.i, ss);
+    }
+    refresh(L, rtot, s, ss, coor, sc.nn, res.err);
+  }
+}
+
+// The scan of Evaporation.f:82-128 / 225-259 that unfixes a list when a site
+// drops to coordination 1: the last occupied reciprocal neighbour (NBuff).
+__device__ bool nbuff_of(const Lattice &L, const P3 &p, const NL &old, int pos[3]) {
+  P3 q[4];
+  Site sq[4];
+  prefetch4(L, old, q, sq);
+  bool f = false;
   for (int j = 0; j < 4; j++) {
-    P3 q = entry(L, old, j);
-    Site sq = rd(L, q.i);
-    int kq = s_kind(sq);
-    if (kind_occupied(kq) && s_coor(sq) <= 3 && kq != KIND_WALL) {
-      if (recip(list_of(L, q, sq), p)) { nbuff[0] = old.x[j]; nbuff[1] = old.y[j]; nbuff[2] = old.z[j]; }
+    int kq = s_kind(sq[j]);
+    if (kind_occupied(kq) && s_coor(sq[j]) <= 3 && kq != KIND_WALL) {
+      if (recip(list_of(L, q[j], sq[j]), p)) {
+        pos[0] = old.x[j]; pos[1] = old.y[j]; pos[2] = old.z[j];
+        f = true;
+      }
     }
   }
+  return f;
+}
+__device__ inline bool one_entry_list(const Lattice &L, Site &sp, const P3 &p, const int pos[3]) {
+  NL nl;
   for (int j = 0; j < 4; j++) nl.x[j] = nl.y[j] = nl.z[j] = 0;
-  nl.x[0] = nbuff[0]; nl.y[0] = nbuff[1]; nl.z[0] = nbuff[2];
+  nl.x[0] = pos[0]; nl.y[0] = pos[1]; nl.z[0] = pos[2];
+  return set_list(L.box, sp, p.x, p.y, p.z, nl);
+}
+
+// Evaporation.f:36-208, the update of neighbour i of the atom leaving s.
+// NBuff is resolved afterwards (evap_nbuff) because the reference carries
+// it from one neighbour to the next.
+__device__ void evap_neighbour(const Lattice &L, double rtot, const P3 &s, int i, EvScratch &sc, int &err) {
+  const P3 p = sc.pts[i];
+  Site sp = rd(L, p.i);
+  int kind = s_kind(sp), coornn = s_coor(sp);
+  NL nnn = list_of(L, p, sp);
+  if (!kind_occupied(kind)) {
+    if (coornn == 1) {
+      if (recip(nnn, s)) {
+        Site z = rd(L, -1);
+        z.temp = sp.temp;  // EraseMC + LattCoo=0, keeping the temperature bits
+        wr(L, p.i, z);
+        set_rate(L, p, CLS_NONE, 0.0);
+      }
+    } else if (coornn == 2) {
+      if (recip(nnn, s)) {
+        int pos[3] = {0, 0, 0};
+        sc.unfixed[i] = 1;
+        sc.found[i] = nbuff_of(L, p, nnn, pos) ? 1 : 0;
+        for (int k = 0; k < 3; k++) sc.found_pos[i][k] = pos[k];
+        s_set_coor(sp, 1);
+        if (!one_entry_list(L, sp, p, pos)) { err = ERR_GEOM; return; }
+        wr(L, p.i, sp);
+        refresh(L, rtot, p, sp, 1, nnn, err);  // old list copy, Evaporation.f:121
+      }
+    } else if (coornn == 3) {
+      if (recip(nnn, s)) {
+        s_set_coor(sp, 2);
+        wr(L, p.i, sp);
+        refresh(L, rtot, p, sp, 2, nnn, err);
+      }
+    } else if (coornn == 4) {
+      if (recip(nnn, s)) {
+        s_set_coor(sp, 3);
+        wr(L, p.i, sp);
+        refresh(L, rtot, p, sp, 3, nnn, err);  // EraseVoid + AddMC
+      }
+    }
+  } else if (kind != KIND_WALL) {
+    if (coornn == 4) {
+      if (recip(nnn, s)) {
+        s_set_coor(sp, 3);
+        wr(L, p.i, sp);
+        refresh(L, rtot, p, sp, 3, nnn, err);  // EraseBulk + AddMC
+      }
+    } else {
+      if (recip(nnn, s)) {
+        coornn--;
+        s_set_coor(sp, coornn);
+        wr(L, p.i, sp);
+        refresh(L, rtot, p, sp, coornn, nnn, err);
+      }
+    }
+  } else {
+    sc.inc[i] = -1;  // wall neighbour
+  }
+}
+
+// The NBuff the reference carries across the neighbours: a neighbour that
+// found no reciprocal partner gets the last one found before it (or the
+// initial (0,0,0)).  Applied neighbour by neighbour, in order; `carried`
+// is the value carried out of the loop into the site's own unfix.
+__device__ void evap_nbuff_one(const Lattice &L, EvScratch &sc, int i, int carried[3], int &err) {
+  if (!sc.unfixed[i]) return;
+  if (sc.found[i]) {
+    for (int k = 0; k < 3; k++) carried[k] = sc.found_pos[i][k];
+  } else if (carried[0] || carried[1] || carried[2]) {
+    const P3 p = sc.pts[i];
+    Site sp = rd(L, p.i);
+    if (!one_entry_list(L, sp, p, carried)) { err = ERR_GEOM; return; }
+    wr(L, p.i, sp);
+  }
 }
 
 // Evaporation, mcevent/Evaporation.f:25-262.
-MSKG_HD void evaporation(const Lattice &L, double rtot, const P3 &s, EvResult &res) {
-  Site ss = rd(L, s.i);
-  int coor = s_coor(ss);
-  s_set_kind(ss, KIND_EMPTY);
-  wr(L, s.i, ss);
-  const NL nn_site = list_of(L, s, ss);
+__device__ void evaporation(const Lattice &L, double rtot, const P3 &s, EvResult &res, EvScratch &sc, int lane) {
+  if (lane == 0) {
+    sc.abort = 0;
+    Site ss = rd(L, s.i);
+    sc.coor_old = s_coor(ss);
+    s_set_kind(ss, KIND_EMPTY);
+    wr(L, s.i, ss);
+    sc.nn = list_of(L, s, ss);
+    resolve_neighbours(L, sc, 4);
+    touched_list(s, sc, 4, res);
+  }
+  MSKG_SYNC();
   int nbuff[3] = {0, 0, 0};
-  res.ntouched = 0;
-  res.touched[res.ntouched++] = s;
-  for (int i = 0; i < 4; i++) {
-    P3 p = entry(L, nn_site, i);
-    Site sp = rd(L, p.i);
-    res.touched[res.ntouched++] = p;
-    int kind = s_kind(sp), coornn = s_coor(sp);
-    NL nnn = list_of(L, p, sp);
-    if (!kind_occupied(kind)) {
-      if (coornn == 1) {
-        if (recip(nnn, s)) {
-          Site z = rd(L, -1);
-          z.temp = sp.temp;  // EraseMC + LattCoo=0, keeping the temperature bits
-          wr(L, p.i, z);
-          set_rate(L, p, CLS_NONE, 0.0);
-        }
-      } else if (coornn == 2) {
-        if (recip(nnn, s)) {
-          NL nl;
-          unfix(L, p, nnn, nl, nbuff);
-          s_set_coor(sp, 1);
-          if (!put_list(L, sp, p, nl, res)) return;
-          wr(L, p.i, sp);
-          refresh(L, rtot, p, sp, 1, nnn, res.err);  // old list copy, Evaporation.f:121
-        }
-      } else if (coornn == 3) {
-        if (recip(nnn, s)) {
-          s_set_coor(sp, 2);
-          wr(L, p.i, sp);
-          refresh(L, rtot, p, sp, 2, nnn, res.err);
-        }
-      } else if (coornn == 4) {
-        if (recip(nnn, s)) {
-          s_set_coor(sp, 3);
-          wr(L, p.i, sp);
-          refresh(L, rtot, p, sp, 3, nnn, res.err);  // EraseVoid + AddMC
-        }
+  if (sc.seq) {
+    if (lane == 0)
+      for (int i = 0; i < 4; i++) {  // in order, NBuff applied as each neighbour is visited
+        evap_neighbour(L, rtot, s, i, sc, sc.err[i]);
+        evap_nbuff_one(L, sc, i, nbuff, sc.err[i]);
       }
-    } else if (kind != KIND_WALL) {
-      if (coornn == 4) {
-        if (recip(nnn, s)) {
-          s_set_coor(sp, 3);
-          wr(L, p.i, sp);
-          refresh(L, rtot, p, sp, 3, nnn, res.err);  // EraseBulk + AddMC
-        }
-      } else {
-        if (recip(nnn, s)) {
-          coornn--;
-          s_set_coor(sp, coornn);
-          wr(L, p.i, sp);
-          refresh(L, rtot, p, sp, coornn, nnn, res.err);
-        }
-      }
-    } else {
-      coor--;  // wall neighbour
+  } else if (lane < 4) {
+    evap_neighbour(L, rtot, s, lane, sc, sc.err[lane]);
+  }
+  MSKG_SYNC();
+  if (lane == 0) {
+    int coor = sc.coor_old;
+    for (int i = 0; i < 4; i++) {
+      coor += sc.inc[i];
+      if (sc.err[i]) res.err = sc.err[i];
     }
-  }
-  ss = rd(L, s.i);
-  s_set_coor(ss, coor);
-  if (coor == 0) {  // EraseMC zeroes the record
-    for (int a = 0; a < 4; a++) ss.d[a][0] = ss.d[a][1] = ss.d[a][2] = 0;
-    ss.valid = 0;
+    if (!sc.seq)
+      for (int i = 0; i < 4; i++) evap_nbuff_one(L, sc, i, nbuff, res.err);
+    Site ss = rd(L, s.i);
+    s_set_coor(ss, coor);
+    if (coor == 0) {  // EraseMC zeroes the record
+      for (int a = 0; a < 4; a++) ss.d[a][0] = ss.d[a][1] = ss.d[a][2] = 0;
+      ss.valid = 0;
+      wr(L, s.i, ss);
+      set_rate(L, s, CLS_NONE, 0.0);
+      return;
+    }
     wr(L, s.i, ss);
-    set_rate(L, s, CLS_NONE, 0.0);
-    return;
+    int cls = 0;
+    double r = 0.0;
+    get_prob(L, rtot, s, ss, coor, sc.nn, cls, r, res.err);
+    if (coor == 1) {
+      int pos[3];
+      if (nbuff_of(L, s, sc.nn, pos))
+        for (int k = 0; k < 3; k++) nbuff[k] = pos[k];
+      if (!one_entry_list(L, ss, s, nbuff)) { res.err = ERR_GEOM; return; }
+      wr(L, s.i, ss);
+    }
+    set_rate(L, s, cls, r);
   }
-  wr(L, s.i, ss);
-  int cls = 0;
-  double r = 0.0;
-  get_prob(L, rtot, s, ss, coor, nn_site, cls, r, res.err);
-  if (coor == 1) {
-    NL nl;
-    unfix(L, s, nn_site, nl, nbuff);
-    if (!put_list(L, ss, s, nl, res)) return;
-    wr(L, s.i, ss);
-  }
-  set_rate(L, s, cls, r);
 }
 
 // absorption, mcevent/absorption.f:25-78 (slot = ListCov index 1..3).
-MSKG_HD void absorption(const Lattice &L, double rtot, const P3 &s, int slot, EvResult &res) {
-  Site ss = rd(L, s.i);
-  int coor = s_coor(ss);
-  s_set_kind(ss, KIND_COV0 + slot - 1);
-  wr(L, s.i, ss);
-  const NL nn_site = list_of(L, s, ss);
-  res.ntouched = 0;
-  res.touched[res.ntouched++] = s;
-  for (int i = 0; i < 4; i++) {
-    P3 p = entry(L, nn_site, i);
+__device__ void absorption(const Lattice &L, double rtot, const P3 &s, int slot, EvResult &res, EvScratch &sc,
+                           int lane) {
+  if (lane == 0) {
+    Site ss = rd(L, s.i);
+    sc.coor_old = s_coor(ss);
+    s_set_kind(ss, KIND_COV0 + slot - 1);
+    wr(L, s.i, ss);
+    sc.nn = list_of(L, s, ss);
+    resolve_neighbours(L, sc, 4);
+    touched_list(s, sc, 4, res);
+  }
+  MSKG_SYNC();
+  for (int i = sc.seq ? 0 : lane; i < 4; i += sc.seq ? 1 : 64) {
+    if (sc.seq && lane != 0) break;
+    const P3 p = sc.pts[i];
     Site sp = rd(L, p.i);
-    res.touched[res.ntouched++] = p;
     int k = s_kind(sp);
     if (kind_occupied(k) && k != KIND_WALL && s_coor(sp) <= 3) {
       NL nnn = list_of(L, p, sp);
-      if (recip(nnn, s)) refresh(L, rtot, p, sp, s_coor(sp), nnn, res.err);
+      if (recip(nnn, s)) refresh(L, rtot, p, sp, s_coor(sp), nnn, sc.err[i]);
     }
   }
-  refresh(L, rtot, s, ss, coor, nn_site, res.err);
+  MSKG_SYNC();
+  if (lane == 0) {
+    for (int i = 0; i < 4; i++)
+      if (sc.err[i]) res.err = sc.err[i];
+    Site ss = rd(L, s.i);
+    refresh(L, rtot, s, ss, sc.coor_old, sc.nn, res.err);
+  }
 }
 
 // desorption, mcevent/desorption.f:25-117.
-MSKG_HD void desorption(const Lattice &L, double rtot, const P3 &s, EvResult &res) {
-  Site ss = rd(L, s.i);
-  int coor = s_coor(ss);
-  s_set_kind(ss, KIND_EMPTY);
-  wr(L, s.i, ss);
-  const NL nn_site = list_of(L, s, ss);
-  res.ntouched = 0;
-  res.touched[res.ntouched++] = s;
-  int nloop = coor == 1 ? 1 : 4;
-  for (int i = 0; i < nloop; i++) {
-    P3 p = entry(L, nn_site, i);
+__device__ void desorption(const Lattice &L, double rtot, const P3 &s, EvResult &res, EvScratch &sc, int lane) {
+  if (lane == 0) {
+    Site ss = rd(L, s.i);
+    sc.coor_old = s_coor(ss);
+    s_set_kind(ss, KIND_EMPTY);
+    wr(L, s.i, ss);
+    sc.nn = list_of(L, s, ss);
+    const int nloop = sc.coor_old == 1 ? 1 : 4;
+    resolve_neighbours(L, sc, nloop);
+    touched_list(s, sc, nloop, res);
+  }
+  MSKG_SYNC();
+  const int nloop = sc.coor_old == 1 ? 1 : 4;
+  for (int i = sc.seq ? 0 : lane; i < nloop; i += sc.seq ? 1 : 64) {
+    if (sc.seq && lane != 0) break;
+    const P3 p = sc.pts[i];
     Site sp = rd(L, p.i);
-    res.touched[res.ntouched++] = p;
     int k = s_kind(sp);
     if (kind_occupied(k) && k != KIND_WALL) {
       NL nnn = list_of(L, p, sp);
-      if (recip(nnn, s)) refresh(L, rtot, p, sp, s_coor(sp), nnn, res.err);
+      if (recip(nnn, s)) refresh(L, rtot, p, sp, s_coor(sp), nnn, sc.err[i]);
     }
   }
-  refresh(L, rtot, s, ss, coor, nn_site, res.err);
+  MSKG_SYNC();
+  if (lane == 0) {
+    for (int i = 0; i < 4; i++)
+      if (sc.err[i]) res.err = sc.err[i];
+    Site ss = rd(L, s.i);
+    refresh(L, rtot, s, ss, sc.coor_old, sc.nn, res.err);
+  }
 }
 
 // The reference event an MC particle of class `cls` executes.  For an
@@ -507,15 +736,17 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   return r;
 }
 
-// Execute one event of type itrans at site s.
-MSKG_HD void execute(const Lattice &L, double rtot, const P3 &s, int itrans, const Choice &ch, EvResult &res) {
+// Execute one event of type itrans at site s.  Called by every lane of the
+// wave with the same arguments; res is complete on lane 0.
+__device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, const Choice &ch, EvResult &res,
+                        EvScratch &sc, int lane) {
   res.err = ERR_NONE;
   res.arm = -1;
   res.ntouched = 0;
-  if (itrans <= 3) deposition(L, rtot, s, itrans, ch, res);
-  else if (itrans == 4) evaporation(L, rtot, s, res);
-  else if (itrans == 5) desorption(L, rtot, s, res);
-  else absorption(L, rtot, s, itrans - 5, res);
+  if (itrans <= 3) deposition(L, rtot, s, itrans, ch, res, sc, lane);
+  else if (itrans == 4) evaporation(L, rtot, s, res, sc, lane);
+  else if (itrans == 5) desorption(L, rtot, s, res, sc, lane);
+  else absorption(L, rtot, s, itrans - 5, res, sc, lane);
 }
 
 }  // namespace mskg

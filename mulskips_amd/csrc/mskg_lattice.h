@@ -81,6 +81,7 @@ struct Tables {
   int ncrystal, ncov;
   int la_mode;          // laser-annealing rates (Get_Prob.f:127-149)
   double p0, tm, sigma, tflex;
+  int frozen_below_z;   // atoms with z below this never evaporate (0: off)
 };
 
 struct Box {

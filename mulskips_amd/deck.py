@@ -182,3 +182,30 @@ def write_start_dat(d, path):
     out += [f"{a} {b} {repr(float(v))}" for a, b, v in d.deso]
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
+
+
+def deck_from_tables(tables, init_st, lens, max_iter=10 ** 9, run_type="R", idum=1, sf=0.38583333333333333):
+    """A start.dat deck carrying the given rate tables (the inverse of
+    Deck.tables(): one evaporation line per species and neighbour composition
+    with 1..3 neighbours, as SetProbability.f:79-100 expects)."""
+    ncr, ncov = int(tables["n_crystal"]), int(tables["n_cov"])
+    if ncov:
+        raise ValueError("coverage species are not supported by deck_from_tables")
+    pe = np.asarray(tables["ptrans_e"])
+    evap = []
+    for s in range(1, ncr + 1):
+        ranges = [range(4)] * ncr
+        import itertools
+        for comb in itertools.product(*ranges):
+            if not 1 <= sum(comb) <= 3:
+                continue
+            n = list(comb) + [0] * (3 - ncr)
+            flat = (s - 1) + 3 * (n[0] + 4 * (n[1] + 4 * n[2]))
+            evap.append((s, "".join(str(c) for c in comb), float(pe[flat])))
+    if len(evap) != ncr * MAX_ELINES[(ncr, 0)] // ncr:
+        raise ValueError("unexpected number of evaporation lines")
+    pd = np.asarray(tables["ptrans_d"])
+    depo = [(s, c, float(pd[(s - 1) + 3 * (c - 1)])) for s in range(1, ncr + 1) for c in (1, 2, 3)]
+    lc = [int(v) for v in tables["list_crystal"][:ncr]]
+    return Deck(ncr, 0, lc, [], init_st, list(lens), float(tables["ptrans_zig"]), sf, "Iter", max_iter=max_iter,
+                out_molmol=max_iter, run_type=run_type, idum=idum, evap=evap, depo=depo)

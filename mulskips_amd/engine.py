@@ -32,7 +32,8 @@ class Engine:
     start.dat.
     """
 
-    def __init__(self, box, tables, seed=1, tile=0, tile_capacity=0, events_per_pass=0.0, la=None):
+    def __init__(self, box, tables, seed=1, tile=0, tile_capacity=0, events_per_pass=0.0, tau_policy=0,
+                 frozen_below_z=0, la=None):
         self.L = _lib.lib()
         lx, ly, lz = box
         self.box = (int(lx), int(ly), int(lz))
@@ -57,6 +58,8 @@ class Engine:
         cfg.tile = int(tile)
         cfg.tile_capacity = int(tile_capacity)
         cfg.events_per_pass = float(events_per_pass)
+        cfg.tau_policy = int(tau_policy)
+        cfg.frozen_below_z = int(frozen_below_z)
         self._cfg = cfg
         h = C.c_void_p()
         rc = self.L.mskg_create(C.byref(cfg), C.byref(h))
@@ -104,6 +107,21 @@ class Engine:
                 keep += [ev, pr]
                 args += [ptr(ev), ptr(pr)]
         self._check(self.L.mskg_load_state(self.h, ptr(lat), *args, float(kmc_time)))
+
+    def build_slab(self, mode, length):
+        """Flat (001) slab built on the device: mode 0 = SetSi (Si, walls),
+        1 = SetSiC3C (SiC)."""
+        self._check(self.L.mskg_build_slab(self.h, int(mode), int(length)))
+
+    def set_event_log(self, capacity):
+        self._check(self.L.mskg_set_event_log(self.h, int(capacity)))
+
+    def event_log(self):
+        n = C.c_int64()
+        self._check(self.L.mskg_get_event_log(self.h, None, 0, C.byref(n)))
+        log = np.zeros(max(n.value, 1), EVENT_DTYPE)
+        self._check(self.L.mskg_get_event_log(self.h, ptr(log), n.value, C.byref(n)))
+        return log[:n.value]
 
     def set_temperature(self, temp_k):
         t = np.ascontiguousarray(temp_k, np.int32)

@@ -67,6 +67,8 @@ struct orc_sim {
   double time; int64_t iter;
   int32_t count_crystal[3], count_crystal_old[3], count_cov[3];
   int last_arm;
+  int force_arm;  /* >= 0: IF_Arm_Choice returns this instead of comparing/drawing */
+  int frozen_below_z;  /* engine option restated: atoms below this z never evaporate */
   int64_t oob_reads;
   int64_t nonbond_arm, nonbond_nn;  /* geometry results that are not one of the 32 bond vectors */
   /* error handling: the reference STOPs; we unwind to the entry point */
@@ -392,6 +394,7 @@ static int is_bond_disp(const orc_sim *s, const int32_t a[3], const int32_t b[3]
 
 /* IF_Arm_Choice.f:21-56 */
 static int if_arm_choice(orc_sim *s, int32_t zig[3][3], int32_t arm[3][3]) {
+  if (s->force_arm >= 0) return s->force_arm;
   int carm = 0, czig = 0;
   for (int i = 0; i < 3; i++) {
     carm += ibits(LC(s, arm[i]), POS_COOR, LEN_COOR);
@@ -554,6 +557,9 @@ static void get_prob_impl(orc_sim *s, int occ, int nsic, int covind, int coor, c
     int c1 = s->ncov >= 1 ? covnn[0] : 0, c2 = s->ncov >= 2 ? covnn[1] : 0, c3 = s->ncov >= 3 ? covnn[2] : 0;
     *prob = pev(s, nsic, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
   }
+  /* frozen substrate: the atom stays an MC particle (so the reference's
+   * list bookkeeping is unchanged) but is never picked */
+  if (*iev == 4 && site[2] < s->frozen_below_z) *prob = 1e-300;
 }
 static void get_prob(orc_sim *s, int occ, int nsic, int covind, int coor, const int32_t site[3],
                      const int32_t nn[4][3], int *iev, double *prob) {
@@ -1598,6 +1604,20 @@ int orc_step(orc_sim *s, orc_event *ev) {
   return 0;
 }
 
+int orc_apply(orc_sim *s, const int32_t site[3], int32_t itrans, int32_t arm) {
+  if (setjmp(s->jb)) { s->have_jb = 0; s->force_arm = -1; return 2; }
+  s->have_jb = 1;
+  int32_t ind = LI(s, site);
+  if (ind < 1 || ind > s->n_ad || !eq3(s->ad[ind].xyz, site))
+    fail(s, "orc_apply: (%d,%d,%d) is not an MC particle", site[0], site[1], site[2]);
+  s->force_arm = arm;
+  do_execute(s, ind, itrans);
+  s->force_arm = -1;
+  s->iter++;
+  s->have_jb = 0;
+  return 0;
+}
+
 int orc_main(orc_sim *s, const char *outdir, orc_event *log, int64_t log_cap, int64_t *log_n, int *stop_reason) {
   volatile int64_t nlog = 0;
   int64_t framecounter = 0;
@@ -1664,6 +1684,7 @@ orc_sim *orc_create(int lx, int ly, int lz) {
   if (!s->latt || !s->lind || !s->ad || !s->vd || !s->abuf) { orc_destroy(s); return NULL; }
   s->time = 0.0;
   s->iter = 1;
+  s->force_arm = -1;
   strcpy(s->run_type, "R");
   strcpy(s->exit_strategy, "Iter");
   s->max_iter = 2100000000;
@@ -1693,6 +1714,21 @@ void orc_anomalies(const orc_sim *s, int64_t out[3]) {
   out[0] = s->nonbond_arm;
   out[1] = s->nonbond_nn;
   out[2] = s->oob_reads;
+}
+
+int orc_set_frozen(orc_sim *s, int z) {
+  if (setjmp(s->jb)) { s->have_jb = 0; return 2; }
+  s->have_jb = 1;
+  s->frozen_below_z = z;
+  /* atoms already listed as MC particles below z keep their place but get
+   * the vanishing rate Get_Prob now gives them */
+  for (int32_t i = 1; i <= s->n_ad; i++)
+    if (s->ad[i].xyz[2] < z && s->ad[i].ev == 4) {
+      s->ad[i].prob = 1e-300;
+      update_tree(s, i, 1e-300);
+    }
+  s->have_jb = 0;
+  return 0;
 }
 
 int32_t *orc_lattcoo(orc_sim *s) { return s->latt; }

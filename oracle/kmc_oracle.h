@@ -84,6 +84,19 @@ int orc_main(orc_sim *s, const char *outdir, orc_event *log, int64_t log_cap,
  * executed event, 1 for the early stop (NumAdAtom<=20), 2 on error. */
 int orc_step(orc_sim *s, orc_event *ev);
 
+/* Execute one given event (Ind_Event itrans at the MC particle at site, with
+ * the stacking choice forced to arm when >= 0) with the reference's update
+ * rules: replays an event log produced elsewhere, e.g. by the engine's
+ * parallel run.  Returns 0, or 2 if the site is not an MC particle. */
+int orc_apply(orc_sim *s, const int32_t site[3], int32_t itrans, int32_t arm);
+
+/* The engine's frozen-substrate option: atoms with z < z never evaporate.
+ * Here they stay MC particles with the vanishing rate 1e-300, so the
+ * reference's list bookkeeping (which list Get_Prob reads a neighbour's
+ * NextNXYZ from) is unchanged; the engine marks them as not active.
+ * No reference counterpart. */
+int orc_set_frozen(orc_sim *s, int z);
+
 /* Write the main xyz frame of WriteMolMolXYZFile.f (and its SiC
  * wrong-site side effect on LattCoo) exactly as the reference does. */
 int orc_write_frame(orc_sim *s, const char *xyz_path, const char *d_path,

@@ -42,6 +42,8 @@ def _load():
         L.orc_load_deck.argtypes = [P, C.c_char_p]
         L.orc_main.argtypes = [P, C.c_char_p, P, I64, P, P]
         L.orc_step.argtypes = [P, P]
+        L.orc_apply.argtypes = [P, P, I, I]
+        L.orc_set_frozen.argtypes = [P, I]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
         L.orc_lattcoo.restype = C.POINTER(C.c_int32)
         L.orc_lattcoo.argtypes = [P]
@@ -121,6 +123,17 @@ class Oracle:
             self._check(rc, "orc_step")
             done += 1
         return log[:done]
+
+    def set_frozen(self, z):
+        self._check(self.L.orc_set_frozen(self.h, int(z)), "orc_set_frozen")
+
+    def apply_log(self, log):
+        """Execute a logged event sequence (e.g. the engine's) in order."""
+        for k, ev in enumerate(log):
+            site = np.ascontiguousarray(ev["site"], np.int32)
+            rc = self.L.orc_apply(self.h, _p(site), int(ev["itrans"]), int(ev["arm"]))
+            if rc:
+                raise RuntimeError(f"event {k}: {self.L.orc_last_error(self.h).decode()}")
 
     def lattcoo(self):
         lx, ly, lz = self.dims

@@ -1,0 +1,91 @@
+"""GPU: the parallel synchronous-sublattice engine and the device builder.
+
+* the device-built slab (mskg_build_slab) equals the reference builders
+  SetSi / SetSiC3C as restated by the oracle, site by site;
+* mskg_run's own event log, replayed through the oracle (reference update
+  rules, serial), reproduces the engine's final lattice bit for bit: the
+  parallel execution of concurrent tiles is a valid serial KMC history;
+* the rates the engine keeps after a run equal a fresh Get_Prob sweep except
+  for the reference's stale-rate cases (placeholder / isolated atom).
+"""
+import numpy as np
+import pytest
+
+from mulskips_amd.engine import Engine
+from oracle.oracle import Oracle
+from tests.helpers import list_map, oracle_state, parity_mask, small_deck
+
+pytestmark = pytest.mark.gpu
+
+CASES = {  # name: (deck, mode, box, lens)
+    "SiC": ("surface", 1, (96, 96, 144), (72, 72, 72)),
+    "Si": ("Si-SL", 0, (96, 96, 144), (72, 120, 120)),
+}
+
+
+def compare(o, eng, box, classes_only_isolated=True):
+    lat_o, mc_o, bulk_o, void_o = oracle_state(o)
+    lat_e, mc_e, bulk_e, void_e = eng.store_state(lattcoo=lat_o.copy())
+    m = parity_mask(box)
+    bad = np.nonzero((lat_o != lat_e) & m)
+    assert bad[0].size == 0, f"LattCoo differs at {bad[0].size} points"
+    for name, a, b in (("MC", mc_o, mc_e), ("bulk", bulk_o, bulk_e), ("void", void_o, void_e)):
+        ma, mb = list_map(a), list_map(b)
+        assert ma.keys() == mb.keys(), f"{name} membership differs at {len(ma.keys() ^ mb.keys())} sites"
+        assert all(ma[k] == mb[k] for k in ma), f"{name} neighbour lists differ"
+    return mc_o
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_device_slab_equals_reference_builder(tmp_path, case):
+    deck, mode, box, lens = CASES[case]
+    path, d = small_deck(tmp_path, deck, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    eng = Engine(box, d.tables())
+    eng.build_slab(mode, lens[0] - lens[0] % 24)
+    mc_o = compare(o, eng, box)
+    cls, rate = eng.site_rates(mc_o["xyz"])
+    np.testing.assert_allclose(rate, mc_o["prob"], rtol=1e-6)
+    eng.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("tile,epp", [(24, 1.0), (16, 4.0)])
+def test_parallel_run_is_a_serial_history(tmp_path, case, tile, epp):
+    deck, mode, box, lens = CASES[case]
+    path, d = small_deck(tmp_path, deck, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    eng = Engine(box, d.tables(), seed=11, tile=tile, events_per_pass=epp)
+    eng.build_slab(mode, lens[0] - lens[0] % 24)
+    eng.set_event_log(400000)
+    n, t, stop = eng.run(100000)
+    log = eng.event_log()
+    print(f"[{case} T={tile}] {n} events, {len(log)} logged, stop={stop}", flush=True)
+    assert len(log) == n and n > 1000 and stop in (0, 1)
+    o.apply_log(log)
+    mc_o = compare(o, eng, box)
+    cls, rate = eng.site_rates(mc_o["xyz"])
+    keep = cls != 6
+    np.testing.assert_allclose(rate[keep], mc_o["prob"][keep], rtol=1e-6)
+    counts, clock, lists = eng.counters()
+    assert counts.sum() == n
+    assert lists == o.counts()
+    eng.close()
+
+
+def test_rates_after_run_match_fresh_sweep(tmp_path):
+    deck, mode, box, lens = CASES["SiC"]
+    path, d = small_deck(tmp_path, deck, lens)
+    eng = Engine(box, d.tables(), seed=5)
+    eng.build_slab(mode, lens[0] - lens[0] % 24)
+    eng.run(50000)
+    _, mc, _, _ = eng.store_state()
+    cls0, r0 = eng.site_rates(mc["xyz"])
+    eng.refresh_rates()
+    cls1, r1 = eng.site_rates(mc["xyz"])
+    stale = np.isin(cls0, (2, 6)) | np.isin(cls1, (2, 6))
+    np.testing.assert_array_equal(cls0[~stale], cls1[~stale])
+    np.testing.assert_allclose(r0[~stale], r1[~stale], rtol=1e-6)
+    eng.close()
