@@ -163,8 +163,22 @@ MSKG_HD double pe_of(const Tables &t, int sp, int a, int b, int c, int d, int e,
 // passes a stale list copy).  rtot is the total rate R used by the
 // isolated-atom rule.  No random number is drawn: the deposition species /
 // absorbed coverage is drawn when the event executes (class CLS_ADSORB).
+// Records that are newer than HBM (updated by this event on another lane):
+// reads of these points take the given record instead.
+struct Ovr {
+  int n;
+  int64_t idx[5];
+  Site rec[5];
+};
+MSKG_HD Site rd_o(const Lattice &L, int64_t i, const Ovr *o) {
+  if (o && i >= 0)
+    for (int k = 0; k < o->n; k++)
+      if (o->idx[k] == i) return o->rec[k];
+  return rd(L, i);
+}
+
 MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &cls,
-                      double &rate, int &err) {
+                      double &rate, int &err, const Ovr *ovr = nullptr) {
   const Tables &t = *L.tab;
   int kind = s_kind(sp);
   int temperature = 0;
@@ -182,7 +196,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   if (kind == KIND_EMPTY) {  // :155-214
     if (coor >= 1 && coor <= 3) {
       bool ok = true;
-      if (coor == 1) ok = s_coor(rd(L, entry(L, nn, 0).i)) >= 2;
+      if (coor == 1) ok = s_coor(rd_o(L, entry(L, nn, 0).i, ovr)) >= 2;
       if (ok) {
         double pdep = pd_of(t, 1, coor, temperature) + pd_of(t, 2, coor, temperature) + pd_of(t, 3, coor, temperature);
         double pabs = t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor];
@@ -208,7 +222,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   P3 qs[4];
   Site sqs[4];
   for (int i = 0; i < 4; i++) qs[i] = entry(L, nn, i);
-  for (int i = 0; i < 4; i++) sqs[i] = rd(L, qs[i].i);  // four independent gathers in flight
+  for (int i = 0; i < 4; i++) sqs[i] = rd_o(L, qs[i].i, ovr);  // four independent gathers in flight
   for (int i = 0; i < 4; i++) {
     const P3 &q = qs[i];
     const Site &sq = sqs[i];
@@ -262,11 +276,14 @@ struct EvResult {
 
 // Get_Prob followed by UpdateTree / AddMC / EraseMC+Add{Bulk,Void}: a
 // zero rate takes the site out of the MC population, its list stays.
-MSKG_HD void refresh(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &err) {
+MSKG_HD void refresh(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &err,
+                     const Ovr *ovr = nullptr, float *out_rate = nullptr, uint8_t *out_cls = nullptr) {
   int cls = 0;
   double r = 0.0;
-  get_prob(L, rtot, p, sp, coor, nn, cls, r, err);
+  get_prob(L, rtot, p, sp, coor, nn, cls, r, err, ovr);
   set_rate(L, p, cls, r);
+  if (out_rate) *out_rate = cls ? (float)r : 0.0f;
+  if (out_cls) *out_cls = (uint8_t)cls;
 }
 
 MSKG_HD bool put_list(const Lattice &L, Site &s, const P3 &p, const NL &l, EvResult &res) {
