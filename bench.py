@@ -31,6 +31,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 SWEEP_BYTES_PER_POINT = 21
+PMC_PROFILE = os.path.join(ROOT, "profiles", "pmc_rate_sweep_sic512.md")
+
+
+def measured_traffic_per_point():
+    """HBM bytes per lattice point of the sweep kernel from the committed
+    counter profile (FETCH_SIZE + WRITE_SIZE, calibrated against a copy)."""
+    try:
+        with open(PMC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_point ="):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -122,6 +136,7 @@ def main():
     sweep_ms = sorted(sweeps)[len(sweeps) // 2]
     npts = tm["sweep_points"]
     achieved = npts * SWEEP_BYTES_PER_POINT / (sweep_ms * 1e-3) / 1e9
+    tpp = measured_traffic_per_point()
     out = {
         "metric": "KMC events/s at 512^3 SiC (1 GPU)",
         "value": total / wall,
@@ -141,7 +156,10 @@ def main():
                    "box": list(box), "tile": a.tile, "colours": 8, "tau_policy": a.tau_policy,
                    "events_per_pass": a.events_per_pass, "events_per_step": eps},
         "roofline": {"kernel": "k_rate_sweep (rate-update sweep)", "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": None if tpp is None else tpp * npts,
+                     "traffic_source": "profiles/pmc_rate_sweep_sic512.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, "
+                                       "per point, times the points of this launch)",
                      "points": npts, "bytes_per_point": SWEEP_BYTES_PER_POINT, "ms": sweep_ms},
         "engine": {"device_ms": dev_ms, "events": total, "kmc_time_s": t_kmc,
                    "counts": [int(c) for c in counts]},
