@@ -69,7 +69,13 @@ typedef struct {
   int32_t frozen_below_z;   /* engine option without a reference counterpart: atoms with
                                z < frozen_below_z never evaporate (a frozen substrate);
                                0 = off, the reference behaviour */
-  int32_t reserved[6];
+  const double *second_factor; /* engine option without a reference counterpart: 13 factors
+                               f[n2] multiplying the evaporation rate of an atom with 1..3
+                               neighbours, n2 = occupied points among its 12 second-neighbour
+                               positions (+-6,+-6,0 and permutations), e.g. exp(-n2*E2/kT)
+                               for a second-neighbour bond energy E2; NULL = off.  Needs
+                               tile >= 24. */
+  int32_t reserved[4];
 } mskg_config;
 
 /* One logged event, as written by the reference loop instrumented at

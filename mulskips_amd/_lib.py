@@ -32,7 +32,7 @@ class Config(C.Structure):
         ("la_mode", C.c_int32), ("tm", C.c_double), ("p0", C.c_double), ("sigma", C.c_double),
         ("tflex", C.c_double), ("seed", C.c_uint64), ("tile", C.c_int32), ("tile_capacity", C.c_int32),
         ("events_per_pass", C.c_double), ("tau_policy", C.c_int32),
-        ("frozen_below_z", C.c_int32), ("reserved", C.c_int32 * 6),
+        ("frozen_below_z", C.c_int32), ("second_factor", C.c_void_p), ("reserved", C.c_int32 * 4),
     ]
 
 

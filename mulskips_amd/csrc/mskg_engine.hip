@@ -144,11 +144,15 @@ __device__ inline int tile_local(const TileGeom &g, int T, const P3 &p) {
 // Every point: an MC particle is an empty or coverage site with coordination
 // 1..3 or an atom with coordination <= 3 (Get_Prob.f:54-331 is evaluated for
 // it); everything else gets class 0.
-__global__ void __launch_bounds__(256) k_rate_sweep(Lattice L, double rtot, int *err) {
+// With atoms_only, only atoms of coordination 1..3 are re-evaluated and every
+// other point keeps its rate (used after the device slab builder, whose
+// atoms need the second-neighbour factor).
+__global__ void __launch_bounds__(256) k_rate_sweep(Lattice L, double rtot, int *err, int atoms_only) {
   const int64_t n = L.box.npts;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const Site sp = L.site[p];
     int kind = s_kind(sp), coor = s_coor(sp);
+    if (atoms_only && !(kind_species(kind) && coor >= 1 && coor <= 3)) continue;
     bool mc = (kind == KIND_EMPTY && coor >= 1 && coor <= 3) || (kind >= KIND_COV0 && coor <= 3) ||
               (kind_species(kind) && coor <= 3);
     int cls = 0;
@@ -283,9 +287,10 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
   float *lr = (float *)(lk + cap);                // rates
   uint8_t *lc = (uint8_t *)(lr + cap);            // classes
   __shared__ int s_n, s_stop, s_nt;
-  __shared__ int s_tk[5];
-  __shared__ float s_tr[5];
-  __shared__ uint8_t s_tc[5];
+  __shared__ P3 s_tp[MAX_TOUCHED];
+  __shared__ int s_tk[MAX_TOUCHED];
+  __shared__ float s_tr[MAX_TOUCHED];
+  __shared__ uint8_t s_tc[MAX_TOUCHED];
   __shared__ EvScratch sc;
 
   RunState *st = P.st;
@@ -413,16 +418,8 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
         }
         int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
         if (slot >= 0 && slot < 8) cnt[slot]++;
-        int nt = 0;
-        for (int q = 0; q < res.ntouched; q++) {
-          int kl = tile_local(g, L.tile, res.touched[q]);
-          if (kl < 0) continue;
-          s_tk[nt] = kl;
-          s_tr[nt] = L.rate[res.touched[q].i];
-          s_tc[nt] = L.cls[res.touched[q].i];
-          nt++;
-        }
-        s_nt = nt;
+        for (int q = 0; q < res.ntouched; q++) s_tp[q] = res.touched[q];
+        s_nt = res.ntouched;
         s_stop = err;
       }
     }
@@ -433,10 +430,22 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
       err = s_stop;
       break;
     }
-    // refresh the on-chip list for touched points inside the tile
+    // the touched points' new rates, one lane each
     const int nt = s_nt;
+    if (lane < nt) {
+      const P3 tp = s_tp[lane];
+      const int kl = tile_local(g, L.tile, tp);
+      s_tk[lane] = kl;
+      if (kl >= 0) {
+        s_tr[lane] = L.rate[tp.i];
+        s_tc[lane] = L.cls[tp.i];
+      }
+    }
+    __syncthreads();
+    // refresh the on-chip list for touched points inside the tile
     for (int q = 0; q < nt; q++) {
       const int kl = s_tk[q];
+      if (kl < 0) continue;
       int found = -1;
       for (int base = 0; base < n; base += WAVE) {
         int j = base + lane;
@@ -713,15 +722,22 @@ static Lattice make_lattice(mskg_engine *e) {
   return L;
 }
 
-static int choose_tile(int lx, int ly, int want) {
+// Same-colour tiles are one tile apart, so two concurrent events are at
+// least tile+1 grid units apart along some axis.  An event writes records
+// and rates at most 3 units from its site per axis (6 with the
+// second-neighbour term, which refreshes the second neighbours) and reads at
+// most 6 (12) units away, so a tile edge of 16 (24) keeps what one event
+// writes out of what any other reads, with room for the rare neighbour
+// entries that are not bond vectors.
+static int choose_tile(int lx, int ly, int want, int min_tile) {
   auto ok = [&](int t) {
-    return t >= 16 && t % 2 == 0 && lx % t == 0 && ly % t == 0 && (lx / t) % 2 == 0 && (ly / t) % 2 == 0;
+    return t >= min_tile && t % 2 == 0 && lx % t == 0 && ly % t == 0 && (lx / t) % 2 == 0 && (ly / t) % 2 == 0;
   };
   if (want) return ok(want) ? want : -1;
   const int pref[] = {24, 20, 18, 28, 30, 32, 36, 40, 42, 48, 16, 22, 26, 34, 38, 44, 46};
   for (int t : pref)
     if (ok(t)) return t;
-  for (int t = 16; t <= std::min(lx, ly) / 2; t += 2)
+  for (int t = min_tile; t <= std::min(lx, ly) / 2; t += 2)
     if (ok(t)) return t;
   return -1;
 }
@@ -802,13 +818,21 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   e->tab.sigma = cfg->sigma;
   e->tab.tflex = cfg->tflex;
   e->tab.frozen_below_z = cfg->frozen_below_z;
+  e->tab.use_f2 = cfg->second_factor != nullptr;
+  if (cfg->second_factor)
+    for (int i = 0; i < 13; i++) {
+      e->tab.f2[i] = cfg->second_factor[i];
+      if (!(e->tab.f2[i] > 0.0) || !std::isfinite(e->tab.f2[i]))
+        return fail_(e, MSKG_EINVAL, "second_factor[%d] must be positive and finite", i);
+    }
   build_geometry(e->geo);
-  e->tile = choose_tile(cfg->len_x, cfg->len_y, cfg->tile);
+  const int min_tile = e->tab.use_f2 ? 24 : 16;
+  e->tile = choose_tile(cfg->len_x, cfg->len_y, cfg->tile, min_tile);
   if (e->tile < 0)
     return fail_(e, MSKG_EINVAL,
-                 "no valid tile edge for box %d x %d (requested %d): need an even tile >= 16 with an even "
+                 "no valid tile edge for box %d x %d (requested %d): need an even tile >= %d with an even "
                  "number of tiles along x and y",
-                 cfg->len_x, cfg->len_y, cfg->tile);
+                 cfg->len_x, cfg->len_y, cfg->tile, min_tile);
   e->cap = cfg->tile_capacity > 0 ? cfg->tile_capacity : 1024;
   e->capg = e->cap;
   e->ntx = cfg->len_x / e->tile;
@@ -970,7 +994,7 @@ int mskg_refresh_rates(mskg_engine *e) {
   e->lists_dirty = true;
   int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
   HIPCHK(e, hipEventRecord(e->ev0, e->stream));
-  hipLaunchKernelGGL(k_rate_sweep, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err);
+  hipLaunchKernelGGL(k_rate_sweep, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
   HIPCHK(e, hipGetLastError());
   HIPCHK(e, hipEventRecord(e->ev1, e->stream));
   int err = 0;
@@ -1241,6 +1265,11 @@ int mskg_build_slab(mskg_engine *e, int32_t mode, int32_t len) {
   int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
   hipLaunchKernelGGL(k_build_slab, dim3(blocks), dim3(256), 0, e->stream, L, len, mode);
   HIPCHK(e, hipGetLastError());
+  if (e->tab.use_f2) {  // the surface atoms' rates take their second-neighbour factor
+    HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
+    hipLaunchKernelGGL(k_rate_sweep, dim3(blocks), dim3(256), 0, e->stream, L, 0.0, e->d_err, 1);
+    HIPCHK(e, hipGetLastError());
+  }
   // the total rate of the initial population seeds the isolated-atom rule
   RunState st;
   memset(&st, 0, sizeof st);

@@ -177,6 +177,31 @@ MSKG_HD Site rd_o(const Lattice &L, int64_t i, const Ovr *o) {
   return rd(L, i);
 }
 
+// The 12 second-neighbour positions of a diamond super-lattice site (the
+// fcc shell: (+-6,+-6,0) and permutations), for the optional
+// second-neighbour energy term; no reference counterpart.
+MSKG_HD P3 second_pos(const Lattice &L, const P3 &p, int k) {
+  const int a = (k & 2) ? -6 : 6, b = (k & 1) ? -6 : 6, g = k >> 2;
+  int dx = g == 2 ? 0 : a, dy = g == 0 ? b : (g == 1 ? 0 : a), dz = g == 0 ? 0 : b;
+  int x = p.x + dx, y = p.y + dy;
+  if (x >= L.box.lx) x -= L.box.lx;
+  if (x < 0) x += L.box.lx;
+  if (y >= L.box.ly) y -= L.box.ly;
+  if (y < 0) y += L.box.ly;
+  return at(L, x, y, p.z + dz);
+}
+// Occupied (non-wall) points among them.
+MSKG_HD int second_count(const Lattice &L, const P3 &p, const Ovr *ovr) {
+  Site q[12];
+  for (int k = 0; k < 12; k++) q[k] = rd_o(L, second_pos(L, p, k).i, ovr);  // twelve gathers in flight
+  int n = 0;
+  for (int k = 0; k < 12; k++) {
+    int kq = s_kind(q[k]);
+    if (kind_occupied(kq) && kq != KIND_WALL) n++;
+  }
+  return n;
+}
+
 MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &cls,
                       double &rate, int &err, const Ovr *ovr = nullptr) {
   const Tables &t = *L.tab;
@@ -257,6 +282,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
     int c1 = t.ncov >= 1 ? covnn[0] : 0, c2 = t.ncov >= 2 ? covnn[1] : 0, c3 = t.ncov >= 3 ? covnn[2] : 0;
     cls = CLS_EVAP;
     rate = pe_of(t, kind, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
+    if (t.use_f2) rate *= t.f2[second_count(L, p, ovr)];
   }
 }
 
@@ -267,8 +293,11 @@ struct Choice {
   double u_arm;  // uniform used when the arm/zig coordination sums tie
 };
 
+constexpr int MAX_TOUCHED = 17;
 struct EvResult {
-  P3 touched[5];  // the site and its neighbours: every point whose rate may have changed
+  P3 touched[MAX_TOUCHED];  // the site, its neighbours and (second-neighbour term) the
+                            // evaporating atoms on its second-neighbour positions: every
+                            // point whose rate may have changed
   int ntouched;
   int arm;        // stacking choice made (-1 none)
   int err;
@@ -311,6 +340,8 @@ struct EvScratch {
   int found_pos[4][3];
   int seq;          // visit the neighbours in order on lane 0
   int abort;
+  P3 sec[12];       // second-neighbour refresh: the point refreshed by lane k (i = -1: none)
+  int err2[12];
 };
 
 #define MSKG_SYNC() __syncthreads()
@@ -753,6 +784,21 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   return r;
 }
 
+// Second-neighbour term: after a change of occupancy at s, the evaporating
+// atom (coordination 1..3) on second-neighbour position k of s, if any,
+// gets a fresh rate from the final state of the event.
+__device__ void second_refresh(const Lattice &L, double rtot, const P3 &s, int k, EvScratch &sc) {
+  const P3 q = second_pos(L, s, k);
+  sc.sec[k].i = -1;
+  sc.err2[k] = 0;
+  if (q.i < 0) return;
+  const Site sq = rd(L, q.i);
+  const int kq = s_kind(sq), cq = s_coor(sq);
+  if (!kind_occupied(kq) || kq == KIND_WALL || cq < 1 || cq > 3) return;
+  refresh(L, rtot, q, sq, cq, list_of(L, q, sq), sc.err2[k]);
+  sc.sec[k] = q;
+}
+
 // Execute one event of type itrans at site s.  Called by every lane of the
 // wave with the same arguments; res is complete on lane 0.
 __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, const Choice &ch, EvResult &res,
@@ -764,6 +810,20 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
   else if (itrans == 4) evaporation(L, rtot, s, res, sc, lane);
   else if (itrans == 5) desorption(L, rtot, s, res, sc, lane);
   else absorption(L, rtot, s, itrans - 5, res, sc, lane);
+  if (L.tab->use_f2 && itrans <= 4) {
+    MSKG_SYNC();  // every lane is past the event's own updates
+    if (lane == 0) sc.abort = res.err;
+    MSKG_SYNC();
+    if (!sc.abort) {
+      if (lane < 12) second_refresh(L, rtot, s, lane, sc);
+      MSKG_SYNC();
+      if (lane == 0)
+        for (int k = 0; k < 12; k++) {
+          if (sc.err2[k]) res.err = sc.err2[k];
+          if (sc.sec[k].i >= 0) res.touched[res.ntouched++] = sc.sec[k];
+        }
+    }
+  }
 }
 
 }  // namespace mskg

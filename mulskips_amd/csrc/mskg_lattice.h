@@ -82,6 +82,8 @@ struct Tables {
   int la_mode;          // laser-annealing rates (Get_Prob.f:127-149)
   double p0, tm, sigma, tflex;
   int frozen_below_z;   // atoms with z below this never evaporate (0: off)
+  int use_f2;           // second-neighbour factor on evaporation rates
+  double f2[13];        // by the number of occupied second-neighbour positions
 };
 
 struct Box {

@@ -60,6 +60,11 @@ class Engine:
         cfg.events_per_pass = float(events_per_pass)
         cfg.tau_policy = int(tau_policy)
         cfg.frozen_below_z = int(frozen_below_z)
+        if tables.get("second_factor") is not None:
+            self._arrays["second_factor"] = np.ascontiguousarray(tables["second_factor"], np.float64)
+            if self._arrays["second_factor"].shape != (13,):
+                raise ValueError("second_factor needs 13 values (0..12 occupied second neighbours)")
+            cfg.second_factor = self._arrays["second_factor"].ctypes.data
         self._cfg = cfg
         h = C.c_void_p()
         rc = self.L.mskg_create(C.byref(cfg), C.byref(h))

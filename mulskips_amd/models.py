@@ -50,11 +50,23 @@ def pair_energy_tables(pair, fluxes, temperature, prefactor=1e13, list_crystal=(
                 n_crystal=ns, n_cov=0, list_crystal=lc, list_cov=np.zeros(3, np.int32))
 
 
-def sic_c2_tables(temperature=1600.0, e_sisi=0.6, e_sic=1.1, e_cc=0.9, flux_si=2.0, flux_c=1.8, prefactor=1e13):
+def second_neighbour_factors(e2, temperature):
+    """exp(-n2*E2/kT) for n2 = 0..12 occupied second-neighbour positions: the
+    engine's second-neighbour energy term (E2 in eV per occupied position)."""
+    return np.exp(-np.arange(13) * e2 / (KB_EV * temperature))
+
+
+def sic_c2_tables(temperature=1600.0, e_sisi=0.6, e_sic=1.1, e_cc=0.9, flux_si=2.0, flux_c=1.8, prefactor=1e13,
+                  e2=0.05):
     """3C-SiC CVD benchmark model: Si-Si 0.6, Si-C 1.1, C-C 0.9 eV pair
-    energies, Si and C fluxes 2 and 1.8 ML/s, T = 1600 K."""
+    energies over the first neighbours, 0.05 eV per occupied
+    second-neighbour position, Si and C fluxes 2 and 1.8 ML/s, T = 1600 K.
+    e2=None leaves the second-neighbour term out (the reference's model
+    class)."""
     pair = [[e_sisi, e_sic], [e_sic, e_cc]]
-    return pair_energy_tables(pair, [flux_si, flux_c], temperature, prefactor, (14, 6))
+    t = pair_energy_tables(pair, [flux_si, flux_c], temperature, prefactor, (14, 6))
+    t["second_factor"] = None if e2 is None else second_neighbour_factors(e2, temperature)
+    return t
 
 
 def si_c1_tables(temperature=1100.0, e_bond=0.55, flux=1.0, prefactor=1e13):

@@ -69,6 +69,8 @@ struct orc_sim {
   int last_arm;
   int force_arm;  /* >= 0: IF_Arm_Choice returns this instead of comparing/drawing */
   int frozen_below_z;  /* engine option restated: atoms below this z never evaporate */
+  int use_f2; double f2[13];  /* engine option restated: evaporation rate factor by the number
+                                 of occupied second-neighbour positions */
   int64_t oob_reads;
   int64_t nonbond_arm, nonbond_nn;  /* geometry results that are not one of the 32 bond vectors */
   /* error handling: the reference STOPs; we unwind to the entry point */
@@ -456,6 +458,30 @@ static int index_event_abs(orc_sim *s, int coor, double ptmp, double pdepo) {
   return r;
 }
 
+/* The 12 second-neighbour positions of a site of the diamond super-lattice
+ * (the fcc shell, (+-6,+-6,0) and permutations in grid units).  No reference
+ * counterpart: the engine's optional second-neighbour energy term counts
+ * the occupied (non-wall) points among them. */
+static const int SECOND[12][3] = {
+  {6, 6, 0}, {6, -6, 0}, {-6, 6, 0}, {-6, -6, 0}, {6, 0, 6}, {6, 0, -6},
+  {-6, 0, 6}, {-6, 0, -6}, {0, 6, 6}, {0, 6, -6}, {0, -6, 6}, {0, -6, -6}};
+static void second_pos(const orc_sim *s, const int32_t site[3], int k, int32_t q[3]) {
+  q[0] = ((site[0] + SECOND[k][0]) % s->lx + s->lx) % s->lx;
+  q[1] = ((site[1] + SECOND[k][1]) % s->ly + s->ly) % s->ly;
+  q[2] = site[2] + SECOND[k][2];
+}
+static int second_count(orc_sim *s, const int32_t site[3]) {
+  int n = 0;
+  for (int k = 0; k < 12; k++) {
+    int32_t q[3];
+    second_pos(s, site, k, q);
+    if (q[2] < 0 || q[2] > s->lz) continue;
+    int32_t w = s->latt[gidx(s, q[0], q[1], q[2])];
+    if (ibits(w, POS_OCC, LEN_OCC) && ibits(w, POS_INDEX, LEN_INDEX) != WALL) n++;
+  }
+  return n;
+}
+
 /* Get_Prob, Get_Prob.f:54-331.  When pure != 0 no random number is drawn
  * and *iev receives the category code of orc_pure_rate instead. */
 static void get_prob_impl(orc_sim *s, int occ, int nsic, int covind, int coor, const int32_t site[3],
@@ -556,6 +582,7 @@ static void get_prob_impl(orc_sim *s, int occ, int nsic, int covind, int coor, c
     *iev = 4;
     int c1 = s->ncov >= 1 ? covnn[0] : 0, c2 = s->ncov >= 2 ? covnn[1] : 0, c3 = s->ncov >= 3 ? covnn[2] : 0;
     *prob = pev(s, nsic, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
+    if (s->use_f2) *prob *= s->f2[second_count(s, site)];
   }
   /* frozen substrate: the atom stays an MC particle (so the reference's
    * list bookkeeping is unchanged) but is never picked */
@@ -1568,13 +1595,40 @@ static int32_t do_pick(orc_sim *s, int32_t *itrans) {
   *itrans = s->ad[ind].ev;
   return ind;
 }
+/* With the second-neighbour term, a change of occupancy at site changes the
+ * rate of every evaporating MC particle (an atom of coordination 1..3) on
+ * its 12 second-neighbour positions: they get a fresh Get_Prob after the
+ * event's own updates (no reference counterpart). */
+static void refresh_second(orc_sim *s, const int32_t site[3]) {
+  for (int k = 0; k < 12; k++) {
+    int32_t q[3];
+    second_pos(s, site, k, q);
+    if (q[2] < 0 || q[2] > s->lz) continue;
+    int32_t w = s->latt[gidx(s, q[0], q[1], q[2])];
+    int coor = ibits(w, POS_COOR, LEN_COOR);
+    if (!ibits(w, POS_OCC, LEN_OCC) || ibits(w, POS_INDEX, LEN_INDEX) == WALL || coor < 1 || coor > 3) continue;
+    int32_t ind = LI(s, q);
+    if (ind < 1 || ind > s->n_ad || !eq3(s->ad[ind].xyz, q)) fail(s, "second neighbour (%d,%d,%d) is not listed", q[0], q[1], q[2]);
+    int iev = 0;
+    double prob = 0.0;
+    get_prob(s, 1, ibits(w, POS_SIC, LEN_SIC), 0, coor, q, s->ad[ind].nn, &iev, &prob);
+    s->ad[ind].ev = iev;
+    s->ad[ind].prob = prob;
+    update_tree(s, ind, prob);
+  }
+}
+
 static void do_execute(orc_sim *s, int32_t ind, int32_t itrans) {
   s->last_arm = -1;
+  int32_t site[3];
+  cp3(site, s->ad[ind].xyz);
   if (itrans <= 3) {
     deposition(s, ind, itrans);
     if (itrans >= 1) s->count_crystal[itrans - 1]++;
+    if (s->use_f2) refresh_second(s, site);
   } else if (itrans == 4) {
     evaporation(s, ind);
+    if (s->use_f2) refresh_second(s, site);
   } else if (itrans == 5) {
     desorption(s, ind);
   } else {
@@ -1727,6 +1781,27 @@ int orc_set_frozen(orc_sim *s, int z) {
       s->ad[i].prob = 1e-300;
       update_tree(s, i, 1e-300);
     }
+  s->have_jb = 0;
+  return 0;
+}
+
+int orc_set_second_factor(orc_sim *s, const double *f2) {
+  if (setjmp(s->jb)) { s->have_jb = 0; return 2; }
+  s->have_jb = 1;
+  s->use_f2 = f2 != NULL;
+  if (f2) memcpy(s->f2, f2, sizeof s->f2);
+  /* evaporating MC particles take the factor now */
+  for (int32_t i = 1; i <= s->n_ad; i++) {
+    int32_t w = LC(s, s->ad[i].xyz);
+    int coor = ibits(w, POS_COOR, LEN_COOR);
+    if (!ibits(w, POS_OCC, LEN_OCC) || coor < 1 || coor > 3) continue;
+    int iev = 0;
+    double prob = 0.0;
+    get_prob(s, 1, ibits(w, POS_SIC, LEN_SIC), 0, coor, s->ad[i].xyz, s->ad[i].nn, &iev, &prob);
+    s->ad[i].ev = iev;
+    s->ad[i].prob = prob;
+    update_tree(s, i, prob);
+  }
   s->have_jb = 0;
   return 0;
 }

@@ -97,6 +97,14 @@ int orc_apply(orc_sim *s, const int32_t site[3], int32_t itrans, int32_t arm);
  * No reference counterpart. */
 int orc_set_frozen(orc_sim *s, int z);
 
+/* The engine's second-neighbour energy option: the evaporation rate of an
+ * atom with 1..3 reciprocal neighbours is multiplied by f2[n2], n2 = the
+ * number of occupied (non-wall) points among its 12 second-neighbour
+ * positions (+-6,+-6,0 and permutations); after a deposition or evaporation
+ * at s, the evaporating MC particles on s's 12 second-neighbour positions get
+ * a fresh rate.  f2 = NULL switches it off.  No reference counterpart. */
+int orc_set_second_factor(orc_sim *s, const double *f2);
+
 /* Write the main xyz frame of WriteMolMolXYZFile.f (and its SiC
  * wrong-site side effect on LattCoo) exactly as the reference does. */
 int orc_write_frame(orc_sim *s, const char *xyz_path, const char *d_path,

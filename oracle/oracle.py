@@ -44,6 +44,7 @@ def _load():
         L.orc_step.argtypes = [P, P]
         L.orc_apply.argtypes = [P, P, I, I]
         L.orc_set_frozen.argtypes = [P, I]
+        L.orc_set_second_factor.argtypes = [P, P]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
         L.orc_lattcoo.restype = C.POINTER(C.c_int32)
         L.orc_lattcoo.argtypes = [P]
@@ -126,6 +127,16 @@ class Oracle:
 
     def set_frozen(self, z):
         self._check(self.L.orc_set_frozen(self.h, int(z)), "orc_set_frozen")
+
+    def set_second_factor(self, f2):
+        """Second-neighbour evaporation factors f2[0..12] (None = off)."""
+        if f2 is None:
+            rc = self.L.orc_set_second_factor(self.h, None)
+        else:
+            self._f2 = np.ascontiguousarray(f2, np.float64)
+            assert self._f2.shape == (13,)
+            rc = self.L.orc_set_second_factor(self.h, _p(self._f2))
+        self._check(rc, "orc_set_second_factor")
 
     def apply_log(self, log):
         """Execute a logged event sequence (e.g. the engine's) in order."""

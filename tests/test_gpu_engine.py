@@ -65,6 +65,7 @@ def test_parallel_run_is_a_serial_history(tmp_path, case, tile, epp):
     print(f"[{case} T={tile}] {n} events, {len(log)} logged, stop={stop}", flush=True)
     assert len(log) == n and n > 1000 and stop in (0, 1)
     o.apply_log(log)
+    assert o.anomalies()["oob_reads"] == 0, "the film reached the top of the box"
     mc_o = compare(o, eng, box)
     cls, rate = eng.site_rates(mc_o["xyz"])
     keep = cls != 6
@@ -89,3 +90,84 @@ def test_rates_after_run_match_fresh_sweep(tmp_path):
     np.testing.assert_array_equal(cls0[~stale], cls1[~stale])
     np.testing.assert_allclose(r0[~stale], r1[~stale], rtol=1e-6)
     eng.close()
+
+
+# ---- second-neighbour energy term (engine option, restated by the oracle) ----
+F2 = np.exp(-np.arange(13) * 0.05 / (8.617333262e-5 * 1600.0))
+
+
+def n2_tables(d):
+    t = d.tables()
+    t["second_factor"] = F2
+    return t
+
+
+def test_second_neighbour_slab_rates(tmp_path):
+    deck, mode, box, lens = CASES["SiC"]
+    path, d = small_deck(tmp_path, deck, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    o.set_second_factor(F2)
+    eng = Engine(box, n2_tables(d), tile=24)
+    eng.build_slab(mode, lens[0] - lens[0] % 24)
+    mc_o = compare(o, eng, box)
+    cls, rate = eng.site_rates(mc_o["xyz"])
+    np.testing.assert_allclose(rate, mc_o["prob"], rtol=1e-6)
+    # the factor is really applied: surface atoms of a flat slab see 8 occupied second neighbours
+    atoms = cls == 4
+    assert atoms.any()
+    plain = d.tables()["ptrans_e"]
+    assert np.all(rate[atoms] < plain.max() * F2[7])
+    eng.close()
+
+
+def test_second_neighbour_replay_bit_exact(tmp_path):
+    deck, mode, box, lens = CASES["SiC"]
+    path, d = small_deck(tmp_path, deck, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    o.set_second_factor(F2)
+    lat0, mc0, bulk0, void0 = oracle_state(o)
+    eng = Engine(box, n2_tables(d), tile=24)
+    eng.load_state(lat0, mc0, bulk0, void0)
+    log = o.step(20000)
+    eng.replay(log)
+    mc_o = compare(o, eng, box)
+    cls, rate = eng.site_rates(mc_o["xyz"])
+    keep = cls != 6
+    np.testing.assert_allclose(rate[keep], mc_o["prob"][keep], rtol=1e-6)
+    counts, clock, _ = eng.counters(lists=False)
+    assert clock == o.time()
+    eng.close()
+
+
+@pytest.mark.parametrize("tile", [24, 48])
+def test_second_neighbour_parallel_run_is_a_serial_history(tmp_path, tile):
+    deck, mode, box, lens = CASES["SiC"]
+    box = (96, 96, 288)  # the film grows faster with the second-neighbour term: keep it off the box top
+    path, d = small_deck(tmp_path, deck, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    o.set_second_factor(F2)
+    eng = Engine(box, n2_tables(d), seed=23, tile=tile)
+    eng.build_slab(mode, lens[0] - lens[0] % 24)
+    eng.set_event_log(400000)
+    n, t, stop = eng.run(100000)
+    log = eng.event_log()
+    print(f"[n2 T={tile}] {n} events, stop={stop}", flush=True)
+    assert len(log) == n and n > 1000 and stop in (0, 1)
+    o.apply_log(log)
+    assert o.anomalies()["oob_reads"] == 0, "the film reached the top of the box"
+    mc_o = compare(o, eng, box)
+    cls, rate = eng.site_rates(mc_o["xyz"])
+    keep = cls != 6
+    np.testing.assert_allclose(rate[keep], mc_o["prob"][keep], rtol=1e-6)
+    eng.close()
+
+
+def test_second_neighbour_needs_tile_24(tmp_path):
+    deck, mode, box, lens = CASES["SiC"]
+    path, d = small_deck(tmp_path, deck, lens)
+    from mulskips_amd.engine import EngineError
+    with pytest.raises(EngineError):
+        Engine(box, n2_tables(d), tile=16)
