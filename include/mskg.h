@@ -148,7 +148,9 @@ int mskg_replay(mskg_engine *e, const mskg_event *log, int64_t n, int32_t clock_
 
 /* Downloads the lattice back into LattCoo and the three lists (ordered by
  * grid position; capacities in records).  Ind_Event of an adsorption site is
- * reported as 1; ProbTrans is the FP32 rate widened to FP64. */
+ * reported as 1; ProbTrans is the FP32 rate widened to FP64 (0 for the atoms
+ * of a frozen substrate, which are listed as MC particles like any atom of
+ * coordination <= 3). */
 int mskg_store_state(mskg_engine *e, int32_t *latt_coo,
                      int32_t cap_ad, int32_t *n_ad, int32_t *ad_xyz, int32_t *ad_nn,
                      int32_t *ad_event, double *ad_prob,
@@ -171,6 +173,42 @@ int mskg_counters(mskg_engine *e, int64_t counts[8], double *kmc_time,
  * points the last rate sweep read. */
 int mskg_last_timing(mskg_engine *e, double *run_ms, double *sweep_ms, int64_t *sweep_points,
                      int64_t *sweeps_done);
+
+/* Serial BKL on a batch of independent lattices in one launch (one
+ * workgroup per engine): the reference's algorithm (mcevent/PickTreeEvent.f,
+ * main.f:304-371) with the selection over all MC particles of the lattice
+ * by an FP64 three-level rate sum and the clock advanced by -ln(u)/R per
+ * event (Philox-4x32-10, key = the engine's seed).  Engines keep their own
+ * state, counters and event logs (the log's time is the clock after each
+ * event and rtot the R of its selection).  All engines must live on the
+ * current device; the launch uses engines[0]'s stream.  max_events[i] bounds
+ * replica i.  stop_reason as mskg_run; the few-MC stop does not apply. */
+int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_events, double max_time,
+                    int64_t *events_done, double *kmc_time, int32_t *stop_reason);
+
+/* Trajectory likelihood statistics (no reference counterpart; for parameter
+ * calibration).  For each engine i, loaded with the initial state of a
+ * recorded trajectory logs[i][0..n_events[i]) with waiting times dts[i][k]
+ * (the time spent in the state before event k), the engine first refreshes
+ * every rate (as mskg_refresh_rates), then replays the log and returns
+ *   S[i][c]  the number of events drawn from rate class c,
+ *   A[i][c]  sum over events k of (points in class c before event k) * dt_k,
+ * so that log L = sum_c S_c log r_c - A_c r_c + scalars[i][2] - scalars[i][1]
+ * for any rate tables r, and d log L / d theta = sum_c (S_c/r_c - A_c) dr_c/dtheta.
+ * Classes (MSKG_KEY_*): evaporation of species s with n1 (Si or wall), n2,
+ * n3 reciprocal neighbours of species 1..3 and m occupied second-neighbour
+ * positions: key (s-1) + 3*(n1 + 4*(n2 + 4*n3)) + 192*m; an adsorption site
+ * of coordination c: MSKG_KEY_ADS + c - 1 (its A; its selections are split
+ * by deposited species in S[MSKG_KEY_N + 3*(s-1) + c-1]); the placeholder
+ * deposition MSKG_KEY_PH (rate 1e-8); isolated atoms MSKG_KEY_ISO, whose
+ * rates 1e5*R are per point and enter only through scalars.
+ * S: n x MSKG_ADJ_NS, A: n x MSKG_KEY_N, scalars: n x 4 = (sum of dt,
+ * integral of the isolated-atom rates, sum of their log rates, events
+ * replayed).  Models without coverage species and without LA rates only.
+ * The engines end in the trajectory's final state. */
+enum { MSKG_KEY_ADS = 2496, MSKG_KEY_PH = 2499, MSKG_KEY_ISO = 2500, MSKG_KEY_N = 2501, MSKG_ADJ_NS = 2510 };
+int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const *logs, const double *const *dts,
+                 const int64_t *n_events, double *S, double *A, double *scalars);
 
 /* Host evaluation of the engine's FP64 geometry -- the same source the
  * kernels run -- for checking it against the reference formulas without a

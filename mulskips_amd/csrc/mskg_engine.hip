@@ -155,18 +155,19 @@ __global__ void __launch_bounds__(256) k_rate_sweep(Lattice L, double rtot, int 
     if (atoms_only && !(kind_species(kind) && coor >= 1 && coor <= 3)) continue;
     bool mc = (kind == KIND_EMPTY && coor >= 1 && coor <= 3) || (kind >= KIND_COV0 && coor <= 3) ||
               (kind_species(kind) && coor <= 3);
-    int cls = 0;
+    int cls = 0, key = -1;
     double r = 0.0;
     if (mc) {
       P3 pt;
       pt.i = p;
       pxyz(L.box, p, pt.x, pt.y, pt.z);
       int e = ERR_NONE;
-      get_prob(L, rtot, pt, sp, coor, list_of(L, pt, sp), cls, r, e);
+      get_prob(L, rtot, pt, sp, coor, list_of(L, pt, sp), cls, r, e, nullptr, &key);
       if (e) atomicExch(err, e);
     }
     L.rate[p] = cls ? (float)r : 0.0f;
     L.cls[p] = (uint8_t)cls;
+    if (L.key) L.key[p] = cls ? key : -1;
   }
 }
 
@@ -632,6 +633,362 @@ __global__ void __launch_bounds__(256) k_build_slab(Lattice L, int len, int sic)
   }
 }
 
+
+// ---- serial BKL over a batch of independent lattices --------------------------
+// One workgroup (one wave) per replica runs the reference's serial algorithm:
+// every event picks one MC particle of the whole lattice with probability
+// rate/R (R the FP64 total) and advances the clock by -ln(u)/R.  The rates
+// are organised as a three-level sum: the FP32 rates of each tile's
+// candidate list, FP64 tile sums, and FP64 sums of 64 consecutive tiles; an
+// event rescans only the lists of the tiles it touched.
+struct SerialDesc {
+  Lattice L;
+  double *tsum;   // [ntiles]
+  double *bsum;   // [nblk]
+  RunState *st;
+  mskg_event *log;
+  unsigned long long log_cap;
+  long long max_events;
+  double max_time;
+  uint32_t key0, key1;
+  int ntiles, nblk, replica;
+};
+
+// Rate sum of one tile's list (every class); entries whose class fell to 0
+// leave the list.  The compaction is in place: an entry moves only to a
+// position at or before the one it was read from in the same step.
+__device__ double tile_sum_compact(const Lattice &L, int t, int lane) {
+  const TileGeom g = tile_geom(t, L.tile, L.ntx, L.nty, L.box.lz);
+  const int m = min(L.tl_n[t], L.capg);
+  int32_t *idx = L.tl_idx + (int64_t)t * L.capg;
+  double part = 0.0;
+  int nout = 0;
+  for (int j0 = 0; j0 < m; j0 += WAVE) {
+    const int j = j0 + lane;
+    int k = 0;
+    bool keep = false;
+    if (j < m) {
+      k = idx[j];
+      const P3 p = tile_point(L.box, g, L.tile, k);
+      const uint8_t c = L.cls[p.i];
+      keep = c != 0;
+      if (keep) part += (double)L.rate[p.i];
+      else L.inlist[p.i] = 0;
+    }
+    const unsigned long long msk = __ballot(keep);
+    const int slot = nout + __popcll(msk & ((1ull << lane) - 1ull));
+    if (keep) idx[slot] = k;
+    nout += __popcll(msk);
+  }
+  part = wave_sum(part);
+  if (lane == 0) L.tl_n[t] = nout;
+  return part;
+}
+
+// Index of the first element of v[0..n) (FP64, read from memory) whose
+// inclusive prefix exceeds target; *below receives the prefix before it.
+// Rounding at the top end falls back to the last positive element.
+__device__ int scan_pick(const double *v, int n, double target, int lane, double *below) {
+  double acc = 0.0;
+  int last_nz = -1;
+  double acc_last = 0.0;
+  for (int base = 0; base < n; base += WAVE) {
+    const int j = base + lane;
+    const double x = j < n ? v[j] : 0.0;
+    const double inc = wave_incl_scan(x, lane) + acc;
+    const unsigned long long nzm = __ballot(x > 0.0);
+    if (nzm) {
+      const int hi = 63 - __clzll((long long)nzm);
+      last_nz = base + hi;
+      acc_last = __shfl(inc - x, hi, WAVE);
+    }
+    const unsigned long long hit = __ballot(j < n && x > 0.0 && inc > target);
+    if (hit) {
+      const int l = __ffsll((long long)hit) - 1;
+      *below = __shfl(inc - x, l, WAVE);
+      return base + l;
+    }
+    acc = __shfl(inc, WAVE - 1, WAVE);
+  }
+  *below = acc_last;
+  return last_nz;
+}
+
+__global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
+  __shared__ EvScratch sc;
+  __shared__ int s_tiles[MAX_TOUCHED], s_ntl, s_err;
+  const int lane = threadIdx.x;
+  const SerialDesc D = descs[blockIdx.x];
+  Lattice L = D.L;
+  L.cur_tile = -1;
+  RunState *st = D.st;
+  if (st->err) return;
+  // initial sums
+  for (int t = 0; t < D.ntiles; t++) {
+    const double ts = tile_sum_compact(L, t, lane);
+    if (lane == 0) D.tsum[t] = ts;
+  }
+  __syncthreads();
+  for (int b = 0; b < D.nblk; b++) {
+    const int t = b * WAVE + lane;
+    const double v = wave_sum(t < D.ntiles ? D.tsum[t] : 0.0);
+    if (lane == 0) D.bsum[b] = v;
+  }
+  __syncthreads();
+  double clock = st->clock;
+  unsigned long long events = st->events, nev = 0;
+  int stop = MSKG_STOP_LIMIT, err = 0;
+  U4 ctr;
+  ctr.x = (uint32_t)D.replica;
+  while ((long long)nev < D.max_events && clock < D.max_time) {
+    double part = 0.0;
+    for (int b = lane; b < D.nblk; b += WAVE) part += D.bsum[b];
+    const double R = wave_sum(part);
+    if (!(R > 0.0)) { stop = MSKG_STOP_NO_EVENTS; break; }
+    ctr.y = (uint32_t)events;
+    ctr.z = (uint32_t)(events >> 32);
+    ctr.w = 0;
+    const U4 d = philox4x32_10(ctr, D.key0, D.key1);
+    const double dt = -log(u01_open(d.x, d.y)) / R;
+    double target = u01_open(d.z, d.w) * R, below = 0.0;
+    // block -> tile -> list entry
+    const int b = scan_pick(D.bsum, D.nblk, target, lane, &below);
+    if (b < 0) { stop = MSKG_STOP_NO_EVENTS; break; }
+    target -= below;
+    const int t0 = b * WAVE;
+    const int tl = scan_pick(D.tsum + t0, min(WAVE, D.ntiles - t0), target, lane, &below);
+    if (tl < 0) { err = ERR_SUMS; break; }
+    const int t = t0 + tl;
+    target -= below;
+    const TileGeom g = tile_geom(t, L.tile, L.ntx, L.nty, L.box.lz);
+    const int m = min(L.tl_n[t], L.capg);
+    const int32_t *idx = L.tl_idx + (int64_t)t * L.capg;
+    double acc = 0.0;
+    int sel = -1, last_nz = -1;
+    for (int base = 0; base < m; base += WAVE) {
+      const int j = base + lane;
+      double v = 0.0;
+      if (j < m) {
+        const P3 p = tile_point(L.box, g, L.tile, idx[j]);
+        if (L.cls[p.i]) v = (double)L.rate[p.i];
+      }
+      const unsigned long long nzm = __ballot(v > 0.0);
+      if (nzm) last_nz = base + 63 - __clzll((long long)nzm);
+      const double inc = wave_incl_scan(v, lane) + acc;
+      const unsigned long long hit = __ballot(j < m && v > 0.0 && inc > target);
+      if (hit) { sel = base + __ffsll((long long)hit) - 1; break; }
+      acc = __shfl(inc, WAVE - 1, WAVE);
+    }
+    if (sel < 0) sel = last_nz;
+    if (sel < 0) { err = ERR_SUMS; break; }
+    const P3 s = tile_point(L.box, g, L.tile, idx[sel]);
+    ctr.w = 1;
+    const U4 d2 = philox4x32_10(ctr, D.key0, D.key1);
+    Choice ch;
+    ch.arm = -1;
+    ch.u_arm = u01_open(d2.z, d2.w);
+    const int itrans = pick_itrans(L, s, L.cls[s.i], u01_open(d2.x, d2.y));
+    const int coor0 = s_coor(L.site[s.i]);
+    __syncthreads();
+    EvResult res;
+    execute(L, R, s, itrans, ch, res, sc, lane);
+    __syncthreads();
+    clock += dt;
+    if (lane == 0) {
+      s_err = res.err;
+      if (D.log) {
+        const unsigned long long li = st->log_n + nev;
+        if (li < D.log_cap) {
+          mskg_event ev;
+          ev.iter = (int64_t)(events + 1);
+          ev.site[0] = s.x; ev.site[1] = s.y; ev.site[2] = s.z;
+          ev.itrans = itrans;
+          ev.arm = res.arm;
+          ev.coor = coor0;
+          ev.rtot = R;
+          ev.time = clock;
+          D.log[li] = ev;
+        }
+      }
+      const int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
+      if (slot >= 0 && slot < 8) st->counts[slot]++;
+      int nt = 0;
+      for (int q = 0; q < res.ntouched; q++) {
+        if (res.touched[q].i < 0) continue;
+        const int tq = tile_of(L, res.touched[q]);
+        bool dup = false;
+        for (int r = 0; r < nt; r++) dup |= s_tiles[r] == tq;
+        if (!dup) s_tiles[nt++] = tq;
+      }
+      s_ntl = nt;
+    }
+    __syncthreads();
+    events++;
+    nev++;
+    if (s_err) { err = s_err; break; }
+    const int ntl = s_ntl;
+    for (int q = 0; q < ntl; q++) {
+      const double ts = tile_sum_compact(L, s_tiles[q], lane);
+      if (lane == 0) D.tsum[s_tiles[q]] = ts;
+    }
+    __syncthreads();
+    for (int q = 0; q < ntl; q++) {
+      const int bq = s_tiles[q] / WAVE;
+      bool dup = false;
+      for (int r = 0; r < q; r++) dup |= s_tiles[r] / WAVE == bq;
+      if (dup) continue;
+      const int tt = bq * WAVE + lane;
+      const double v = wave_sum(tt < D.ntiles ? D.tsum[tt] : 0.0);
+      if (lane == 0) D.bsum[bq] = v;
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    st->clock = clock;
+    st->events = events;
+    st->log_n += nev;
+    st->stop = stop;
+    if (err) st->err = err;
+  }
+}
+
+// ---- adjoint: replay with per-class accounting ---------------------------------
+// For a recorded trajectory {(event k, dt_k)} the log-likelihood is
+//   sum_k [ log r(event k) - R_k dt_k ]  with R_k = sum_c n_c(k) r_c,
+// where n_c(k) is the number of points whose rate comes from class c before
+// event k.  Replaying the log, this kernel accumulates S_c (selections of
+// class c) and A_c = sum_k n_c(k) dt_k; the likelihood and its gradient with
+// respect to any parameter of the rate tables follow on the host as
+// sum_c S_c log r_c - A_c r_c.  A_c is accumulated lazily per class in LDS
+// (A_c += n_c (t - tau_c) whenever n_c changes) on a clock that restarts
+// every ADJ_EPOCH events, so that the interval lengths keep full precision.
+constexpr int ADJ_EPOCH = 1024;
+constexpr int ADJ_NS = KEY_N + 9;  // + deposition of species s at coordination c
+struct AdjDesc {
+  Lattice L;                 // L.key set
+  int32_t *key_seen;         // key each point was last accounted under
+  float *rate_seen;          // its rate then (isolated atoms)
+  const mskg_event *log;
+  const double *dt;
+  long long n;
+  const int *cnt0;           // [KEY_N] initial class populations
+  double iso0;               // initial sum of isolated-atom rates
+  double *S;                 // [ADJ_NS]
+  double *A;                 // [KEY_N]
+  double *scal;              // [4]: sum dt, integral of isolated rates, sum of their log rates, events
+  RunState *st;
+};
+
+__global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
+  extern __shared__ unsigned char smem[];
+  double *A = (double *)smem;             // [KEY_N]
+  double *tau = A + KEY_N;                // [KEY_N]
+  int *cnt = (int *)(tau + KEY_N);        // [KEY_N]
+  __shared__ EvScratch sc;
+  __shared__ int s_bad;
+  const int lane = threadIdx.x;
+  const AdjDesc D = descs[blockIdx.x];
+  Lattice L = D.L;
+  L.cur_tile = -1;
+  for (int c = lane; c < KEY_N; c += WAVE) {
+    A[c] = 0.0;
+    tau[c] = 0.0;
+    cnt[c] = D.cnt0[c];
+  }
+  for (int c = lane; c < ADJ_NS; c += WAVE) D.S[c] = 0.0;
+  __syncthreads();
+  double t = 0.0, ttot = 0.0, r_iso = D.iso0, tau_iso = 0.0, iso_int = 0.0, iso_log = 0.0;
+  int err = 0;
+  long long k = 0;
+  for (; k < D.n; k++) {
+    const mskg_event ev = D.log[k];
+    const double dtk = D.dt[k];
+    t += dtk;
+    ttot += dtk;
+    const P3 s = at(L, ev.site[0], ev.site[1], ev.site[2]);
+    if (lane == 0) {
+      int bad = 0;
+      const int ks = s.i >= 0 ? D.key_seen[s.i] : -1;
+      if (s.i < 0 || L.cls[s.i] == 0) bad = ERR_NOT_MC;
+      else if (ks < 0) bad = ERR_COVERAGE;  // a rate class the likelihood does not cover
+      else if (ks >= KEY_ADS && ks < KEY_PH) {
+        if (ev.itrans < 1 || ev.itrans > 3) bad = ERR_NOT_MC;
+        else D.S[KEY_N + (ev.itrans - 1) * 3 + (ks - KEY_ADS)] += 1.0;
+      } else if (ks == KEY_ISO) {
+        iso_log += log((double)D.rate_seen[s.i]);
+        D.S[KEY_ISO] += 1.0;
+      } else {
+        D.S[ks] += 1.0;
+      }
+      s_bad = bad;
+    }
+    __syncthreads();
+    if (s_bad) { err = s_bad; break; }
+    Choice ch;
+    ch.arm = ev.arm;
+    ch.u_arm = 0.0;
+    EvResult res;
+    execute(L, ev.rtot, s, ev.itrans, ch, res, sc, lane);
+    __syncthreads();
+    if (lane == 0) {
+      int bad = res.err;
+      for (int q = 0; q < res.ntouched && !bad; q++) {
+        const int64_t p = res.touched[q].i;
+        if (p < 0) continue;
+        const int o = D.key_seen[p], nw = L.key[p];
+        if (nw == KEY_UNSUPPORTED) { bad = ERR_COVERAGE; break; }
+        if (o != nw) {
+          if (o >= 0 && o != KEY_ISO) { A[o] += cnt[o] * (t - tau[o]); tau[o] = t; cnt[o]--; }
+          if (nw >= 0 && nw != KEY_ISO) { A[nw] += cnt[nw] * (t - tau[nw]); tau[nw] = t; cnt[nw]++; }
+        }
+        const float rnew = L.rate[p];
+        const double ro = o == KEY_ISO ? (double)D.rate_seen[p] : 0.0;
+        const double rn = nw == KEY_ISO ? (double)rnew : 0.0;
+        if (ro != rn) {
+          iso_int += r_iso * (t - tau_iso);
+          tau_iso = t;
+          r_iso += rn - ro;
+        }
+        D.key_seen[p] = nw;
+        D.rate_seen[p] = rnew;
+      }
+      s_bad = bad;
+    }
+    __syncthreads();
+    if (s_bad) { err = s_bad; k++; break; }
+    if ((k + 1) % ADJ_EPOCH == 0) {  // restart the accounting clock
+      for (int c = lane; c < KEY_N; c += WAVE) {
+        A[c] += cnt[c] * (t - tau[c]);
+        tau[c] = 0.0;
+      }
+      iso_int += r_iso * (t - tau_iso);
+      tau_iso = 0.0;
+      t = 0.0;
+      __syncthreads();
+    }
+  }
+  for (int c = lane; c < KEY_N; c += WAVE) D.A[c] = A[c] + cnt[c] * (t - tau[c]);
+  if (lane == 0) {
+    iso_int += r_iso * (t - tau_iso);
+    D.scal[0] = ttot;
+    D.scal[1] = iso_int;
+    D.scal[2] = iso_log;
+    D.scal[3] = (double)k;
+    if (err) D.st->err = err;
+  }
+}
+
+// Initial class populations and isolated-rate sum of one lattice.
+__global__ void __launch_bounds__(256) k_key_hist(Lattice L, int *cnt, double *iso) {
+  const int64_t n = L.box.npts;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const int k = L.key[p];
+    if (k == KEY_ISO) atomicAdd(iso, (double)L.rate[p]);
+    else if (k >= 0) atomicAdd(&cnt[k], 1);
+    else if (k == KEY_UNSUPPORTED) atomicAdd(&cnt[KEY_N], 1);
+  }
+}
+
 // ---- host side ---------------------------------------------------------------
 #define HIPCHK(e, x)                                                                  \
   do {                                                                                \
@@ -670,6 +1027,11 @@ struct mskg_engine {
   hipGraphExec_t graph_exec = nullptr;
   mskg_event *d_log = nullptr;
   int64_t log_cap = 0;
+  // serial mode: tile and block rate sums
+  double *d_tsum = nullptr, *d_bsum = nullptr;
+  // likelihood accounting: class key of every point, and as last accounted
+  int32_t *d_key = nullptr, *d_key_seen = nullptr;
+  float *d_rate_seen = nullptr;
   double run_ms = 0, sweep_ms = 0;
   int64_t sweep_points = 0, sweeps_done = 0;
   bool loaded = false;
@@ -719,6 +1081,7 @@ static Lattice make_lattice(mskg_engine *e) {
   L.tile = e->tile;
   L.ntx = e->ntx;
   L.nty = e->nty;
+  L.key = nullptr;
   return L;
 }
 
@@ -750,6 +1113,8 @@ static const char *err_name(int c) {
     case ERR_COVERAGE: return "coverage species not in ListCov";
     case ERR_CAPACITY: return "tile holds more MC particles than tile_capacity";
     case ERR_TEMPERATURE: return "zero temperature in laser-annealing mode";
+    case ERR_TILE_LIST: return "a tile's candidate list overflowed tile_capacity";
+    case ERR_SUMS: return "serial selection found no event inside a positive rate sum";
     default: return "unknown";
   }
 }
@@ -881,6 +1246,7 @@ void mskg_destroy(mskg_engine *e) {
   if (e->graph) hipGraphDestroy(e->graph);
   hipFree(e->d_inlist); hipFree(e->d_tl_idx); hipFree(e->d_tl_n); hipFree(e->d_list_err); hipFree(e->d_win);
   hipFree(e->d_geo); hipFree(e->d_tab); hipFree(e->d_st); hipFree(e->d_err); hipFree(e->d_log);
+  hipFree(e->d_tsum); hipFree(e->d_bsum); hipFree(e->d_key); hipFree(e->d_key_seen); hipFree(e->d_rate_seen);
   if (e->ev0) hipEventDestroy(e->ev0);
   if (e->ev1) hipEventDestroy(e->ev1);
   if (e->stream) hipStreamDestroy(e->stream);
@@ -1171,12 +1537,14 @@ int mskg_store_state(mskg_engine *e, int32_t *latt, int32_t cap_ad, int32_t *n_a
           nn[3 * i + 1] = l.y[i];
           nn[3 * i + 2] = l.z[i];
         }
-        if (cls[p]) {
+        // the reference's list membership: an atom of coordination <= 3 is an MC
+        // particle even when it has no rate (a frozen-substrate atom)
+        if (cls[p] || (kind_species(kind) && coor <= 3)) {
           if (na < cap_ad) {
             if (ad_xyz) memcpy(ad_xyz + 3 * na, xyz, 12);
             if (ad_nn) memcpy(ad_nn + 12 * na, nn, 48);
             int c = cls[p];
-            if (ad_event) ad_event[na] = (c == CLS_EVAP || c == CLS_ISOLATED) ? 4 : c == CLS_DESORB ? 5 : 1;
+            if (ad_event) ad_event[na] = (c == CLS_EVAP || c == CLS_ISOLATED || !c) ? 4 : c == CLS_DESORB ? 5 : 1;
             if (ad_prob) ad_prob[na] = (double)rate[p];
           } else {
             over = true;
@@ -1348,6 +1716,214 @@ int mskg_geometry_find_sn(const int32_t site[3], const int32_t sitec[3], const i
       arm[3 * i + k] = aa[i][k];
     }
   return MSKG_OK;
+}
+
+int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_events, double max_time,
+                    int64_t *events_done, double *kmc_time, int32_t *stop_reason) {
+  if (!engines || n <= 0 || !max_events) return MSKG_EINVAL;
+  mskg_engine *e0 = engines[0];
+  if (!e0) return MSKG_EINVAL;
+  std::vector<SerialDesc> descs(n);
+  std::vector<unsigned long long> ev0(n), log0(n);
+  for (int i = 0; i < n; i++) {
+    mskg_engine *e = engines[i];
+    if (!e || !e->loaded) return fail_(e0, MSKG_EINVAL, "serial run: replica %d has no state loaded", i);
+    if (max_events[i] < 0) return fail_(e, MSKG_EINVAL, "max_events < 0");
+    for (int j = 0; j < i; j++)
+      if (engines[j] == e) return fail_(e0, MSKG_EINVAL, "serial run: replica %d appears twice", i);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    const int nblk = (e->ntiles + WAVE - 1) / WAVE;
+    if (!e->d_tsum) {
+      HIPCHK(e, hipMalloc(&e->d_tsum, e->ntiles * sizeof(double)));
+      HIPCHK(e, hipMalloc(&e->d_bsum, nblk * sizeof(double)));
+    }
+    Lattice L = make_lattice(e);
+    if (e->lists_dirty) {
+      HIPCHK(e, hipMemsetAsync(e->d_list_err, 0, sizeof(int), e->stream));
+      hipLaunchKernelGGL(k_build_lists, dim3((e->ntiles + 3) / 4), dim3(256), 0, e->stream, L, e->ntiles);
+      HIPCHK(e, hipGetLastError());
+      e->lists_dirty = false;
+    }
+    RunState st;
+    int rc = read_state(e, st);
+    if (rc) return rc;
+    if (st.err) return fail_(e, MSKG_ESTATE, "engine is in an error state: %s", err_name(st.err));
+    int lerr = 0;
+    HIPCHK(e, hipMemcpy(&lerr, e->d_list_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (lerr) return fail_(e, MSKG_ECAPACITY, "a tile holds more MC particles than tile_capacity");
+    ev0[i] = st.events;
+    log0[i] = st.log_n;
+    SerialDesc &D = descs[i];
+    D.L = L;
+    D.tsum = e->d_tsum;
+    D.bsum = e->d_bsum;
+    D.st = e->d_st;
+    D.log = e->d_log;
+    D.log_cap = (unsigned long long)e->log_cap;
+    D.max_events = max_events[i];
+    D.max_time = max_time;
+    D.key0 = (uint32_t)e->cfg.seed;
+    D.key1 = (uint32_t)(e->cfg.seed >> 32) ^ 0x5e1a1u;  // a stream apart from the sublattice passes
+    D.ntiles = e->ntiles;
+    D.nblk = nblk;
+    D.replica = i;
+  }
+  SerialDesc *d_descs = nullptr;
+  HIPCHK(e0, hipMalloc(&d_descs, n * sizeof(SerialDesc)));
+  hipError_t ce = hipMemcpyAsync(d_descs, descs.data(), n * sizeof(SerialDesc), hipMemcpyHostToDevice, e0->stream);
+  if (ce == hipSuccess) {
+    HIPCHK(e0, hipEventRecord(e0->ev0, e0->stream));
+    hipLaunchKernelGGL(k_serial, dim3(n), dim3(WAVE), 0, e0->stream, d_descs);
+    ce = hipGetLastError();
+    if (ce == hipSuccess) ce = hipEventRecord(e0->ev1, e0->stream);
+    if (ce == hipSuccess) ce = hipStreamSynchronize(e0->stream);
+  }
+  hipFree(d_descs);
+  if (ce != hipSuccess) return fail_(e0, MSKG_EDEVICE, "k_serial: %s", hipGetErrorString(ce));
+  float ms = 0;
+  HIPCHK(e0, hipEventElapsedTime(&ms, e0->ev0, e0->ev1));
+  int rc_all = MSKG_OK;
+  for (int i = 0; i < n; i++) {
+    mskg_engine *e = engines[i];
+    e->run_ms = ms;
+    e->lists_dirty = false;
+    RunState st;
+    int rc = read_state(e, st);
+    if (rc) return rc;
+    if (events_done) events_done[i] = (int64_t)(st.events - ev0[i]);
+    if (kmc_time) kmc_time[i] = st.clock;
+    if (stop_reason) stop_reason[i] = st.stop;
+    if (st.err && rc_all == MSKG_OK)
+      rc_all = fail_(e, MSKG_ESTATE, "serial run of replica %d stopped: %s", i, err_name(st.err));
+    if (e->d_log && st.log_n > (unsigned long long)e->log_cap && rc_all == MSKG_OK)
+      rc_all = fail_(e, MSKG_ECAPACITY, "event log of replica %d overflowed", i);
+    (void)log0;
+  }
+  return rc_all;
+}
+
+int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const *logs, const double *const *dts,
+                 const int64_t *n_events, double *S, double *A, double *scalars) {
+  if (!engines || n <= 0 || !logs || !dts || !n_events || !S || !A || !scalars) return MSKG_EINVAL;
+  mskg_engine *e0 = engines[0];
+  if (!e0) return MSKG_EINVAL;
+  std::vector<AdjDesc> descs(n);
+  std::vector<void *> scratch;
+  auto cleanup = [&]() {
+    for (void *q : scratch) hipFree(q);
+  };
+  auto dev = [&](size_t bytes) -> void * {
+    void *q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
+    scratch.push_back(q);
+    return q;
+  };
+  for (int i = 0; i < n; i++) {
+    mskg_engine *e = engines[i];
+    if (!e || !e->loaded) { cleanup(); return fail_(e0, MSKG_EINVAL, "adjoint: replica %d has no state loaded", i); }
+    for (int j = 0; j < i; j++)
+      if (engines[j] == e) { cleanup(); return fail_(e0, MSKG_EINVAL, "adjoint: replica %d appears twice", i); }
+    if (e->tab.ncov || e->tab.la_mode) {
+      cleanup();
+      return fail_(e, MSKG_EINVAL, "the likelihood covers models without coverage species and without LA rates");
+    }
+    if (n_events[i] < 0 || (n_events[i] > 0 && (!logs[i] || !dts[i]))) { cleanup(); return fail_(e, MSKG_EINVAL, "bad log"); }
+    const int64_t np = e->box.npts;
+    if (!e->d_key) {
+      if (hipMalloc(&e->d_key, np * sizeof(int32_t)) != hipSuccess ||
+          hipMalloc(&e->d_key_seen, np * sizeof(int32_t)) != hipSuccess ||
+          hipMalloc(&e->d_rate_seen, np * sizeof(float)) != hipSuccess) {
+        cleanup();
+        return fail_(e, MSKG_ENOMEM, "adjoint: no device memory for the class keys");
+      }
+    }
+    RunState st;
+    int rc = read_state(e, st);
+    if (rc) { cleanup(); return rc; }
+    if (st.err) { cleanup(); return fail_(e, MSKG_ESTATE, "engine is in an error state: %s", err_name(st.err)); }
+    // the trajectory starts from fresh rates: a sweep that also assigns every point its class key
+    Lattice L = make_lattice(e);
+    L.inlist = nullptr;
+    L.key = e->d_key;
+    e->lists_dirty = true;
+    int *d_cnt = (int *)dev((KEY_N + 1) * sizeof(int));
+    double *d_iso = (double *)dev(sizeof(double));
+    mskg_event *d_log = (mskg_event *)dev(std::max<int64_t>(1, n_events[i]) * sizeof(mskg_event));
+    double *d_dt = (double *)dev(std::max<int64_t>(1, n_events[i]) * sizeof(double));
+    double *d_S = (double *)dev(ADJ_NS * sizeof(double));
+    double *d_A = (double *)dev(KEY_N * sizeof(double));
+    double *d_sc = (double *)dev(4 * sizeof(double));
+    if (!d_cnt || !d_iso || !d_log || !d_dt || !d_S || !d_A || !d_sc) {
+      cleanup();
+      return fail_(e, MSKG_ENOMEM, "adjoint: no device memory for the log");
+    }
+    int blocks = (int)std::min<int64_t>((np + 255) / 256, 256 * 64);
+    HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
+    hipLaunchKernelGGL(k_rate_sweep, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemsetAsync(d_cnt, 0, (KEY_N + 1) * sizeof(int), e->stream));
+    HIPCHK(e, hipMemsetAsync(d_iso, 0, sizeof(double), e->stream));
+    hipLaunchKernelGGL(k_key_hist, dim3(blocks), dim3(256), 0, e->stream, L, d_cnt, d_iso);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(e->d_key_seen, e->d_key, np * sizeof(int32_t), hipMemcpyDeviceToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(e->d_rate_seen, e->d_rate, np * sizeof(float), hipMemcpyDeviceToDevice, e->stream));
+    if (n_events[i] > 0) {
+      HIPCHK(e, hipMemcpyAsync(d_log, logs[i], n_events[i] * sizeof(mskg_event), hipMemcpyHostToDevice, e->stream));
+      HIPCHK(e, hipMemcpyAsync(d_dt, dts[i], n_events[i] * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    }
+    int err = 0, unsup = 0;
+    HIPCHK(e, hipMemcpyAsync(&err, e->d_err, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipMemcpyAsync(&unsup, d_cnt + KEY_N, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (err) { cleanup(); return fail_(e, MSKG_ESTATE, "rate sweep: %s", err_name(err)); }
+    if (unsup) { cleanup(); return fail_(e, MSKG_EINVAL, "%d points have rates the likelihood does not cover", unsup); }
+    double iso0 = 0.0;
+    HIPCHK(e, hipMemcpy(&iso0, d_iso, sizeof(double), hipMemcpyDeviceToHost));
+    AdjDesc &D = descs[i];
+    D.L = L;
+    D.key_seen = e->d_key_seen;
+    D.rate_seen = e->d_rate_seen;
+    D.log = d_log;
+    D.dt = d_dt;
+    D.n = n_events[i];
+    D.cnt0 = d_cnt;
+    D.iso0 = iso0;
+    D.S = d_S;
+    D.A = d_A;
+    D.scal = d_sc;
+    D.st = e->d_st;
+  }
+  AdjDesc *d_descs = (AdjDesc *)dev(n * sizeof(AdjDesc));
+  if (!d_descs) { cleanup(); return fail_(e0, MSKG_ENOMEM, "adjoint: descriptors"); }
+  hipError_t ce = hipMemcpyAsync(d_descs, descs.data(), n * sizeof(AdjDesc), hipMemcpyHostToDevice, e0->stream);
+  const size_t shm = (size_t)KEY_N * (8 + 8 + 4);
+  if (ce == hipSuccess) ce = hipEventRecord(e0->ev0, e0->stream);
+  if (ce == hipSuccess) {
+    hipLaunchKernelGGL(k_adjoint, dim3(n), dim3(WAVE), shm, e0->stream, d_descs);
+    ce = hipGetLastError();
+  }
+  if (ce == hipSuccess) ce = hipEventRecord(e0->ev1, e0->stream);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(e0->stream);
+  if (ce != hipSuccess) { cleanup(); return fail_(e0, MSKG_EDEVICE, "k_adjoint: %s", hipGetErrorString(ce)); }
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0->ev0, e0->ev1);
+  int rc_all = MSKG_OK;
+  for (int i = 0; i < n; i++) {
+    mskg_engine *e = engines[i];
+    e->run_ms = ms;
+    ce = hipMemcpy(S + (size_t)i * ADJ_NS, descs[i].S, ADJ_NS * sizeof(double), hipMemcpyDeviceToHost);
+    if (ce == hipSuccess) ce = hipMemcpy(A + (size_t)i * KEY_N, descs[i].A, KEY_N * sizeof(double), hipMemcpyDeviceToHost);
+    if (ce == hipSuccess) ce = hipMemcpy(scalars + (size_t)i * 4, descs[i].scal, 4 * sizeof(double), hipMemcpyDeviceToHost);
+    if (ce != hipSuccess) { cleanup(); return fail_(e, MSKG_EDEVICE, "adjoint download: %s", hipGetErrorString(ce)); }
+    RunState st;
+    int rc = read_state(e, st);
+    if (rc) { cleanup(); return rc; }
+    if (st.err && rc_all == MSKG_OK)
+      rc_all = fail_(e, MSKG_ESTATE, "adjoint replay of replica %d stopped after %lld events: %s", i,
+                     (long long)scalars[(size_t)i * 4 + 3], err_name(st.err));
+  }
+  cleanup();
+  return rc_all;
 }
 
 }  // extern "C"

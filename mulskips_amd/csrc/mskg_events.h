@@ -22,7 +22,8 @@ enum : int {
   ERR_COVERAGE = 4,   // coverage index not in ListCov
   ERR_CAPACITY = 5,   // tile candidate list overflow
   ERR_TEMPERATURE = 6, // zero temperature in LA mode
-  ERR_TILE_LIST = 7    // a tile's candidate list overflowed
+  ERR_TILE_LIST = 7,   // a tile's candidate list overflowed
+  ERR_SUMS = 8         // serial selection: the rate sums disagree with the lists
 };
 
 // Per-tile candidate lists (synchronous-sublattice mode): every MC particle
@@ -43,7 +44,19 @@ struct Lattice {
   const Tables *tab;
   int tile, ntx, nty, capg;
   int cur_tile;           // the tile whose pass is running: its list lives on chip
+  int32_t *key;           // may be null: rate-class key of every point (likelihood accounting)
 };
+
+// Rate-class keys for the trajectory likelihood (models without coverage
+// species).  A key names the table entry a point's rate comes from:
+//   evaporation  (species-1) + 3*(n_Si+n_wall + 4*(n_C + 4*n_B)) + 192*n2
+//                (n2 = occupied second-neighbour positions, 0 without that term)
+//   adsorption site of coordination c       KEY_ADS + c - 1
+//   placeholder deposition (1e-8)           KEY_PH
+//   isolated atom (1e5*R, per point)        KEY_ISO
+// and -1 for no rate.
+enum : int { KEY_EVAP_N = 192 * 13, KEY_ADS = KEY_EVAP_N, KEY_PH = KEY_ADS + 3, KEY_ISO = KEY_PH + 1,
+             KEY_N = KEY_ISO + 1, KEY_UNSUPPORTED = -2 };
 
 struct P3 {
   int x, y, z;
@@ -133,10 +146,11 @@ MSKG_HD void enlist(const Lattice &L, const P3 &p) {
   if (slot < L.capg) L.tl_idx[(int64_t)t * L.capg + slot] = tile_offset(L, p);
   else *L.list_err = ERR_TILE_LIST;
 }
-MSKG_HD void set_rate(const Lattice &L, const P3 &p, int cls, double rate) {
+MSKG_HD void set_rate(const Lattice &L, const P3 &p, int cls, double rate, int key = -1) {
   if (p.i < 0) return;
   L.rate[p.i] = cls ? (float)rate : 0.0f;
   L.cls[p.i] = (uint8_t)cls;
+  if (L.key) L.key[p.i] = cls ? key : -1;
   if (cls) enlist(L, p);
 }
 
@@ -203,8 +217,11 @@ MSKG_HD int second_count(const Lattice &L, const P3 &p, const Ovr *ovr) {
 }
 
 MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &cls,
-                      double &rate, int &err, const Ovr *ovr = nullptr) {
+                      double &rate, int &err, const Ovr *ovr = nullptr, int *key = nullptr) {
   const Tables &t = *L.tab;
+  int kdummy;
+  if (!key) key = &kdummy;
+  *key = -1;
   int kind = s_kind(sp);
   int temperature = 0;
   if (t.la_mode) {
@@ -214,7 +231,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   int slot = kind_cov_slot(kind);
   if (slot) {  // desorption, :93-123
     if (slot > t.ncov) { err = ERR_COVERAGE; cls = CLS_NONE; rate = 0.0; return; }
-    if (coor <= 3) { cls = CLS_DESORB; rate = t.pdes[(slot - 1) + 3 * coor]; }
+    if (coor <= 3) { cls = CLS_DESORB; rate = t.pdes[(slot - 1) + 3 * coor]; *key = KEY_UNSUPPORTED; }
     else { cls = CLS_NONE; rate = 0.0; }
     return;
   }
@@ -227,9 +244,11 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
         double pabs = t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor];
         cls = CLS_ADSORB;
         rate = pdep + pabs;
+        *key = t.ncov ? KEY_UNSUPPORTED : KEY_ADS + coor - 1;
       } else {
         cls = CLS_PLACEHOLDER;
         rate = (double)1.e-8f;  // REAL(4) literal, Get_Prob.f:183
+        *key = KEY_PH;
       }
     } else {
       cls = CLS_NONE;  // Coor 0 leaves the reference's outputs unassigned; Coor 4 is a vacancy
@@ -278,11 +297,17 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   } else if (tot == 0) {
     cls = CLS_ISOLATED;
     rate = (double)100000.0f * rtot;
+    *key = KEY_ISO;
   } else {
     int c1 = t.ncov >= 1 ? covnn[0] : 0, c2 = t.ncov >= 2 ? covnn[1] : 0, c3 = t.ncov >= 3 ? covnn[2] : 0;
     cls = CLS_EVAP;
     rate = pe_of(t, kind, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
-    if (t.use_f2) rate *= t.f2[second_count(L, p, ovr)];
+    int n2 = 0;
+    if (t.use_f2) {
+      n2 = second_count(L, p, ovr);
+      rate *= t.f2[n2];
+    }
+    *key = (t.ncov || t.la_mode) ? KEY_UNSUPPORTED : (kind - 1) + 3 * (n_si + n_wall + 4 * (n_c + 4 * n_b)) + 192 * n2;
   }
 }
 
@@ -307,10 +332,10 @@ struct EvResult {
 // zero rate takes the site out of the MC population, its list stays.
 MSKG_HD void refresh(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &err,
                      const Ovr *ovr = nullptr, float *out_rate = nullptr, uint8_t *out_cls = nullptr) {
-  int cls = 0;
+  int cls = 0, key = -1;
   double r = 0.0;
-  get_prob(L, rtot, p, sp, coor, nn, cls, r, err, ovr);
-  set_rate(L, p, cls, r);
+  get_prob(L, rtot, p, sp, coor, nn, cls, r, err, ovr, &key);
+  set_rate(L, p, cls, r, key);
   if (out_rate) *out_rate = cls ? (float)r : 0.0f;
   if (out_cls) *out_cls = (uint8_t)cls;
 }
@@ -386,12 +411,12 @@ __device__ void dep_neighbour(const Lattice &L, double rtot, const P3 &s, int i,
       NL nnn;
       for (int j = 0; j < 4; j++) nnn.x[j] = nnn.y[j] = nnn.z[j] = 0;
       nnn.x[0] = s.x; nnn.y[0] = s.y; nnn.z[0] = s.z;
-      int cls = 0;
+      int cls = 0, key = -1;
       double r = 0.0;
-      get_prob(L, rtot, p, sp, coornn, nnn, cls, r, err);
+      get_prob(L, rtot, p, sp, coornn, nnn, cls, r, err, nullptr, &key);
       if (cls && !set_list(L.box, sp, p.x, p.y, p.z, nnn)) { err = ERR_GEOM; return; }  // AddMC stores the list
       wr(L, p.i, sp);
-      set_rate(L, p, cls, r);
+      set_rate(L, p, cls, r, key);
     } else if (coornn == 1) {
       NL nnn = list_of(L, p, sp);
       int dx = nnn.x[0] - s.x, dy = nnn.y[0] - s.y, dz = nnn.z[0] - s.z;
@@ -671,9 +696,9 @@ __device__ void evaporation(const Lattice &L, double rtot, const P3 &s, EvResult
       return;
     }
     wr(L, s.i, ss);
-    int cls = 0;
+    int cls = 0, key = -1;
     double r = 0.0;
-    get_prob(L, rtot, s, ss, coor, sc.nn, cls, r, res.err);
+    get_prob(L, rtot, s, ss, coor, sc.nn, cls, r, res.err, nullptr, &key);
     if (coor == 1) {
       int pos[3];
       if (nbuff_of(L, s, sc.nn, pos))
@@ -681,7 +706,7 @@ __device__ void evaporation(const Lattice &L, double rtot, const P3 &s, EvResult
       if (!one_entry_list(L, ss, s, nbuff)) { res.err = ERR_GEOM; return; }
       wr(L, s.i, ss);
     }
-    set_rate(L, s, cls, r);
+    set_rate(L, s, cls, r, key);
   }
 }
 

@@ -143,6 +143,11 @@ class Engine:
         self._check(self.L.mskg_run(self.h, int(max_events), mt, C.byref(n), C.byref(t), C.byref(s)))
         return n.value, t.value, s.value
 
+    def run_serial(self, max_events, max_time=float("inf")):
+        """Serial BKL (one workgroup): see run_serial()."""
+        n, t, s = run_serial([self], [max_events], max_time)
+        return n[0], t[0], s[0]
+
     def replay(self, log, clock_mode=0):
         log = np.ascontiguousarray(log, EVENT_DTYPE)
         self._check(self.L.mskg_replay(self.h, ptr(log), len(log), int(clock_mode)))
@@ -209,3 +214,53 @@ def find_sn(site, sitec, sn, lx, ly):
     arm = np.zeros(9, np.int32)
     L.mskg_geometry_find_sn(ptr(a), ptr(c), ptr(n), int(lx), int(ly), ptr(zig), ptr(arm))
     return zig.reshape(3, 3), arm.reshape(3, 3)
+
+
+def _engine_array(engines):
+    arr = (C.c_void_p * len(engines))(*[e.h for e in engines])
+    return arr
+
+
+def run_serial(engines, max_events, max_time=float("inf")):
+    """Run the serial BKL algorithm on several engines (independent lattices)
+    in one launch, one workgroup each.  Returns per-engine (events, clock,
+    stop reason) arrays."""
+    L = _lib.lib()
+    n = len(engines)
+    me = np.ascontiguousarray(np.broadcast_to(np.asarray(max_events, np.int64), (n,)))
+    done = np.zeros(n, np.int64)
+    clock = np.zeros(n, np.float64)
+    stop = np.zeros(n, np.int32)
+    mt = float(max_time) if np.isfinite(max_time) else 1e300
+    rc = L.mskg_run_serial(_engine_array(engines), n, ptr(me), mt, ptr(done), ptr(clock), ptr(stop))
+    if rc:
+        bad = [e for e in engines if L.mskg_last_error(e.h)]
+        msg = "; ".join(L.mskg_last_error(e.h).decode() for e in bad) or "mskg_run_serial failed"
+        raise EngineError(rc, msg)
+    return done, clock, stop
+
+
+def adjoint_stats(engines, logs, dts):
+    """Likelihood statistics of recorded trajectories (mskg_adjoint): engine i
+    holds the initial state of trajectory i, logs[i] its events and dts[i]
+    the waiting time before each.  Returns S (n, ADJ_NS), A (n, KEY_N) and
+    scalars (n, 4) = (sum dt, isolated-rate integral, isolated log-rate sum,
+    events replayed).  The engines end in the final states."""
+    L = _lib.lib()
+    n = len(engines)
+    logs = [np.ascontiguousarray(lg, EVENT_DTYPE) for lg in logs]
+    dts = [np.ascontiguousarray(d, np.float64) for d in dts]
+    for lg, d in zip(logs, dts):
+        if len(lg) != len(d):
+            raise ValueError("each log needs one waiting time per event")
+    lp = (C.c_void_p * n)(*[lg.ctypes.data for lg in logs])
+    dp = (C.c_void_p * n)(*[d.ctypes.data for d in dts])
+    ne = np.array([len(lg) for lg in logs], np.int64)
+    S = np.zeros((n, _lib.ADJ_NS))
+    A = np.zeros((n, _lib.KEY_N))
+    sc = np.zeros((n, 4))
+    rc = L.mskg_adjoint(_engine_array(engines), n, lp, dp, ptr(ne), ptr(S), ptr(A), ptr(sc))
+    if rc:
+        msg = "; ".join(L.mskg_last_error(e.h).decode() for e in engines if L.mskg_last_error(e.h))
+        raise EngineError(rc, msg or "mskg_adjoint failed")
+    return S, A, sc

@@ -37,7 +37,7 @@ static const int JUMP_DIA[8][3] = { /* defsystem.f:112-113, JumpDia(1:3,i) */
   {3, 3, 3}, {3, -3, -3}, {-3, -3, 3}, {-3, 3, -3},
   {3, 3, -3}, {3, -3, 3}, {-3, -3, -3}, {-3, 3, 3}};
 
-typedef struct { int32_t xyz[3]; int32_t nn[4][3]; int32_t ev; int32_t pad; double prob; } mc_rec;
+typedef struct { int32_t xyz[3]; int32_t nn[4][3]; int32_t ev; int32_t key; double prob; } mc_rec;
 typedef struct { int32_t xyz[3]; int32_t nn[4][3]; } at_rec;
 typedef struct { int32_t xyz[3]; int32_t type; } buf_rec;
 
@@ -71,6 +71,7 @@ struct orc_sim {
   int frozen_below_z;  /* engine option restated: atoms below this z never evaporate */
   int use_f2; double f2[13];  /* engine option restated: evaporation rate factor by the number
                                  of occupied second-neighbour positions */
+  int last_key;               /* rate-class key of the last Get_Prob (likelihood accounting) */
   int64_t oob_reads;
   int64_t nonbond_arm, nonbond_nn;  /* geometry results that are not one of the 32 bond vectors */
   /* error handling: the reference STOPs; we unwind to the entry point */
@@ -233,6 +234,7 @@ static void add_mc(orc_sim *s, const int32_t site[3], const int32_t nn[4][3], in
   memcpy(r->nn, nn, sizeof r->nn);
   r->ev = ev;
   r->prob = prob;
+  r->key = s->last_key;
   update_tree(s, s->n_ad, prob);
 }
 
@@ -484,9 +486,18 @@ static int second_count(orc_sim *s, const int32_t site[3]) {
 
 /* Get_Prob, Get_Prob.f:54-331.  When pure != 0 no random number is drawn
  * and *iev receives the category code of orc_pure_rate instead. */
+/* Rate-class keys of the trajectory likelihood (models without coverage
+ * species): evaporation (nsic-1) + 3*(n1 + 4*(n2 + 4*n3)) + 192*m with m the
+ * occupied second-neighbour positions (0 without that term); an adsorption
+ * site of coordination c: KEY_ADS + c - 1; placeholder KEY_PH; isolated
+ * atom KEY_ISO; -1 no rate; -2 a class the likelihood does not cover.  The
+ * same numbering as the engine's mskg_adjoint.  No reference counterpart. */
+enum { KEY_ADS = 2496, KEY_PH = 2499, KEY_ISO = 2500, KEY_N = 2501, ADJ_NS = 2510 };
+
 static void get_prob_impl(orc_sim *s, int occ, int nsic, int covind, int coor, const int32_t site[3],
                           const int32_t nn[4][3], int *iev, double *prob, int pure) {
   int temperature = 0;
+  s->last_key = -1;
   if (s->la_mode) {
     temperature = ibits(LC(s, site), POS_T, LEN_T);
     if (temperature == 0) fail(s, "Temperature is zero in get_prob()");
@@ -500,6 +511,7 @@ static void get_prob_impl(orc_sim *s, int occ, int nsic, int covind, int coor, c
       if (icov == 0) fail(s, "ERROR: could not find CovInd=%d inside ListCov", covind);
       *iev = 5;
       *prob = PDES(s, icov, coor);
+      s->last_key = -2;
     } else {
       *iev = 0;
       *prob = 0.0;
@@ -530,9 +542,11 @@ static void get_prob_impl(orc_sim *s, int occ, int nsic, int covind, int coor, c
           }
         }
         *prob = ptot;
+        s->last_key = s->ncov ? -2 : KEY_ADS + coor - 1;
       } else {
         *iev = pure ? 2 : 1;
         *prob = (double)1.e-8f; /* REAL(4) literal, Get_Prob.f:183 */
+        s->last_key = KEY_PH;
       }
     } else if (coor >= 4) {
       *iev = 0;
@@ -578,15 +592,24 @@ static void get_prob_impl(orc_sim *s, int occ, int nsic, int covind, int coor, c
   } else if (tot == 0) {
     *iev = pure ? 6 : 4;
     *prob = (double)100000.0f * s->tree[1];
+    s->last_key = KEY_ISO;
   } else {
     *iev = 4;
     int c1 = s->ncov >= 1 ? covnn[0] : 0, c2 = s->ncov >= 2 ? covnn[1] : 0, c3 = s->ncov >= 3 ? covnn[2] : 0;
     *prob = pev(s, nsic, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
-    if (s->use_f2) *prob *= s->f2[second_count(s, site)];
+    int m = 0;
+    if (s->use_f2) {
+      m = second_count(s, site);
+      *prob *= s->f2[m];
+    }
+    s->last_key = (s->ncov || s->la_mode) ? -2 : (nsic - 1) + 3 * (n_si + n_wall + 4 * (n_c + 4 * n_b)) + 192 * m;
   }
   /* frozen substrate: the atom stays an MC particle (so the reference's
    * list bookkeeping is unchanged) but is never picked */
-  if (*iev == 4 && site[2] < s->frozen_below_z) *prob = 1e-300;
+  if (*iev == 4 && site[2] < s->frozen_below_z) {
+    *prob = 1e-300;
+    s->last_key = -1;
+  }
 }
 static void get_prob(orc_sim *s, int occ, int nsic, int covind, int coor, const int32_t site[3],
                      const int32_t nn[4][3], int *iev, double *prob) {
@@ -597,6 +620,7 @@ static void get_prob(orc_sim *s, int occ, int nsic, int covind, int coor, const 
 static void get_prob_ini(orc_sim *s, int occ, int nsic, int coor, const int32_t site[3], const int32_t nn[4][3],
                          int *iev, double *prob) {
   int temperature = 0;
+  s->last_key = -2; /* initial rates: refreshed before any likelihood accounting */
   if (s->la_mode) {
     temperature = ibits(LC(s, site), POS_T, LEN_T);
     if (temperature == 0) fail(s, "Temperature is zero in get_prob_ini()");
@@ -728,7 +752,7 @@ static void deposition(orc_sim *s, int32_t ind_void, int iatom) {
             add_void(s, p, nnn);
           } else {
             s->ad[indnn].ev = iev;
-            s->ad[indnn].prob = prob;
+            s->ad[indnn].prob = prob; s->ad[indnn].key = s->last_key;
             update_tree(s, indnn, prob);
           }
         }
@@ -743,7 +767,7 @@ static void deposition(orc_sim *s, int32_t ind_void, int iatom) {
             add_void(s, p, nnn);
           } else {
             s->ad[indnn].ev = iev;
-            s->ad[indnn].prob = prob;
+            s->ad[indnn].prob = prob; s->ad[indnn].key = s->last_key;
             update_tree(s, indnn, prob);
           }
         }
@@ -763,7 +787,7 @@ static void deposition(orc_sim *s, int32_t ind_void, int iatom) {
               add_bulk(s, p, nnn);
             } else {
               s->ad[indnn].ev = iev;
-              s->ad[indnn].prob = prob;
+              s->ad[indnn].prob = prob; s->ad[indnn].key = s->last_key;
               update_tree(s, indnn, prob);
             }
             if (coor_old == 1 && i >= 1) coor++;
@@ -784,7 +808,7 @@ static void deposition(orc_sim *s, int32_t ind_void, int iatom) {
     add_bulk(s, site, nn_site);
   } else {
     s->ad[ind_void].ev = iev;
-    s->ad[ind_void].prob = prob;
+    s->ad[ind_void].prob = prob; s->ad[ind_void].key = s->last_key;
     update_tree(s, ind_void, prob);
   }
 }
@@ -846,7 +870,7 @@ static void evaporation(orc_sim *s, int32_t ind_atom) {
           cp3(s->ad[indnn].nn[0], nbuff);
           get_prob(s, 0, 0, covind, coornn, p, nnn, &iev, &prob); /* old list, :121 */
           s->ad[indnn].ev = iev;
-          s->ad[indnn].prob = prob;
+          s->ad[indnn].prob = prob; s->ad[indnn].key = s->last_key;
           update_tree(s, indnn, prob);
         }
       } else if (coornn == 3) {
@@ -856,7 +880,7 @@ static void evaporation(orc_sim *s, int32_t ind_atom) {
           mvbits(coornn, LEN_COOR, LCp(s, p), POS_COOR);
           get_prob(s, 0, 0, covind, coornn, p, nnn, &iev, &prob);
           s->ad[indnn].ev = iev;
-          s->ad[indnn].prob = prob;
+          s->ad[indnn].prob = prob; s->ad[indnn].key = s->last_key;
           update_tree(s, indnn, prob);
         }
       } else if (coornn == 4) {
@@ -890,7 +914,7 @@ static void evaporation(orc_sim *s, int32_t ind_atom) {
             int sic = ibits(LC(s, p), POS_SIC, LEN_SIC);
             get_prob(s, 1, sic, covind, coornn, p, nnn, &iev, &prob);
             s->ad[indnn].ev = iev;
-            s->ad[indnn].prob = prob;
+            s->ad[indnn].prob = prob; s->ad[indnn].key = s->last_key;
             update_tree(s, indnn, prob);
           }
         }
@@ -908,7 +932,7 @@ static void evaporation(orc_sim *s, int32_t ind_atom) {
     double prob = 0.0;
     get_prob(s, 0, 0, 0, coor, site, nn_site, &iev, &prob);
     s->ad[ind_atom].ev = iev;
-    s->ad[ind_atom].prob = prob;
+    s->ad[ind_atom].prob = prob; s->ad[ind_atom].key = s->last_key;
     if (coor == 1) {
       int32_t cur[4][3];
       memcpy(cur, s->ad[ind_atom].nn, sizeof cur);
@@ -955,7 +979,7 @@ static void absorption(orc_sim *s, int32_t ind_void, int covind_new) {
           double prob = 0.0;
           get_prob(s, 1, sic, covn, coorn, p, nnn, &iev, &prob);
           s->ad[indnn].ev = iev;
-          s->ad[indnn].prob = prob;
+          s->ad[indnn].prob = prob; s->ad[indnn].key = s->last_key;
           update_tree(s, indnn, prob);
         }
       }
@@ -965,7 +989,7 @@ static void absorption(orc_sim *s, int32_t ind_void, int covind_new) {
   double prob = 0.0;
   get_prob(s, 0, 0, covind_new, coor, site, nn_site, &iev, &prob);
   s->ad[ind_void].ev = iev;
-  s->ad[ind_void].prob = prob;
+  s->ad[ind_void].prob = prob; s->ad[ind_void].key = s->last_key;
   update_tree(s, ind_void, prob);
 }
 
@@ -997,7 +1021,7 @@ static void desorption(orc_sim *s, int32_t ind_atom) {
           double prob = 0.0;
           get_prob(s, 1, sic, covn, coorn, p, nnn, &iev, &prob);
           s->ad[indnn].ev = iev;
-          s->ad[indnn].prob = prob;
+          s->ad[indnn].prob = prob; s->ad[indnn].key = s->last_key;
           update_tree(s, indnn, prob);
         }
       }
@@ -1007,7 +1031,7 @@ static void desorption(orc_sim *s, int32_t ind_atom) {
   double prob = 0.0;
   get_prob(s, 0, 0, 0, coor, site, nn_site, &iev, &prob);
   s->ad[ind_atom].ev = iev;
-  s->ad[ind_atom].prob = prob;
+  s->ad[ind_atom].prob = prob; s->ad[ind_atom].key = s->last_key;
   update_tree(s, ind_atom, prob);
 }
 
@@ -1074,6 +1098,7 @@ static void set_flat(orc_sim *s, int len, int sic) {
         mvbits(2, LEN_COOR, w, POS_COOR);
         site[0] = x; site[1] = y; site[2] = z;
         dia_nn(s, t, site, nn);
+        s->last_key = -2;
         add_mc(s, site, nn, 4, PE(s, sp, 2, 0, 0, 0, 0, 0));
       }
   for (int y = 0; y < s->ly; y++)
@@ -1613,7 +1638,7 @@ static void refresh_second(orc_sim *s, const int32_t site[3]) {
     double prob = 0.0;
     get_prob(s, 1, ibits(w, POS_SIC, LEN_SIC), 0, coor, q, s->ad[ind].nn, &iev, &prob);
     s->ad[ind].ev = iev;
-    s->ad[ind].prob = prob;
+    s->ad[ind].prob = prob; s->ad[ind].key = s->last_key;
     update_tree(s, ind, prob);
   }
 }
@@ -1779,6 +1804,7 @@ int orc_set_frozen(orc_sim *s, int z) {
   for (int32_t i = 1; i <= s->n_ad; i++)
     if (s->ad[i].xyz[2] < z && s->ad[i].ev == 4) {
       s->ad[i].prob = 1e-300;
+      s->ad[i].key = -1;
       update_tree(s, i, 1e-300);
     }
   s->have_jb = 0;
@@ -1799,9 +1825,108 @@ int orc_set_second_factor(orc_sim *s, const double *f2) {
     double prob = 0.0;
     get_prob(s, 1, ibits(w, POS_SIC, LEN_SIC), 0, coor, s->ad[i].xyz, s->ad[i].nn, &iev, &prob);
     s->ad[i].ev = iev;
-    s->ad[i].prob = prob;
+    s->ad[i].prob = prob; s->ad[i].key = s->last_key;
     update_tree(s, i, prob);
   }
+  s->have_jb = 0;
+  return 0;
+}
+
+/* ---- trajectory likelihood (no reference counterpart) ----------------------
+ * The restatement the engine's mskg_adjoint is checked against, computed by
+ * brute force: before every event the whole MC list is scanned. */
+
+/* Get_Prob for every MC particle (Get_Prob.f:54-331 with its side effects:
+ * an adsorption site redraws its species), so that every stored rate and
+ * class key comes from the current state. */
+int orc_refresh_all(orc_sim *s) {
+  if (setjmp(s->jb)) { s->have_jb = 0; return 2; }
+  s->have_jb = 1;
+  for (int32_t i = 1; i <= s->n_ad; i++) {
+    const int32_t *x = s->ad[i].xyz;
+    int32_t w = LC(s, x);
+    int iev = 0;
+    double prob = 0.0;
+    get_prob(s, ibits(w, POS_OCC, LEN_OCC), ibits(w, POS_SIC, LEN_SIC), ibits(w, POS_INDEX, LEN_INDEX),
+             ibits(w, POS_COOR, LEN_COOR), x, s->ad[i].nn, &iev, &prob);
+    if (iev == 0 || !(prob > 0.0)) fail(s, "orc_refresh_all: MC particle (%d,%d,%d) without a rate", x[0], x[1], x[2]);
+    s->ad[i].ev = iev;
+    s->ad[i].prob = prob;
+    s->ad[i].key = s->last_key;
+    update_tree(s, i, prob);
+  }
+  s->have_jb = 0;
+  return 0;
+}
+
+/* Replays log[0..n) (as orc_apply) with waiting times dt[k] and returns,
+ * with the numbering of mskg_adjoint: S[ADJ_NS] selections per class,
+ * A[KEY_N] = sum_k (MC particles of class c before event k) * dt[k], and
+ * scal = (sum dt, sum_k (isolated-atom rates before event k) * dt[k],
+ * sum of log rates of selected isolated atoms, sum_k [log r(event k) -
+ * Tree(1) dt[k]]), the last the log-likelihood evaluated directly from the
+ * reference's tree. */
+/* Neumaier's compensated summation: the finite-difference checks subtract
+ * two log-likelihoods of ~1e5 that differ in the ~1e-4. */
+typedef struct { double sum, c; } ksum;
+static void kadd(ksum *k, double v) {
+  double t = k->sum + v;
+  if (fabs(k->sum) >= fabs(v)) k->c += (k->sum - t) + v;
+  else k->c += (v - t) + k->sum;
+  k->sum = t;
+}
+static double kval(const ksum *k) { return k->sum + k->c; }
+
+int orc_adjoint_stats(orc_sim *s, const orc_event *evs, int64_t n, const double *dt, double *S, double *A,
+                      double *scal) {
+  if (setjmp(s->jb)) { s->have_jb = 0; s->force_arm = -1; return 2; }
+  s->have_jb = 1;
+  memset(S, 0, sizeof(double) * ADJ_NS);
+  memset(A, 0, sizeof(double) * KEY_N);
+  ksum ttot = {0, 0}, iso_int = {0, 0}, iso_log = {0, 0}, logl = {0, 0};
+  for (int64_t k = 0; k < n; k++) {
+    const double d = dt[k];
+    kadd(&ttot, d);
+    for (int32_t i = 1; i <= s->n_ad; i++) {
+      const int key = s->ad[i].key;
+      if (key == KEY_ISO) kadd(&iso_int, s->ad[i].prob * d);
+      else if (key >= 0) A[key] += d;
+      else if (key == -2) fail(s, "orc_adjoint_stats: a rate class the likelihood does not cover");
+    }
+    kadd(&logl, -s->tree[1] * d);
+    const int32_t *site = evs[k].site;
+    int32_t ind = LI(s, site);
+    if (ind < 1 || ind > s->n_ad || !eq3(s->ad[ind].xyz, site))
+      fail(s, "orc_adjoint_stats: event %lld at (%d,%d,%d) is not an MC particle", (long long)k, site[0], site[1],
+           site[2]);
+    const int key = s->ad[ind].key, it = evs[k].itrans;
+    double r;
+    if (key >= KEY_ADS && key < KEY_PH) {
+      const int c = key - KEY_ADS + 1;
+      if (it < 1 || it > 3) fail(s, "orc_adjoint_stats: adsorption site without a deposition event");
+      S[KEY_N + 3 * (it - 1) + (c - 1)] += 1.0;
+      r = PD(s, it, c);
+    } else if (key == KEY_ISO) {
+      S[KEY_ISO] += 1.0;
+      r = s->ad[ind].prob;
+      kadd(&iso_log, log(r));
+    } else if (key >= 0) {
+      S[key] += 1.0;
+      r = s->ad[ind].prob;
+    } else {
+      fail(s, "orc_adjoint_stats: event %lld on a site without a rate class", (long long)k);
+      r = 0.0;
+    }
+    kadd(&logl, log(r));
+    s->force_arm = evs[k].arm;
+    do_execute(s, ind, it);
+    s->force_arm = -1;
+    s->iter++;
+  }
+  scal[0] = kval(&ttot);
+  scal[1] = kval(&iso_int);
+  scal[2] = kval(&iso_log);
+  scal[3] = kval(&logl);
   s->have_jb = 0;
   return 0;
 }

@@ -105,6 +105,19 @@ int orc_set_frozen(orc_sim *s, int z);
  * a fresh rate.  f2 = NULL switches it off.  No reference counterpart. */
 int orc_set_second_factor(orc_sim *s, const double *f2);
 
+/* Get_Prob for every MC particle, storing rate, event and rate-class key
+ * (an adsorption site redraws its species, consuming ran3). */
+int orc_refresh_all(orc_sim *s);
+
+/* Trajectory-likelihood statistics by brute force (the check for the
+ * engine's mskg_adjoint, same class numbering): replays evs[0..n) with
+ * waiting times dt and fills S[2510], A[2501] and scal[4] = (sum dt,
+ * isolated-rate integral, isolated log-rate sum, log-likelihood evaluated
+ * directly as sum_k log r(event k) - Tree(1) dt[k]).  Models without
+ * coverage species and LA rates.  No reference counterpart. */
+int orc_adjoint_stats(orc_sim *s, const orc_event *evs, int64_t n, const double *dt, double *S, double *A,
+                      double *scal);
+
 /* Write the main xyz frame of WriteMolMolXYZFile.f (and its SiC
  * wrong-site side effect on LattCoo) exactly as the reference does. */
 int orc_write_frame(orc_sim *s, const char *xyz_path, const char *d_path,

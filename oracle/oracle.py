@@ -45,6 +45,8 @@ def _load():
         L.orc_apply.argtypes = [P, P, I, I]
         L.orc_set_frozen.argtypes = [P, I]
         L.orc_set_second_factor.argtypes = [P, P]
+        L.orc_refresh_all.argtypes = [P]
+        L.orc_adjoint_stats.argtypes = [P, P, I64, P, P, P, P]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
         L.orc_lattcoo.restype = C.POINTER(C.c_int32)
         L.orc_lattcoo.argtypes = [P]
@@ -137,6 +139,24 @@ class Oracle:
             assert self._f2.shape == (13,)
             rc = self.L.orc_set_second_factor(self.h, _p(self._f2))
         self._check(rc, "orc_set_second_factor")
+
+    def refresh_all(self):
+        """Get_Prob for every MC particle (fresh rates and class keys)."""
+        self._check(self.L.orc_refresh_all(self.h), "orc_refresh_all")
+
+    def adjoint_stats(self, log, dt):
+        """Brute-force likelihood statistics of a trajectory replayed from the
+        current state: S (2510,), A (2501,), scal (4,) = (sum dt, isolated-rate
+        integral, isolated log-rate sum, direct log-likelihood)."""
+        log = np.ascontiguousarray(log, EVENT_DTYPE)
+        dt = np.ascontiguousarray(dt, np.float64)
+        assert len(dt) == len(log)
+        S = np.zeros(2510)
+        A = np.zeros(2501)
+        sc = np.zeros(4)
+        self._check(self.L.orc_adjoint_stats(self.h, _p(log), len(log), _p(dt), _p(S), _p(A), _p(sc)),
+                    "orc_adjoint_stats")
+        return S, A, sc
 
     def apply_log(self, log):
         """Execute a logged event sequence (e.g. the engine's) in order."""
