@@ -38,6 +38,8 @@ constexpr int WAVE = 64;
 // Per-sweep accumulators, striped over slots so that thousands of tile
 // waves do not serialise on one address; k_sweep_end folds them.
 constexpr int NSLOT = 128;
+// Device-side stop at the run's event or time limit (reported as MSKG_STOP_LIMIT).
+constexpr int STOP_AT_LIMIT = 100;
 struct Slot {
   unsigned long long rmax;  // bit pattern of the largest (non-negative) tile rate
   double rsum;
@@ -58,6 +60,8 @@ struct RunState {
   unsigned long long sweeps_run;  // sweeps that advanced the clock
   long long n_mc;                 // MC particles at the end of the last sweep
   unsigned long long log_n;
+  double max_time;                  // the run's limits: the sweep that reaches one stops the run
+  unsigned long long max_events;
   Slot slot[NSLOT];
   int err;
   int err_site[3];
@@ -563,6 +567,7 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
   if (!st->stop) {
     if (nmc <= 20) st->stop = MSKG_STOP_FEW_MC;
     else if (!(rmax > 0.0)) st->stop = MSKG_STOP_NO_EVENTS;
+    else if (st->clock >= st->max_time || st->events >= st->max_events) st->stop = STOP_AT_LIMIT;
   }
 }
 
@@ -1435,6 +1440,8 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
     memset(s0.slot, 0, sizeof s0.slot);
     s0.tau_policy = e->cfg.tau_policy;
     s0.events_per_pass = e->cfg.events_per_pass > 0 ? e->cfg.events_per_pass : 1.0;
+    s0.max_time = max_time;
+    s0.max_events = st.events + (unsigned long long)max_events;
     HIPCHK(e, hipMemcpyAsync(e->d_st, &s0, sizeof s0, hipMemcpyHostToDevice, e->stream));
   }
   // tile totals of the current rates, then the first window
@@ -1476,7 +1483,7 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
     if (rc) return rc;
     if (st.err) break;
     int64_t done = (int64_t)(st.events - ev_start);
-    if (st.stop) { stop = st.stop; break; }
+    if (st.stop) { stop = st.stop == STOP_AT_LIMIT ? MSKG_STOP_LIMIT : st.stop; break; }
     if (done >= max_events || st.clock >= max_time) break;
     // size the next batch of sweeps from the observed events per sweep
     double per = (double)done / (double)sweeps;

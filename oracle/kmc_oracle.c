@@ -1683,6 +1683,18 @@ int orc_step(orc_sim *s, orc_event *ev) {
   return 0;
 }
 
+int orc_run(orc_sim *s, int64_t max_events, double max_time, int64_t *done) {
+  int64_t n = 0;
+  int rc = 0;
+  while (n < max_events && s->time < max_time) {
+    rc = orc_step(s, NULL);
+    if (rc) break;
+    n++;
+  }
+  if (done) *done = n;
+  return rc;
+}
+
 int orc_apply(orc_sim *s, const int32_t site[3], int32_t itrans, int32_t arm) {
   if (setjmp(s->jb)) { s->have_jb = 0; s->force_arm = -1; return 2; }
   s->have_jb = 1;

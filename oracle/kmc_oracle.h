@@ -84,6 +84,12 @@ int orc_main(orc_sim *s, const char *outdir, orc_event *log, int64_t log_cap,
  * executed event, 1 for the early stop (NumAdAtom<=20), 2 on error. */
 int orc_step(orc_sim *s, orc_event *ev);
 
+/* orc_step repeated until max_events events have run or the clock has
+ * reached max_time (checked before each event, so the run ends with the
+ * first event at or past max_time, as main.f:363-366 does).  Returns as
+ * orc_step; *done receives the events executed. */
+int orc_run(orc_sim *s, int64_t max_events, double max_time, int64_t *done);
+
 /* Execute one given event (Ind_Event itrans at the MC particle at site, with
  * the stacking choice forced to arm when >= 0) with the reference's update
  * rules: replays an event log produced elsewhere, e.g. by the engine's

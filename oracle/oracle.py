@@ -46,6 +46,7 @@ def _load():
         L.orc_set_frozen.argtypes = [P, I]
         L.orc_set_second_factor.argtypes = [P, P]
         L.orc_refresh_all.argtypes = [P]
+        L.orc_run.argtypes = [P, I64, D, P]
         L.orc_adjoint_stats.argtypes = [P, P, I64, P, P, P, P]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
         L.orc_lattcoo.restype = C.POINTER(C.c_int32)
@@ -139,6 +140,16 @@ class Oracle:
             assert self._f2.shape == (13,)
             rc = self.L.orc_set_second_factor(self.h, _p(self._f2))
         self._check(rc, "orc_set_second_factor")
+
+    def run(self, max_events, max_time=float("inf")):
+        """Serial loop until max_events or the clock reaches max_time;
+        returns (events, stop) with stop 0 limit, 1 few MC particles."""
+        done = C.c_int64(0)
+        rc = self.L.orc_run(self.h, int(max_events), float(max_time) if np.isfinite(max_time) else 1e300,
+                            C.byref(done))
+        if rc == 2:
+            self._check(rc, "orc_run")
+        return done.value, rc
 
     def refresh_all(self):
         """Get_Prob for every MC particle (fresh rates and class keys)."""

@@ -59,3 +59,37 @@ def parity_mask(box):
 def list_map(d):
     """{site tuple: neighbour tuple} of a list dict."""
     return {tuple(int(v) for v in s): tuple(int(v) for v in n.reshape(-1)) for s, n in zip(d["xyz"], d["nn"])}
+
+
+def oracle_run_to_time(o, t_end):
+    """Run the oracle's serial loop until its clock reaches t_end (or it
+    stops); returns the number of events."""
+    n, _ = o.run(10 ** 15, t_end)
+    return n
+
+
+def oracle_observables(args):
+    """Worker (separate process): run one deck through the oracle to t_end
+    with RunType 'R' and seed idum, return the film observables."""
+    import os as _os
+    import tempfile as _tf
+
+    from mulskips_amd import deck as _deck
+    from mulskips_amd import observables as _ob
+    from oracle.oracle import Oracle as _Oracle
+    path, box, idum, t_end, z0, frozen = args
+    d = _deck.read_start_dat(path)
+    d.run_type, d.idum = "R", int(idum)
+    with _tf.TemporaryDirectory() as tmp:
+        p = _os.path.join(tmp, "start.dat")
+        _deck.write_start_dat(d, p)
+        o = _Oracle(*box)
+        o.load_deck(p)
+        if frozen:
+            o.set_frozen(frozen)
+        n = oracle_run_to_time(o, t_end)
+        r = _ob.film(o.lattcoo(), box, z0)
+        r["events"] = n
+        r["time"] = o.time()
+        o.close()
+    return r
