@@ -62,7 +62,120 @@ def parse():
     ap.add_argument("--events-per-pass", type=float, default=1.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--replicas", type=int, default=256, help="batched-ensemble leg (0 = skip)")
+    ap.add_argument("--trajectories", type=int, default=64, help="adjoint leg (0 = skip)")
+    ap.add_argument("--trajectory-events", type=int, default=200000)
     return ap.parse_args()
+
+
+def ensemble(n, seed=2026):
+    """The batched-ensemble replicas: Si(1-x)Ge(x) (001) slabs of uneven
+    lateral size (4..32 conventional cells) and height, T uniform in
+    [900, 1500] K, flux log-uniform in [0.1, 10] ML/s, Ge fraction uniform
+    in [0, 0.5] (random substitutional alloy, also the Ge share of the
+    flux), pair energies Ge-Ge 0.45 / Si-Ge 0.5 / Si-Si 0.55 eV, a frozen
+    substrate; each replica runs for the KMC time in which 20 ML are
+    deposited (20 / flux)."""
+    import numpy as np
+
+    from mulskips_amd import models
+    rng = np.random.default_rng(seed)
+    reps = []
+    for i in range(n):
+        lc = int(rng.choice([4, 8, 16, 32]))
+        hc = int(rng.choice([8, 12, 16]))
+        temp = float(rng.uniform(900.0, 1500.0))
+        flux = float(10 ** rng.uniform(-1.0, 1.0))
+        x = float(rng.uniform(0.0, 0.5))
+        tab = models.pair_energy_tables([[0.55, 0.5], [0.5, 0.45]], [flux * (1 - x), flux * x], temp, 1e13, (14, 32))
+        reps.append(dict(box=(12 * lc, 12 * lc, 12 * hc), tables=tab, x=x, t_end=20.0 / flux, seed=10_000 + i))
+    return reps
+
+
+def batched_leg(n):
+    """Aggregate events/s of the ensemble in one launch (serial BKL, one
+    workgroup per replica) against the same replicas run one after another
+    through the single-lattice synchronous-sublattice engine."""
+    import numpy as np
+
+    from mulskips_amd.engine import Engine, run_serial
+    reps = ensemble(n)
+
+    def make(r):
+        e = Engine(r["box"], r["tables"], seed=r["seed"], tile=24, frozen_below_z=36)
+        e.build_alloy_slab(48, r["x"])
+        e.refresh_rates()
+        return e
+
+    # sequential plain launches
+    seq_events, seq_wall = 0, 0.0
+    for r in reps:
+        e = make(r)
+        t0 = time.perf_counter()
+        k, _, _ = e.run(10 ** 12, max_time=r["t_end"])
+        seq_wall += time.perf_counter() - t0
+        seq_events += k
+        e.close()
+    # one batched launch
+    engs = [make(r) for r in reps]
+    t0 = time.perf_counter()
+    done, clock, stop = run_serial(engs, [10 ** 12] * n, [r["t_end"] for r in reps])
+    wall = time.perf_counter() - t0
+    for e in engs:
+        e.close()
+    sizes = sorted({r["box"][0] // 12 for r in reps})
+    return {"replicas": n, "events": int(done.sum()), "value": float(done.sum()) / wall, "unit": "events/s",
+            "wall_s": wall, "sequential_plain": {"events": int(seq_events), "value": seq_events / seq_wall,
+                                                 "wall_s": seq_wall},
+            "speedup": (float(done.sum()) / wall) / (seq_events / seq_wall),
+            "workload": f"{n} SiGe replicas, lateral {sizes} cells, heights 8-16 cells, 20 ML deposited each, "
+                        "one launch (one workgroup per replica, serial BKL) vs sequential mskg_run"}
+
+
+def adjoint_leg(n_traj, n_events):
+    """Adjoint gradients/s: n_traj SiC trajectories (144^3 grid, second-
+    neighbour model, frozen substrate) recorded by the serial kernel, then
+    the adjoint kernel's class statistics and the 8-parameter gradient of
+    each trajectory's log-likelihood."""
+    import numpy as np
+    import torch
+
+    from mulskips_amd import adjoint
+    from mulskips_amd.engine import Engine, run_serial
+    theta0 = adjoint.theta_sic()
+    tab = adjoint.tables_from_theta(theta0, 1600.0)
+    box = (144, 144, 144)
+
+    def make(seed, log):
+        e = Engine(box, tab, seed=seed, tile=24, frozen_below_z=36)
+        e.build_slab(1, 48)
+        e.refresh_rates()
+        if log:
+            e.set_event_log(n_events)
+        return e
+
+    gens = [make(500 + i, True) for i in range(n_traj)]
+    t0 = time.perf_counter()
+    run_serial(gens, [n_events] * n_traj)
+    t_gen = time.perf_counter() - t0
+    logs = [g.event_log() for g in gens]
+    for g in gens:
+        g.close()
+    dts = [adjoint.waiting_times(lg) for lg in logs]
+    engs = [make(0, False) for _ in range(n_traj)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S, A, sc = adjoint.trajectory_stats(engs, logs, dts)
+    grads = [adjoint.gradient(theta0, adjoint.TrajectoryStats(S[i], A[i], sc[i]), 1600.0)[1] for i in range(n_traj)]
+    t_adj = time.perf_counter() - t0
+    for e in engs:
+        e.close()
+    g = torch.stack(grads).numpy()
+    return {"value": n_traj * len(theta0) / t_adj, "unit": "gradients/s (trajectories x parameters / s)",
+            "trajectories": n_traj, "events_per_trajectory": n_events, "parameters": list(adjoint.PARAMS),
+            "adjoint_s": t_adj, "record_s": t_gen, "record_events_per_s": n_traj * n_events / t_gen,
+            "mean_gradient": [float(v) for v in g.mean(axis=0)],
+            "workload": "3C-SiC second-neighbour model, 144^3 grid, frozen 40-ML-equivalent substrate, 1600 K"}
 
 
 def cpu_baseline(tables, substrate_len, seconds):
@@ -142,7 +255,7 @@ def main():
     achieved = npts * SWEEP_BYTES_PER_POINT / (sweep_ms * 1e-3) / 1e9
     tpp = measured_traffic_per_point()
     out = {
-        "metric": "KMC events/s at 512^3 SiC (1 GPU)",
+        "metric": "KMC events/s at 512^3 SiC (plain vs batched, 1 GPU); adjoint grads/s; %HBM peak",
         "value": total / wall,
         "unit": "events/s",
         "n_gpus": 1,
@@ -170,6 +283,13 @@ def main():
                    "counts": [int(c) for c in counts]},
     }
     eng.close()
+    if a.replicas > 0:
+        out["batched"] = batched_leg(a.replicas)
+        print(f"# batched leg: {out['batched']['value']:.4g} events/s, {out['batched']['speedup']:.2f}x sequential",
+              file=sys.stderr, flush=True)
+    if a.trajectories > 0:
+        out["adjoint"] = adjoint_leg(a.trajectories, a.trajectory_events)
+        print(f"# adjoint leg: {out['adjoint']['value']:.4g} gradients/s", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tables, substrate, a.cpu_seconds)
     print(json.dumps(out), flush=True)

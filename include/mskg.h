@@ -119,6 +119,13 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt_coo,
  * empty site is drawn when it is filled rather than when the slab is built. */
 int mskg_build_slab(mskg_engine *e, int32_t mode, int32_t len);
 
+/* The same slab as mode 0 (the SetSi structure with its walls) as a random
+ * substitutional alloy: every lattice site holds species 2 (Ge in SiGe)
+ * with probability `fraction`, drawn from the engine's Philox stream --
+ * the random alloy of initial/SetSiGex.f, which draws from the compiler's
+ * RAND().  Needs n_crystal >= 2.  The atoms' rates come from Get_Prob. */
+int mskg_build_alloy_slab(mskg_engine *e, int32_t len, double fraction);
+
 /* Per-site temperature in K for laser-annealing mode, (len_x, len_y,
  * len_z+1) like LattCoo's T bits (SetT.f:25-50). */
 int mskg_set_temperature(mskg_engine *e, const int32_t *temp_k);
@@ -181,9 +188,11 @@ int mskg_last_timing(mskg_engine *e, double *run_ms, double *sweep_ms, int64_t *
  * event (Philox-4x32-10, key = the engine's seed).  Engines keep their own
  * state, counters and event logs (the log's time is the clock after each
  * event and rtot the R of its selection).  All engines must live on the
- * current device; the launch uses engines[0]'s stream.  max_events[i] bounds
- * replica i.  stop_reason as mskg_run; the few-MC stop does not apply. */
-int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_events, double max_time,
+ * current device; the launch uses engines[0]'s stream.  max_events[i] and
+ * max_time[i] bound replica i.  stop_reason as mskg_run; the few-MC stop
+ * does not apply.  A replica's trajectory does not depend on the batch it
+ * runs in. */
+int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_events, const double *max_time,
                     int64_t *events_done, double *kmc_time, int32_t *stop_reason);
 
 /* Trajectory likelihood statistics (no reference counterpart; for parameter

@@ -17,7 +17,7 @@ EXPORTS = [
     "mskg_create", "mskg_destroy", "mskg_last_error", "mskg_load_state", "mskg_set_temperature",
     "mskg_refresh_rates", "mskg_run", "mskg_replay", "mskg_store_state", "mskg_site_rates",
     "mskg_counters", "mskg_last_timing", "mskg_geometry_find_nn", "mskg_geometry_find_sn",
-    "mskg_build_slab", "mskg_set_event_log", "mskg_get_event_log", "mskg_run_serial", "mskg_adjoint",
+    "mskg_build_slab", "mskg_set_event_log", "mskg_get_event_log", "mskg_run_serial", "mskg_adjoint", "mskg_build_alloy_slab",
 ]
 
 # rate-class keys of the trajectory likelihood (include/mskg.h)
@@ -73,7 +73,8 @@ def lib():
     L.mskg_build_slab.argtypes = [P, I32, I32]
     L.mskg_set_event_log.argtypes = [P, I64]
     L.mskg_get_event_log.argtypes = [P, P, I64, P]
-    L.mskg_run_serial.argtypes = [P, I32, P, D, P, P, P]
+    L.mskg_run_serial.argtypes = [P, I32, P, P, P, P, P]
+    L.mskg_build_alloy_slab.argtypes = [P, I32, D]
     L.mskg_adjoint.argtypes = [P, I32, P, P, P, P, P, P]
     L.mskg_geometry_find_nn.argtypes = [P, P, P, I32, I32, P]
     L.mskg_geometry_find_sn.argtypes = [P, P, P, I32, I32, P, P]

@@ -591,7 +591,8 @@ __constant__ int c_jump_dia[8][3] = {{3, 3, 3}, {3, -3, -3}, {-3, -3, 3}, {-3, 3
 // A, C on B, no walls): bulk below len-3, the top atomic layer as MC
 // particles with PTransE(s,2,0,...), the empty sites above with
 // coordination 2 and their deposition rate.  Every point is written.
-__global__ void __launch_bounds__(256) k_build_slab(Lattice L, int len, int sic) {
+__global__ void __launch_bounds__(256) k_build_slab(Lattice L, int len, int sic, double fraction, uint32_t key0,
+                                                   uint32_t key1) {
   const Tables &t = *L.tab;
   const int64_t n = L.box.npts;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
@@ -602,12 +603,18 @@ __global__ void __launch_bounds__(256) k_build_slab(Lattice L, int len, int sic)
     s.kc = 0; s.valid = 0; s.temp = 0;
     int cls = CLS_NONE;
     double r = 0.0;
-    const bool wall = !sic && ((z >= 0 && z <= 5) || (z >= L.box.lz - 6 && z <= L.box.lz - 1));
+    const bool wall = sic != 1 && ((z >= 0 && z <= 5) || (z >= L.box.lz - 6 && z <= L.box.lz - 1));
     const int ty = dia_type(x, y, z);
     if (wall) {
       s.kc = KIND_WALL;
     } else if (ty && z <= len) {
-      const int sp = (sic && ty == 2) ? 2 : 1;
+      int sp = (sic == 1 && ty == 2) ? 2 : 1;
+      if (sic == 2) {  // random substitutional alloy: species 2 with probability `fraction`
+        U4 c;
+        c.x = (uint32_t)p; c.y = (uint32_t)(p >> 32); c.z = 0x5e6e0001u; c.w = 0;
+        const U4 d = philox4x32_10(c, key0, key1);
+        if (u01_open(d.x, d.y) < fraction) sp = 2;
+      }
       NL l;
       for (int i = 0; i < 4; i++) {
         const int *j = c_jump_dia[(ty == 1 ? 0 : 4) + i];
@@ -744,7 +751,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
   unsigned long long events = st->events, nev = 0;
   int stop = MSKG_STOP_LIMIT, err = 0;
   U4 ctr;
-  ctr.x = (uint32_t)D.replica;
+  ctr.x = 0x5e71a1u;  // the stream depends on the engine's seed only, not on its place in the batch
   while ((long long)nev < D.max_events && clock < D.max_time) {
     double part = 0.0;
     for (int b = lane; b < D.nblk; b += WAVE) part += D.bsum[b];
@@ -1631,16 +1638,30 @@ int mskg_last_timing(mskg_engine *e, double *run_ms, double *sweep_ms, int64_t *
   return MSKG_OK;
 }
 
+static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction);
+
 int mskg_build_slab(mskg_engine *e, int32_t mode, int32_t len) {
   if (!e || !e->d_site) return MSKG_EINVAL;
   if (mode != 0 && mode != 1) return fail_(e, MSKG_EINVAL, "mode must be 0 (SetSi) or 1 (SetSiC3C)");
-  if (len < 6 || len > e->box.lz - (mode == 0 ? 7 : 1) || len % 3)
+  return build_slab(e, mode, len, 0.0);
+}
+
+int mskg_build_alloy_slab(mskg_engine *e, int32_t len, double fraction) {
+  if (!e || !e->d_site) return MSKG_EINVAL;
+  if (!(fraction >= 0.0 && fraction <= 1.0)) return fail_(e, MSKG_EINVAL, "alloy fraction must be in [0, 1]");
+  if (e->ncrystal < 2) return fail_(e, MSKG_EINVAL, "an alloy needs n_crystal >= 2");
+  return build_slab(e, 2, len, fraction);
+}
+
+static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction) {
+  if (len < 6 || len > e->box.lz - (mode != 1 ? 7 : 1) || len % 3)
     return fail_(e, MSKG_EINVAL, "slab height %d does not fit the box height %d", len, e->box.lz);
   Lattice L = make_lattice(e);
   int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
-  hipLaunchKernelGGL(k_build_slab, dim3(blocks), dim3(256), 0, e->stream, L, len, mode);
+  hipLaunchKernelGGL(k_build_slab, dim3(blocks), dim3(256), 0, e->stream, L, len, mode, fraction,
+                     (uint32_t)e->cfg.seed, (uint32_t)(e->cfg.seed >> 32));
   HIPCHK(e, hipGetLastError());
-  if (e->tab.use_f2) {  // the surface atoms' rates take their second-neighbour factor
+  if (e->tab.use_f2 || mode == 2) {  // the surface atoms' rates from their neighbours (species, second shell)
     HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
     hipLaunchKernelGGL(k_rate_sweep, dim3(blocks), dim3(256), 0, e->stream, L, 0.0, e->d_err, 1);
     HIPCHK(e, hipGetLastError());
@@ -1725,9 +1746,9 @@ int mskg_geometry_find_sn(const int32_t site[3], const int32_t sitec[3], const i
   return MSKG_OK;
 }
 
-int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_events, double max_time,
+int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_events, const double *max_time,
                     int64_t *events_done, double *kmc_time, int32_t *stop_reason) {
-  if (!engines || n <= 0 || !max_events) return MSKG_EINVAL;
+  if (!engines || n <= 0 || !max_events || !max_time) return MSKG_EINVAL;
   mskg_engine *e0 = engines[0];
   if (!e0) return MSKG_EINVAL;
   std::vector<SerialDesc> descs(n);
@@ -1768,7 +1789,7 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
     D.log = e->d_log;
     D.log_cap = (unsigned long long)e->log_cap;
     D.max_events = max_events[i];
-    D.max_time = max_time;
+    D.max_time = max_time[i];
     D.key0 = (uint32_t)e->cfg.seed;
     D.key1 = (uint32_t)(e->cfg.seed >> 32) ^ 0x5e1a1u;  // a stream apart from the sublattice passes
     D.ntiles = e->ntiles;

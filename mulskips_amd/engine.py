@@ -118,6 +118,10 @@ class Engine:
         1 = SetSiC3C (SiC)."""
         self._check(self.L.mskg_build_slab(self.h, int(mode), int(length)))
 
+    def build_alloy_slab(self, length, fraction):
+        """The SetSi slab as a random alloy: species 2 with probability fraction."""
+        self._check(self.L.mskg_build_alloy_slab(self.h, int(length), float(fraction)))
+
     def set_event_log(self, capacity):
         self._check(self.L.mskg_set_event_log(self.h, int(capacity)))
 
@@ -228,11 +232,12 @@ def run_serial(engines, max_events, max_time=float("inf")):
     L = _lib.lib()
     n = len(engines)
     me = np.ascontiguousarray(np.broadcast_to(np.asarray(max_events, np.int64), (n,)))
+    mt = np.ascontiguousarray(np.broadcast_to(np.asarray(max_time, np.float64), (n,)))
+    mt = np.where(np.isfinite(mt), mt, 1e300)
     done = np.zeros(n, np.int64)
     clock = np.zeros(n, np.float64)
     stop = np.zeros(n, np.int32)
-    mt = float(max_time) if np.isfinite(max_time) else 1e300
-    rc = L.mskg_run_serial(_engine_array(engines), n, ptr(me), mt, ptr(done), ptr(clock), ptr(stop))
+    rc = L.mskg_run_serial(_engine_array(engines), n, ptr(me), ptr(mt), ptr(done), ptr(clock), ptr(stop))
     if rc:
         bad = [e for e in engines if L.mskg_last_error(e.h)]
         msg = "; ".join(L.mskg_last_error(e.h).decode() for e in bad) or "mskg_run_serial failed"
