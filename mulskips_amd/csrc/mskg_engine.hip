@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
       ch.arm = -1;
       ch.u_arm = u01_open(d2.z, d2.w);
       const int itrans = pick_itrans(L, s, lc[sel], u01_open(d2.x, d2.y));
-      const int coor0 = s_coor(L.site[s.i]);
+      const int coor0 = P.log ? s_coor(L.site[s.i]) : 0;
       __syncthreads();  // every lane has read the site before it changes
       EvResult res;
       execute(L, rtot, s, itrans, ch, res, sc, lane);
@@ -423,8 +423,8 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
         }
         int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
         if (slot >= 0 && slot < 8) cnt[slot]++;
-        for (int q = 0; q < res.ntouched; q++) s_tp[q] = res.touched[q];
-        s_nt = res.ntouched;
+        for (int q = 0; q < sc.ntouched; q++) s_tp[q] = sc.touched[q];
+        s_nt = sc.ntouched;
         s_stop = err;
       }
     }
@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
   for (int j0 = 0; j0 < n; j0 += WAVE) {
     int j = j0 + lane;
     bool keep = j < n && lc[j] != 0;
-    if (j < n && !keep) L.inlist[tile_point(L.box, g, L.tile, lk[j]).i] = 0;
+    if (j < n) L.inlist[tile_point(L.box, g, L.tile, lk[j]).i] = keep ? 1 : 0;
     if (keep && lc[j] != CLS_ISOLATED) part += (double)lr[j];
     unsigned long long m = __ballot(keep);
     int slot = nout + __popcll(m & ((1ull << lane) - 1ull));
@@ -825,9 +825,9 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
       const int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
       if (slot >= 0 && slot < 8) st->counts[slot]++;
       int nt = 0;
-      for (int q = 0; q < res.ntouched; q++) {
-        if (res.touched[q].i < 0) continue;
-        const int tq = tile_of(L, res.touched[q]);
+      for (int q = 0; q < sc.ntouched; q++) {
+        if (sc.touched[q].i < 0) continue;
+        const int tq = tile_of(L, sc.touched[q]);
         bool dup = false;
         for (int r = 0; r < nt; r++) dup |= s_tiles[r] == tq;
         if (!dup) s_tiles[nt++] = tq;
@@ -944,8 +944,8 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
     __syncthreads();
     if (lane == 0) {
       int bad = res.err;
-      for (int q = 0; q < res.ntouched && !bad; q++) {
-        const int64_t p = res.touched[q].i;
+      for (int q = 0; q < sc.ntouched && !bad; q++) {
+        const int64_t p = sc.touched[q].i;
         if (p < 0) continue;
         const int o = D.key_seen[p], nw = L.key[p];
         if (nw == KEY_UNSUPPORTED) { bad = ERR_COVERAGE; break; }

@@ -137,10 +137,11 @@ MSKG_HD void widen_window(const Lattice &L, int colour, int tz) {
 }
 // A site that becomes an MC particle joins its tile's list (once).
 MSKG_HD void enlist(const Lattice &L, const P3 &p) {
-  if (!L.inlist || L.inlist[p.i]) return;
-  L.inlist[p.i] = 1;
+  if (!L.inlist) return;
   const int t = tile_of(L, p);
-  if (t == L.cur_tile) return;  // the running tile keeps its list on chip
+  if (t == L.cur_tile) return;  // the running tile keeps its list on chip and flags it on write-back
+  if (L.inlist[p.i]) return;
+  L.inlist[p.i] = 1;
   widen_window(L, tile_colour(p.x / L.tile, p.y / L.tile, p.z / L.tile), p.z / L.tile);
   const int slot = atomic_add_i32(&L.tl_n[t], 1);
   if (slot < L.capg) L.tl_idx[(int64_t)t * L.capg + slot] = tile_offset(L, p);
@@ -204,18 +205,6 @@ MSKG_HD P3 second_pos(const Lattice &L, const P3 &p, int k) {
   if (y < 0) y += L.box.ly;
   return at(L, x, y, p.z + dz);
 }
-// Occupied (non-wall) points among them.
-MSKG_HD int second_count(const Lattice &L, const P3 &p, const Ovr *ovr) {
-  Site q[12];
-  for (int k = 0; k < 12; k++) q[k] = rd_o(L, second_pos(L, p, k).i, ovr);  // twelve gathers in flight
-  int n = 0;
-  for (int k = 0; k < 12; k++) {
-    int kq = s_kind(q[k]);
-    if (kind_occupied(kq) && kq != KIND_WALL) n++;
-  }
-  return n;
-}
-
 MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &cls,
                       double &rate, int &err, const Ovr *ovr = nullptr, int *key = nullptr) {
   const Tables &t = *L.tab;
@@ -265,8 +254,14 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   int n_si = 0, n_c = 0, n_b = 0, n_wall = 0, covnn[3] = {0, 0, 0};
   P3 qs[4];
   Site sqs[4];
+  uint8_t kc2[12];
   for (int i = 0; i < 4; i++) qs[i] = entry(L, nn, i);
   for (int i = 0; i < 4; i++) sqs[i] = rd_o(L, qs[i].i, ovr);  // four independent gathers in flight
+  if (t.use_f2)  // ... and the twelve second-shell kinds with them
+    for (int k = 0; k < 12; k++) {
+      const int64_t i2 = second_pos(L, p, k).i;
+      kc2[k] = i2 < 0 ? (uint8_t)0 : (ovr ? rd_o(L, i2, ovr).kc : L.site[i2].kc);
+    }
   for (int i = 0; i < 4; i++) {
     const P3 &q = qs[i];
     const Site &sq = sqs[i];
@@ -304,7 +299,10 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
     rate = pe_of(t, kind, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
     int n2 = 0;
     if (t.use_f2) {
-      n2 = second_count(L, p, ovr);
+      for (int k = 0; k < 12; k++) {
+        const int kq = kc2[k] & 7;
+        if (kind_occupied(kq) && kq != KIND_WALL) n2++;
+      }
       rate *= t.f2[n2];
     }
     *key = (t.ncov || t.la_mode) ? KEY_UNSUPPORTED : (kind - 1) + 3 * (n_si + n_wall + 4 * (n_c + 4 * n_b)) + 192 * n2;
@@ -320,10 +318,6 @@ struct Choice {
 
 constexpr int MAX_TOUCHED = 17;
 struct EvResult {
-  P3 touched[MAX_TOUCHED];  // the site, its neighbours and (second-neighbour term) the
-                            // evaporating atoms on its second-neighbour positions: every
-                            // point whose rate may have changed
-  int ntouched;
   int arm;        // stacking choice made (-1 none)
   int err;
 };
@@ -367,6 +361,11 @@ struct EvScratch {
   int abort;
   P3 sec[12];       // second-neighbour refresh: the point refreshed by lane k (i = -1: none)
   int err2[12];
+  // every point whose rate may have changed: the site, its neighbours and
+  // (second-neighbour term) the evaporating atoms on its second-neighbour
+  // positions; complete after execute() returns
+  P3 touched[MAX_TOUCHED];
+  int ntouched;
 };
 
 #define MSKG_SYNC() __syncthreads()
@@ -391,10 +390,37 @@ __device__ inline void resolve_neighbours(const Lattice &L, EvScratch &sc, int n
   }
 }
 
-__device__ inline void touched_list(const P3 &s, const EvScratch &sc, int n, EvResult &res) {
-  res.ntouched = 0;
-  res.touched[res.ntouched++] = s;
-  for (int i = 0; i < n; i++) res.touched[res.ntouched++] = sc.pts[i];
+__device__ inline void touched_list(const P3 &s, EvScratch &sc, int n) {
+  sc.ntouched = 0;
+  sc.touched[sc.ntouched++] = s;
+  for (int i = 0; i < n; i++) sc.touched[sc.ntouched++] = sc.pts[i];
+}
+
+// Second-neighbour term: after a change of occupancy at s, the evaporating
+// atom (coordination 1..3) on second-neighbour position k of s, if any,
+// gets a fresh rate from the final state of the event.  It runs beside the
+// site's own refresh: its evaluation reads the site only for its kind (and,
+// for the rare non-bond list entry, its list), which are final by then.
+struct FinalRefresh {
+  bool active;
+  P3 p;
+  Site sp;
+  int coor;
+  NL nn;
+};
+__device__ inline void second_setup(const Lattice &L, const P3 &s, int k, EvScratch &sc, FinalRefresh &f) {
+  const P3 q = second_pos(L, s, k);
+  sc.sec[k].i = -1;
+  if (q.i < 0) return;
+  const Site sq = rd(L, q.i);
+  const int kq = s_kind(sq), cq = s_coor(sq);
+  if (!kind_occupied(kq) || kq == KIND_WALL || cq < 1 || cq > 3) return;
+  sc.sec[k] = q;
+  f.active = true;
+  f.p = q;
+  f.sp = sq;
+  f.coor = cq;
+  f.nn = list_of(L, q, sq);
 }
 
 // deposition.f:78-235, the update of neighbour i of the new atom at s.
@@ -524,7 +550,7 @@ __device__ void deposition(const Lattice &L, double rtot, const P3 &s, int speci
     }
     sc.nn = nn_site;
     resolve_neighbours(L, sc, 4);
-    touched_list(s, sc, 4, res);
+    touched_list(s, sc, 4);
   }
   MSKG_SYNC();
   if (sc.abort) {
@@ -538,6 +564,11 @@ __device__ void deposition(const Lattice &L, double rtot, const P3 &s, int speci
     dep_neighbour(L, rtot, s, lane, sc, sc.err[lane]);
   }
   MSKG_SYNC();
+  // last phase: lane 0 refreshes the site and, with the second-neighbour
+  // term, lanes 1..12 the atoms of its second shell -- one call site, so the
+  // lanes run the evaluation together
+  FinalRefresh f;
+  f.active = false;
   if (lane == 0) {
     int coor = sc.coor_old;
     for (int i = 0; i < 4; i++) {
@@ -549,7 +580,21 @@ __device__ void deposition(const Lattice &L, double rtot, const P3 &s, int speci
       s_set_coor(ss, coor);
       wr(L, s.i, ss);
     }
-    refresh(L, rtot, s, ss, coor, sc.nn, res.err);
+    f.active = true;
+    f.p = s;
+    f.sp = ss;
+    f.coor = coor;
+    f.nn = sc.nn;
+  } else if (lane <= 12 && L.tab->use_f2) {
+    second_setup(L, s, lane - 1, sc, f);
+  }
+  if (f.active) {
+    int e = ERR_NONE;
+    refresh(L, rtot, f.p, f.sp, f.coor, f.nn, e);
+    if (e) {
+      if (lane == 0) res.err = e;
+      else sc.err2[lane - 1] = e;
+    }
   }
 }
 
@@ -664,7 +709,7 @@ __device__ void evaporation(const Lattice &L, double rtot, const P3 &s, EvResult
     wr(L, s.i, ss);
     sc.nn = list_of(L, s, ss);
     resolve_neighbours(L, sc, 4);
-    touched_list(s, sc, 4, res);
+    touched_list(s, sc, 4);
   }
   MSKG_SYNC();
   int nbuff[3] = {0, 0, 0};
@@ -678,35 +723,54 @@ __device__ void evaporation(const Lattice &L, double rtot, const P3 &s, EvResult
     evap_neighbour(L, rtot, s, lane, sc, sc.err[lane]);
   }
   MSKG_SYNC();
+  // last phase, as in deposition: lane 0 the site, lanes 1..12 the second
+  // shell, one Get_Prob call site
+  FinalRefresh f;
+  f.active = false;
+  Site ss;
+  int coor = 0;
   if (lane == 0) {
-    int coor = sc.coor_old;
+    coor = sc.coor_old;
     for (int i = 0; i < 4; i++) {
       coor += sc.inc[i];
       if (sc.err[i]) res.err = sc.err[i];
     }
     if (!sc.seq)
       for (int i = 0; i < 4; i++) evap_nbuff_one(L, sc, i, nbuff, res.err);
-    Site ss = rd(L, s.i);
+    ss = rd(L, s.i);
     s_set_coor(ss, coor);
     if (coor == 0) {  // EraseMC zeroes the record
       for (int a = 0; a < 4; a++) ss.d[a][0] = ss.d[a][1] = ss.d[a][2] = 0;
       ss.valid = 0;
       wr(L, s.i, ss);
       set_rate(L, s, CLS_NONE, 0.0);
-      return;
-    }
-    wr(L, s.i, ss);
-    int cls = 0, key = -1;
-    double r = 0.0;
-    get_prob(L, rtot, s, ss, coor, sc.nn, cls, r, res.err, nullptr, &key);
-    if (coor == 1) {
-      int pos[3];
-      if (nbuff_of(L, s, sc.nn, pos))
-        for (int k = 0; k < 3; k++) nbuff[k] = pos[k];
-      if (!one_entry_list(L, ss, s, nbuff)) { res.err = ERR_GEOM; return; }
+    } else {
       wr(L, s.i, ss);
+      f.active = true;
+      f.p = s;
+      f.sp = ss;
+      f.coor = coor;
+      f.nn = sc.nn;
     }
-    set_rate(L, s, cls, r, key);
+  } else if (lane <= 12 && L.tab->use_f2) {
+    second_setup(L, s, lane - 1, sc, f);
+  }
+  int cls = 0, key = -1, e = ERR_NONE;
+  double r = 0.0;
+  if (f.active) get_prob(L, rtot, f.p, f.sp, f.coor, f.nn, cls, r, e, nullptr, &key);
+  if (lane == 0 && f.active && coor == 1) {  // the site keeps one reciprocal bond (NBuff)
+    int pos[3];
+    if (nbuff_of(L, s, sc.nn, pos))
+      for (int k = 0; k < 3; k++) nbuff[k] = pos[k];
+    if (!one_entry_list(L, ss, s, nbuff)) e = ERR_GEOM;
+    else wr(L, s.i, ss);
+  }
+  if (f.active) {
+    set_rate(L, f.p, cls, r, key);
+    if (e) {
+      if (lane == 0) res.err = e;
+      else sc.err2[lane - 1] = e;
+    }
   }
 }
 
@@ -720,7 +784,7 @@ __device__ void absorption(const Lattice &L, double rtot, const P3 &s, int slot,
     wr(L, s.i, ss);
     sc.nn = list_of(L, s, ss);
     resolve_neighbours(L, sc, 4);
-    touched_list(s, sc, 4, res);
+    touched_list(s, sc, 4);
   }
   MSKG_SYNC();
   for (int i = sc.seq ? 0 : lane; i < 4; i += sc.seq ? 1 : 64) {
@@ -752,7 +816,7 @@ __device__ void desorption(const Lattice &L, double rtot, const P3 &s, EvResult 
     sc.nn = list_of(L, s, ss);
     const int nloop = sc.coor_old == 1 ? 1 : 4;
     resolve_neighbours(L, sc, nloop);
-    touched_list(s, sc, nloop, res);
+    touched_list(s, sc, nloop);
   }
   MSKG_SYNC();
   const int nloop = sc.coor_old == 1 ? 1 : 4;
@@ -809,45 +873,28 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   return r;
 }
 
-// Second-neighbour term: after a change of occupancy at s, the evaporating
-// atom (coordination 1..3) on second-neighbour position k of s, if any,
-// gets a fresh rate from the final state of the event.
-__device__ void second_refresh(const Lattice &L, double rtot, const P3 &s, int k, EvScratch &sc) {
-  const P3 q = second_pos(L, s, k);
-  sc.sec[k].i = -1;
-  sc.err2[k] = 0;
-  if (q.i < 0) return;
-  const Site sq = rd(L, q.i);
-  const int kq = s_kind(sq), cq = s_coor(sq);
-  if (!kind_occupied(kq) || kq == KIND_WALL || cq < 1 || cq > 3) return;
-  refresh(L, rtot, q, sq, cq, list_of(L, q, sq), sc.err2[k]);
-  sc.sec[k] = q;
-}
-
 // Execute one event of type itrans at site s.  Called by every lane of the
 // wave with the same arguments; res is complete on lane 0.
 __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, const Choice &ch, EvResult &res,
                         EvScratch &sc, int lane) {
   res.err = ERR_NONE;
   res.arm = -1;
-  res.ntouched = 0;
+  if (lane == 0) sc.ntouched = 0;
+  if (lane < 12) {
+    sc.sec[lane].i = -1;
+    sc.err2[lane] = 0;
+  }
   if (itrans <= 3) deposition(L, rtot, s, itrans, ch, res, sc, lane);
   else if (itrans == 4) evaporation(L, rtot, s, res, sc, lane);
   else if (itrans == 5) desorption(L, rtot, s, res, sc, lane);
   else absorption(L, rtot, s, itrans - 5, res, sc, lane);
   if (L.tab->use_f2 && itrans <= 4) {
-    MSKG_SYNC();  // every lane is past the event's own updates
-    if (lane == 0) sc.abort = res.err;
-    MSKG_SYNC();
-    if (!sc.abort) {
-      if (lane < 12) second_refresh(L, rtot, s, lane, sc);
-      MSKG_SYNC();
-      if (lane == 0)
-        for (int k = 0; k < 12; k++) {
-          if (sc.err2[k]) res.err = sc.err2[k];
-          if (sc.sec[k].i >= 0) res.touched[res.ntouched++] = sc.sec[k];
-        }
-    }
+    MSKG_SYNC();  // the second-shell refreshes (run in the event's last phase) are done
+    if (lane == 0)
+      for (int k = 0; k < 12; k++) {
+        if (sc.err2[k]) res.err = sc.err2[k];
+        if (sc.sec[k].i >= 0) sc.touched[sc.ntouched++] = sc.sec[k];
+      }
   }
 }
 
