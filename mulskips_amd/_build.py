@@ -20,6 +20,10 @@ HIPCC_FLAGS = [
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
+    # every device function inlined into its kernel: no call ABI, whose
+    # callee-saved registers pin the event kernels at 256 VGPRs and one wave
+    # per SIMD; with it inlined the pass kernel fits two waves per SIMD
+    "-mllvm", "-amdgpu-early-inline-all=true", "-mllvm", "-amdgpu-function-calls=false",
 ]
 
 

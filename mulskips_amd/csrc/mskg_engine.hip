@@ -283,7 +283,9 @@ __global__ void __launch_bounds__(256) k_tile_rates(Lattice L, int ntiles, RunSt
   if (lane == 0 && cnt) slot_add(st, t, acc, (unsigned long long)cnt, 0, nullptr);
 }
 
-__global__ void __launch_bounds__(WAVE) k_pass(PassParams P) {
+// Two waves per SIMD: the window of one colour pass holds about 2048 tiles
+// at the 512^3 workload, all of them in flight at once on 256 CUs.
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) k_pass(PassParams P) {
   extern __shared__ unsigned char smem[];
   const int lane = threadIdx.x;
   Lattice L = P.L;

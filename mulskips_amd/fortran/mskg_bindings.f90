@@ -16,6 +16,8 @@ module mskg_bindings
   integer(c_int), parameter :: MSKG_OK = 0, MSKG_EINVAL = 1, MSKG_ESTATE = 2, &
                                MSKG_ECAPACITY = 3, MSKG_EDEVICE = 4, MSKG_ENOMEM = 5
   integer(c_int), parameter :: MSKG_STOP_LIMIT = 0, MSKG_STOP_FEW_MC = 1, MSKG_STOP_NO_EVENTS = 3
+  integer(c_int), parameter :: MSKG_KEY_ADS = 2496, MSKG_KEY_PH = 2499, MSKG_KEY_ISO = 2500, MSKG_KEY_N = 2501, &
+                               MSKG_ADJ_NS = 2510
 
   ! mskg_config of include/mskg.h, field for field
   type, bind(c) :: mskg_config
@@ -146,6 +148,35 @@ module mskg_bindings
       integer(c_int64_t), intent(out) :: counts(8)
       real(c_double), intent(out) :: kmc_time
       integer(c_int64_t), intent(out) :: n_adatom, n_atoms, n_voids
+    end function
+
+    integer(c_int) function mskg_build_alloy_slab(eng, len, fraction) bind(c, name="mskg_build_alloy_slab")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: eng
+      integer(c_int32_t), value :: len
+      real(c_double), value :: fraction
+    end function
+
+    ! serial BKL on n engines in one launch: engines, max_events, max_time are arrays of n
+    integer(c_int) function mskg_run_serial(engines, n, max_events, max_time, events_done, kmc_time, stop_reason) &
+        bind(c, name="mskg_run_serial")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), intent(in) :: engines(*)
+      integer(c_int32_t), value :: n
+      integer(c_int64_t), intent(in) :: max_events(*)
+      real(c_double), intent(in) :: max_time(*)
+      integer(c_int64_t), intent(out) :: events_done(*)
+      real(c_double), intent(out) :: kmc_time(*)
+      integer(c_int32_t), intent(out) :: stop_reason(*)
+    end function
+
+    ! trajectory-likelihood statistics: logs and dts are arrays of n pointers
+    integer(c_int) function mskg_adjoint(engines, n, logs, dts, n_events, S, A, scalars) bind(c, name="mskg_adjoint")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), intent(in) :: engines(*), logs(*), dts(*)
+      integer(c_int32_t), value :: n
+      integer(c_int64_t), intent(in) :: n_events(*)
+      real(c_double), intent(out) :: S(*), A(*), scalars(*)
     end function
 
     integer(c_int) function mskg_last_timing(eng, run_ms, sweep_ms, sweep_points, sweeps_done) &
