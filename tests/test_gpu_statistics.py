@@ -12,7 +12,9 @@ difference:
 * growth and stacking-fault (non-cubic site) fraction: the cube-zigzag
   regression deck (PtransZig 0.8) on a 144^3 grid;
 * growth and coverage: the CVD-Si-SiH2 regression deck (H coverage).
-The seeds are fixed, so the test is deterministic.
+Both window sizes the engine is run with are covered: about 1 and 4 events
+per tile and colour pass (tau policy 1).  The seeds are fixed, so the test
+is deterministic.
 """
 import multiprocessing as mp
 import os
@@ -53,8 +55,9 @@ def case(tmp_path, name):
     raise KeyError(name)
 
 
+@pytest.mark.parametrize("epp", [1.0, 4.0])
 @pytest.mark.parametrize("name", ["si-pvd", "cube-zigzag", "CVD-Si-SiH2"])
-def test_sublattice_matches_serial_statistics(tmp_path, name):
+def test_sublattice_matches_serial_statistics(tmp_path, name, epp):
     path, d, box, z0, frozen, t_end, keys = case(tmp_path, name)
     # serial reference: the oracle, one process per seed
     ctx = mp.get_context("spawn")
@@ -69,7 +72,8 @@ def test_sublattice_matches_serial_statistics(tmp_path, name):
     o.close()
     parallel = []
     for s in range(SEEDS):
-        eng = Engine(box, d.tables(), seed=7000 + s, tile=24, tau_policy=1, frozen_below_z=frozen)
+        eng = Engine(box, d.tables(), seed=7000 + s, tile=24, tau_policy=1, events_per_pass=epp,
+                     frozen_below_z=frozen)
         eng.load_state(lat0, mc0, bulk0, void0)
         n, t, stop = eng.run(10 ** 12, max_time=t_end)
         assert stop == 0 and t >= t_end
@@ -79,7 +83,8 @@ def test_sublattice_matches_serial_statistics(tmp_path, name):
         parallel.append(r)
         eng.close()
     res = observables.compare(parallel, serial, keys + ("events",), nsigma=2.0)
-    print(f"\n[{name}] t_end={t_end}, {SEEDS} seeds each (parallel vs serial):", flush=True)
+    print(f"\n[{name}, {epp} events per tile and pass] t_end={t_end}, {SEEDS} seeds each (parallel vs serial):",
+          flush=True)
     for k, (ma, mb, dd, bound, ok) in res.items():
         print(f"  {k:10s} {ma:14.5g} {mb:14.5g}  diff {dd:+.4g}  2se {bound:.4g}  {'ok' if ok else 'OUT'}", flush=True)
     for k in keys:
