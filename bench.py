@@ -70,6 +70,32 @@ def parse():
     return ap.parse_args()
 
 
+def stream_probe(n_floats=1 << 30, reps=10):
+    """STREAM copy and triad (torch kernels on 4 GiB FP32 arrays) on the same
+    device: what a plain bandwidth-bound kernel achieves on this part."""
+    import torch
+    a = torch.empty(n_floats, device="cuda")
+    b = torch.rand(n_floats, device="cuda")
+    c = torch.rand(n_floats, device="cuda")
+
+    def timed(f):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    copy = 8.0 * n_floats / timed(lambda: a.copy_(b)) / 1e9
+    triad = 12.0 * n_floats / timed(lambda: torch.add(b, c, alpha=2.0, out=a)) / 1e9
+    del a, b, c
+    torch.cuda.empty_cache()
+    return {"copy_GBps": copy, "triad_GBps": triad}
+
+
 def ensemble(n, seed=2026):
     """The batched-ensemble replicas: Si(1-x)Ge(x) (001) slabs of uneven
     lateral size (4..32 conventional cells) and height, T uniform in
@@ -256,6 +282,8 @@ def main():
     npts = tm["sweep_points"]
     achieved = npts * SWEEP_BYTES_PER_POINT / (sweep_ms * 1e-3) / 1e9
     tpp = measured_traffic_per_point()
+    eng.close()
+    probe = stream_probe()
     out = {
         "metric": "KMC events/s at 512^3 SiC (plain vs batched, 1 GPU); adjoint grads/s; %HBM peak",
         "value": total / wall,
@@ -280,11 +308,11 @@ def main():
                      "traffic": None if tpp is None else tpp * npts,
                      "traffic_source": "profiles/pmc_rate_sweep_sic512.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, "
                                        "per point, times the points of this launch)",
-                     "points": npts, "bytes_per_point": SWEEP_BYTES_PER_POINT, "ms": sweep_ms},
+                     "points": npts, "bytes_per_point": SWEEP_BYTES_PER_POINT, "ms": sweep_ms,
+                     "stream_probe": probe, "frac_of_triad": achieved / probe["triad_GBps"]},
         "engine": {"device_ms": dev_ms, "events": total, "kmc_time_s": t_kmc,
                    "counts": [int(c) for c in counts]},
     }
-    eng.close()
     if a.replicas > 0:
         out["batched"] = batched_leg(a.replicas)
         print(f"# batched leg: {out['batched']['value']:.4g} events/s, {out['batched']['speedup']:.2f}x sequential",

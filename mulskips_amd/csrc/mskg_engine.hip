@@ -151,27 +151,67 @@ __device__ inline int tile_local(const TileGeom &g, int T, const P3 &p) {
 // With atoms_only, only atoms of coordination 1..3 are re-evaluated and every
 // other point keeps its rate (used after the device slab builder, whose
 // atoms need the second-neighbour factor).
-__global__ void __launch_bounds__(256) k_rate_sweep(Lattice L, double rtot, int *err, int atoms_only) {
+//
+// Each thread takes four consecutive points: four 16-byte record loads in
+// flight, one 16-byte store of the rates and one 4-byte store of the
+// classes.  Nearly every point is not an MC particle (a few per cent of the
+// stored points are), so the Get_Prob branch is rare and its registers
+// (spilled at four waves per SIMD) cost little.
+__device__ inline void sweep_point(const Lattice &L, double rtot, int64_t p, const Site &sp, int atoms_only,
+                                   int *err, float &r_out, uint8_t &c_out, bool &keep) {
+  const int kind = s_kind(sp), coor = s_coor(sp);
+  keep = atoms_only && !(kind_species(kind) && coor >= 1 && coor <= 3);
+  const bool mc = (kind == KIND_EMPTY && coor >= 1 && coor <= 3) || (kind >= KIND_COV0 && coor <= 3) ||
+                  (kind_species(kind) && coor <= 3);
+  int cls = 0, key = -1;
+  double r = 0.0;
+  if (mc && !keep) {
+    P3 pt;
+    pt.i = p;
+    pxyz(L.box, p, pt.x, pt.y, pt.z);
+    int e = ERR_NONE;
+    get_prob(L, rtot, pt, sp, coor, list_of(L, pt, sp), cls, r, e, nullptr, &key);
+    if (e) atomicExch(err, e);
+  }
+  r_out = cls ? (float)r : 0.0f;
+  c_out = (uint8_t)cls;
+  if (L.key && !keep) L.key[p] = cls ? key : -1;
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_rate_sweep(Lattice L, double rtot, int *err, int atoms_only) {
   const int64_t n = L.box.npts;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-    const Site sp = L.site[p];
-    int kind = s_kind(sp), coor = s_coor(sp);
-    if (atoms_only && !(kind_species(kind) && coor >= 1 && coor <= 3)) continue;
-    bool mc = (kind == KIND_EMPTY && coor >= 1 && coor <= 3) || (kind >= KIND_COV0 && coor <= 3) ||
-              (kind_species(kind) && coor <= 3);
-    int cls = 0, key = -1;
-    double r = 0.0;
-    if (mc) {
-      P3 pt;
-      pt.i = p;
-      pxyz(L.box, p, pt.x, pt.y, pt.z);
-      int e = ERR_NONE;
-      get_prob(L, rtot, pt, sp, coor, list_of(L, pt, sp), cls, r, e, nullptr, &key);
-      if (e) atomicExch(err, e);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x * 4;
+  for (int64_t p0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; p0 < n; p0 += stride) {
+    if (p0 + 4 <= n) {
+      Site sp[4];
+      for (int k = 0; k < 4; k++) sp[k] = L.site[p0 + k];
+      float r[4];
+      uint8_t c[4];
+      bool keep[4];
+      for (int k = 0; k < 4; k++) sweep_point(L, rtot, p0 + k, sp[k], atoms_only, err, r[k], c[k], keep[k]);
+      if (!(keep[0] || keep[1] || keep[2] || keep[3])) {
+        *(float4 *)(L.rate + p0) = make_float4(r[0], r[1], r[2], r[3]);
+        *(uchar4 *)(L.cls + p0) = make_uchar4(c[0], c[1], c[2], c[3]);
+      } else {
+        for (int k = 0; k < 4; k++)
+          if (!keep[k]) {
+            L.rate[p0 + k] = r[k];
+            L.cls[p0 + k] = c[k];
+          }
+      }
+    } else {
+      for (int64_t q = p0; q < n; q++) {
+        float r;
+        uint8_t c;
+        bool keep;
+        sweep_point(L, rtot, q, L.site[q], atoms_only, err, r, c, keep);
+        if (!keep) {
+          L.rate[q] = r;
+          L.cls[q] = c;
+        }
+      }
     }
-    L.rate[p] = cls ? (float)r : 0.0f;
-    L.cls[p] = (uint8_t)cls;
-    if (L.key) L.key[p] = cls ? key : -1;
   }
 }
 
