@@ -66,7 +66,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--replicas", type=int, default=256, help="batched-ensemble leg (0 = skip)")
     ap.add_argument("--trajectories", type=int, default=64, help="adjoint leg (0 = skip)")
-    ap.add_argument("--trajectory-events", type=int, default=200000)
+    ap.add_argument("--trajectory-events", type=int, default=1000000)
+    ap.add_argument("--checkpoint-every", type=int, default=100000)
     return ap.parse_args()
 
 
@@ -160,83 +161,45 @@ def batched_leg(n):
                         "one launch (one workgroup per replica, serial BKL) vs sequential mskg_run"}
 
 
-def adjoint_leg(n_traj, n_events):
+def adjoint_leg(n_traj, n_events, segment):
     """Adjoint gradients/s: n_traj SiC trajectories (144^3 grid, second-
-    neighbour model, frozen substrate) recorded by the serial kernel, then
-    the adjoint kernel's class statistics and the 8-parameter gradient of
-    each trajectory's log-likelihood."""
-    import numpy as np
+    neighbour model, frozen substrate) recorded by the serial kernel with a
+    checkpoint every `segment` events; the adjoint kernel replays all
+    segments of all trajectories in one launch, and each trajectory's
+    8-parameter log-likelihood gradient follows from its summed statistics."""
     import torch
 
     from mulskips_amd import adjoint
-    from mulskips_amd.engine import Engine, run_serial
+    from mulskips_amd.engine import Engine
     theta0 = adjoint.theta_sic()
     tab = adjoint.tables_from_theta(theta0, 1600.0)
     box = (144, 144, 144)
 
-    def make(seed, log):
-        e = Engine(box, tab, seed=seed, tile=24, frozen_below_z=36)
-        e.build_slab(1, 48)
-        e.refresh_rates()
-        if log:
-            e.set_event_log(n_events)
+    def make(seed):
+        e = Engine(box, tab, seed=seed if seed is not None else 1, tile=24, frozen_below_z=36)
+        if seed is not None:
+            e.build_slab(1, 48)
+            e.refresh_rates()
         return e
 
-    gens = [make(500 + i, True) for i in range(n_traj)]
     t0 = time.perf_counter()
-    run_serial(gens, [n_events] * n_traj)
-    t_gen = time.perf_counter() - t0
-    logs = [g.event_log() for g in gens]
-    for g in gens:
-        g.close()
-    dts = [adjoint.waiting_times(lg) for lg in logs]
-    engs = [make(0, False) for _ in range(n_traj)]
+    logs, dts, segments = adjoint.record_segmented(make, [500 + i for i in range(n_traj)], n_events, segment)
+    t_rec = time.perf_counter() - t0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    S, A, sc = adjoint.trajectory_stats(engs, logs, dts)
-    grads = [adjoint.gradient(theta0, adjoint.TrajectoryStats(S[i], A[i], sc[i]), 1600.0)[1] for i in range(n_traj)]
+    stats = adjoint.segmented_stats(segments)
+    grads = [adjoint.gradient(theta0, adjoint.TrajectoryStats(S, A, sc), 1600.0)[1] for S, A, sc in stats]
     t_adj = time.perf_counter() - t0
-    for e in engs:
-        e.close()
+    for traj in segments:
+        for c, _, _ in traj:
+            c.close()
     g = torch.stack(grads).numpy()
     return {"value": n_traj * len(theta0) / t_adj, "unit": "gradients/s (trajectories x parameters / s)",
-            "trajectories": n_traj, "events_per_trajectory": n_events, "parameters": list(adjoint.PARAMS),
-            "adjoint_s": t_adj, "record_s": t_gen, "record_events_per_s": n_traj * n_events / t_gen,
-            "mean_gradient": [float(v) for v in g.mean(axis=0)],
-            "workload": "3C-SiC second-neighbour model, 144^3 grid, frozen 40-ML-equivalent substrate, 1600 K"}
-
-
-def cpu_baseline(tables, substrate_len, seconds):
-    """The oracle (C restatement of the reference engine, serial, one core) on
-    the same model and substrate at a 480 x 480 x 240 grid, for about
-    `seconds` of wall time after its initialisation."""
-    import tempfile
-
-    from mulskips_amd import deck as deckmod
-    from oracle.oracle import Oracle
-    box = (480, 480, 240)
-    with tempfile.TemporaryDirectory() as tmp:
-        d = deckmod.deck_from_tables(tables, "F", [substrate_len, 120, 120])
-        path = os.path.join(tmp, "start.dat")
-        deckmod.write_start_dat(d, path)
-        o = Oracle(*box)
-        o.load_deck(path)
-        o.set_frozen(substrate_len + 1)
-        if tables.get("second_factor") is not None:
-            o.set_second_factor(tables["second_factor"])
-        o.step(20000, record=False)  # leave the initial transient
-        n = 0
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < seconds:
-            k = len(o.step(100000, record=False))
-            n += k
-            if k < 100000:
-                break
-        dt = time.perf_counter() - t0
-        o.close()
-    return {"value": n / dt, "unit": "events/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (reference engine restated in C, serial BKL, no 98 KB table copy), same SiC model "
-                      f"and frozen 40-ML substrate on a 480x480x240 grid, {n} events in {dt:.1f} s"}
+            "trajectories": n_traj, "events_per_trajectory": n_events, "checkpoint_every": segment,
+            "parameters": list(adjoint.PARAMS), "adjoint_s": t_adj, "replayed_events_per_s": n_traj * n_events / t_adj,
+            "record_s": t_rec, "mean_gradient": [float(v) for v in g.mean(axis=0)],
+            "workload": "3C-SiC second-neighbour model, 144^3 grid, frozen substrate, 1600 K; trajectories "
+                        "recorded by the serial kernel, segments replayed in one launch"}
 
 
 def main():
@@ -318,7 +281,7 @@ def main():
         print(f"# batched leg: {out['batched']['value']:.4g} events/s, {out['batched']['speedup']:.2f}x sequential",
               file=sys.stderr, flush=True)
     if a.trajectories > 0:
-        out["adjoint"] = adjoint_leg(a.trajectories, a.trajectory_events)
+        out["adjoint"] = adjoint_leg(a.trajectories, a.trajectory_events, a.checkpoint_every)
         print(f"# adjoint leg: {out['adjoint']['value']:.4g} gradients/s", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tables, substrate, a.cpu_seconds)

@@ -198,12 +198,15 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
 /* Trajectory likelihood statistics (no reference counterpart; for parameter
  * calibration).  For each engine i, loaded with the initial state of a
  * recorded trajectory logs[i][0..n_events[i]) with waiting times dts[i][k]
- * (the time spent in the state before event k), the engine first refreshes
- * every rate (as mskg_refresh_rates), then replays the log and returns
+ * (the time spent in the state before event k), the engine replays the log
+ * and returns
  *   S[i][c]  the number of events drawn from rate class c,
  *   A[i][c]  sum over events k of (points in class c before event k) * dt_k,
  * so that log L = sum_c S_c log r_c - A_c r_c + scalars[i][2] - scalars[i][1]
  * for any rate tables r, and d log L / d theta = sum_c (S_c/r_c - A_c) dr_c/dtheta.
+ * An engine that tracks class keys (mskg_track_classes) starts from its
+ * current rates and keys; any other engine first refreshes every rate (as
+ * mskg_refresh_rates).
  * Classes (MSKG_KEY_*): evaporation of species s with n1 (Si or wall), n2,
  * n3 reciprocal neighbours of species 1..3 and m occupied second-neighbour
  * positions: key (s-1) + 3*(n1 + 4*(n2 + 4*n3)) + 192*m; an adsorption site
@@ -216,6 +219,19 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
  * replayed).  Models without coverage species and without LA rates only.
  * The engines end in the trajectory's final state. */
 enum { MSKG_KEY_ADS = 2496, MSKG_KEY_PH = 2499, MSKG_KEY_ISO = 2500, MSKG_KEY_N = 2501, MSKG_ADJ_NS = 2510 };
+
+/* Keep every point's rate-class key current through runs (on != 0; a keyed
+ * rate sweep sets them first when a state is loaded).  A tracked engine's
+ * state can be copied mid-trajectory (mskg_copy_state) and handed to
+ * mskg_adjoint as the start of a trajectory segment: the adjoint then uses
+ * the copied rates and keys instead of refreshing them, so the segments of
+ * one trajectory replay independently and their statistics add up to the
+ * whole trajectory's.  No reference counterpart. */
+int mskg_track_classes(mskg_engine *e, int32_t on);
+
+/* Copy src's lattice (records, rates, classes, class keys when both track
+ * them) and clock into dst, an engine of the same box and tile. */
+int mskg_copy_state(mskg_engine *dst, const mskg_engine *src);
 int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const *logs, const double *const *dts,
                  const int64_t *n_events, double *S, double *A, double *scalars);
 

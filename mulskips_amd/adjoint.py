@@ -153,3 +153,53 @@ def waiting_times(log, t0=0.0):
     """dt_k from a log's clock (time after each event)."""
     t = np.asarray(log["time"], np.float64)
     return np.diff(np.concatenate([[t0], t]))
+
+
+def record_segmented(make_engine, seeds, n_events, segment):
+    """Record one serial trajectory per seed (all in one batched launch per
+    segment) and checkpoint each trajectory every `segment` events.
+
+    make_engine(seed) returns a loaded engine (its state is the trajectory's
+    start); it is also called with seed=None for the checkpoint engines,
+    which receive their state by copy.  Returns (logs, dts, segments) where
+    segments[i] is the list of (engine at the segment start, log slice,
+    waiting-time slice) of trajectory i; the segments replay independently
+    (mskg_adjoint) and their statistics add up to the trajectory's."""
+    from .engine import run_serial
+    gens = []
+    for s in seeds:
+        g = make_engine(s)
+        g.track_classes()
+        g.set_event_log(n_events)
+        gens.append(g)
+    t0 = [g.counters(lists=False)[1] for g in gens]
+    starts = [[] for _ in gens]
+    done = 0
+    while done < n_events:
+        k = min(segment, n_events - done)
+        for g, st in zip(gens, starts):
+            c = make_engine(None)
+            c.track_classes()
+            c.copy_state_from(g)
+            st.append((c, done, done + k))
+        run_serial(gens, [k] * len(gens))
+        done += k
+    logs = [g.event_log() for g in gens]
+    for g in gens:
+        g.close()
+    dts = [waiting_times(lg, t) for lg, t in zip(logs, t0)]
+    segments = [[(c, logs[i][a:b], dts[i][a:b]) for c, a, b in starts[i]] for i in range(len(gens))]
+    return logs, dts, segments
+
+
+def segmented_stats(segments):
+    """Adjoint statistics of segmented trajectories in one launch; returns
+    per-trajectory (S, A, scalars) summed over each trajectory's segments."""
+    flat = [seg for traj in segments for seg in traj]
+    S, A, sc = trajectory_stats([c for c, _, _ in flat], [lg for _, lg, _ in flat], [d for _, _, d in flat])
+    out, k = [], 0
+    for traj in segments:
+        m = len(traj)
+        out.append((S[k:k + m].sum(axis=0), A[k:k + m].sum(axis=0), sc[k:k + m].sum(axis=0)))
+        k += m
+    return out

@@ -1086,6 +1086,8 @@ struct mskg_engine {
   // likelihood accounting: class key of every point, and as last accounted
   int32_t *d_key = nullptr, *d_key_seen = nullptr;
   float *d_rate_seen = nullptr;
+  bool track = false;       // keep every point's class key current through runs
+  bool keys_valid = false;  // d_key matches the rates (a keyed sweep or a tracked run since)
   double run_ms = 0, sweep_ms = 0;
   int64_t sweep_points = 0, sweeps_done = 0;
   bool loaded = false;
@@ -1135,7 +1137,7 @@ static Lattice make_lattice(mskg_engine *e) {
   L.tile = e->tile;
   L.ntx = e->ntx;
   L.nty = e->nty;
-  L.key = nullptr;
+  L.key = e->track ? e->d_key : nullptr;
   return L;
 }
 
@@ -1380,6 +1382,7 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt, int32_t n_ad, const int
   HIPCHK(e, hipStreamSynchronize(e->stream));
   e->loaded = true;
   e->lists_dirty = true;
+  e->keys_valid = false;
   return MSKG_OK;
 }
 
@@ -1416,6 +1419,7 @@ int mskg_refresh_rates(mskg_engine *e) {
   HIPCHK(e, hipEventRecord(e->ev0, e->stream));
   hipLaunchKernelGGL(k_rate_sweep, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
   HIPCHK(e, hipGetLastError());
+  if (e->track) e->keys_valid = true;
   HIPCHK(e, hipEventRecord(e->ev1, e->stream));
   int err = 0;
   HIPCHK(e, hipMemcpyAsync(&err, e->d_err, sizeof(int), hipMemcpyDeviceToHost, e->stream));
@@ -1720,6 +1724,59 @@ static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction
   HIPCHK(e, hipStreamSynchronize(e->stream));
   e->loaded = true;
   e->lists_dirty = true;
+  e->keys_valid = false;
+  return MSKG_OK;
+}
+
+int mskg_track_classes(mskg_engine *e, int32_t on) {
+  if (!e || !e->d_site) return MSKG_EINVAL;
+  if (!on) {
+    e->track = false;
+    e->keys_valid = false;
+    return MSKG_OK;
+  }
+  if (e->tab.ncov || e->tab.la_mode)
+    return fail_(e, MSKG_EINVAL, "class keys cover models without coverage species and without LA rates");
+  const int64_t np = e->box.npts;
+  if (!e->d_key) {
+    HIPCHK(e, hipMalloc(&e->d_key, np * sizeof(int32_t)));
+    HIPCHK(e, hipMalloc(&e->d_key_seen, np * sizeof(int32_t)));
+    HIPCHK(e, hipMalloc(&e->d_rate_seen, np * sizeof(float)));
+  }
+  e->track = true;
+  e->keys_valid = false;
+  if (!e->loaded) return MSKG_OK;
+  return mskg_refresh_rates(e);  // a keyed sweep: rates and keys from the state
+}
+
+int mskg_copy_state(mskg_engine *dst, const mskg_engine *src) {
+  if (!dst || !src || !src->loaded) return MSKG_EINVAL;
+  if (dst == src) return MSKG_OK;
+  if (dst->box.lx != src->box.lx || dst->box.ly != src->box.ly || dst->box.lz != src->box.lz ||
+      dst->tile != src->tile)
+    return fail_(dst, MSKG_EINVAL, "copy_state: engines differ in box or tile");
+  const int64_t np = dst->box.npts;
+  HIPCHK(dst, hipStreamSynchronize(src->stream));
+  HIPCHK(dst, hipMemcpyAsync(dst->d_site, src->d_site, np * sizeof(Site), hipMemcpyDeviceToDevice, dst->stream));
+  HIPCHK(dst, hipMemcpyAsync(dst->d_rate, src->d_rate, np * sizeof(float), hipMemcpyDeviceToDevice, dst->stream));
+  HIPCHK(dst, hipMemcpyAsync(dst->d_cls, src->d_cls, np, hipMemcpyDeviceToDevice, dst->stream));
+  dst->keys_valid = false;
+  if (dst->track && src->track && src->keys_valid) {
+    HIPCHK(dst, hipMemcpyAsync(dst->d_key, src->d_key, np * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
+    dst->keys_valid = true;
+  }
+  RunState st;
+  HIPCHK(dst, hipMemcpyAsync(&st, src->d_st, sizeof st, hipMemcpyDeviceToHost, dst->stream));
+  HIPCHK(dst, hipStreamSynchronize(dst->stream));
+  RunState s0;
+  memset(&s0, 0, sizeof s0);
+  s0.clock = st.clock;
+  s0.rtot = st.rtot;
+  s0.events_per_pass = dst->cfg.events_per_pass > 0 ? dst->cfg.events_per_pass : 1.0;
+  HIPCHK(dst, hipMemcpyAsync(dst->d_st, &s0, sizeof s0, hipMemcpyHostToDevice, dst->stream));
+  HIPCHK(dst, hipStreamSynchronize(dst->stream));
+  dst->loaded = true;
+  dst->lists_dirty = true;
   return MSKG_OK;
 }
 
@@ -1911,7 +1968,9 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
     int rc = read_state(e, st);
     if (rc) { cleanup(); return rc; }
     if (st.err) { cleanup(); return fail_(e, MSKG_ESTATE, "engine is in an error state: %s", err_name(st.err)); }
-    // the trajectory starts from fresh rates: a sweep that also assigns every point its class key
+    // the trajectory starts from the engine's rates: a tracked engine's keys
+    // are current; otherwise a sweep refreshes the rates and assigns the keys
+    const bool fresh = !(e->track && e->keys_valid);
     Lattice L = make_lattice(e);
     L.inlist = nullptr;
     L.key = e->d_key;
@@ -1929,8 +1988,10 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
     }
     int blocks = (int)std::min<int64_t>((np + 255) / 256, 256 * 64);
     HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
-    hipLaunchKernelGGL(k_rate_sweep, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
-    HIPCHK(e, hipGetLastError());
+    if (fresh) {
+      hipLaunchKernelGGL(k_rate_sweep, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
+      HIPCHK(e, hipGetLastError());
+    }
     HIPCHK(e, hipMemsetAsync(d_cnt, 0, (KEY_N + 1) * sizeof(int), e->stream));
     HIPCHK(e, hipMemsetAsync(d_iso, 0, sizeof(double), e->stream));
     hipLaunchKernelGGL(k_key_hist, dim3(blocks), dim3(256), 0, e->stream, L, d_cnt, d_iso);

@@ -122,6 +122,14 @@ class Engine:
         """The SetSi slab as a random alloy: species 2 with probability fraction."""
         self._check(self.L.mskg_build_alloy_slab(self.h, int(length), float(fraction)))
 
+    def track_classes(self, on=True):
+        """Keep the rate-class keys current through runs (likelihood segments)."""
+        self._check(self.L.mskg_track_classes(self.h, 1 if on else 0))
+
+    def copy_state_from(self, src):
+        """This engine takes src's lattice, rates, class keys and clock."""
+        self._check(self.L.mskg_copy_state(self.h, src.h))
+
     def set_event_log(self, capacity):
         self._check(self.L.mskg_set_event_log(self.h, int(capacity)))
 
