@@ -235,6 +235,26 @@ int mskg_copy_state(mskg_engine *dst, const mskg_engine *src);
 int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const *logs, const double *const *dts,
                  const int64_t *n_events, double *S, double *A, double *scalars);
 
+/* Frame compaction for the xyz output (inpout/WriteMolMolXYZFile.f:77-260):
+ * one pass over the lattice writes, as (x, y, z, code) int32 quadruples in
+ * the reference's grid order (z, then y, then x; points with z <= LenZ-1):
+ *   main_rec  the points the main frame lists -- atoms of coordination <= 3
+ *             (code = species 1..3) and coverage points (code = -k for
+ *             ListCov(k));
+ *   void_rec  vacancies (empty, coordination 4; code 0);
+ *   noepi_rec bulk atoms off the cubic diamond sites, the _d frame
+ *             (code = species);
+ * and counts = (main, voids, non-epitaxial records, atoms of coordination
+ * <= 3 with 1 <= z <= LenZ-1, covered points with 1 <= z <= LenZ-1), the
+ * last two for the frame header.  Call with zero capacities to size the
+ * buffers.  The reference lists the vacancies in ListVoid order, which
+ * depends on the run's history; here they come in grid order.  The
+ * reference's writer also flips and mis-restores the species of SiC
+ * "wrong" sites (:262-398); that side effect on the lattice is not
+ * reproduced. */
+int mskg_frame(mskg_engine *e, int32_t *main_rec, int64_t cap_main, int32_t *void_rec, int64_t cap_void,
+               int32_t *noepi_rec, int64_t cap_noepi, int64_t counts[5]);
+
 /* Host evaluation of the engine's FP64 geometry -- the same source the
  * kernels run -- for checking it against the reference formulas without a
  * GPU.  FIND_NN (lattice/FIND_NN.f:25-77): out = the two new neighbours of

@@ -1043,6 +1043,110 @@ __global__ void __launch_bounds__(256) k_key_hist(Lattice L, int *cnt, double *i
   }
 }
 
+
+// ---- frame compaction (the xyz output of inpout/WriteMolMolXYZFile.f) --------
+// Three ordered lists in the reference's grid order (z, then y, then x; the
+// stored-point order restricted to one z plane is the same), z <= LenZ-1:
+//   0 the points the main frame lists: coverage species and atoms of
+//     coordination <= 3 (WriteMolMolXYZFile.f:131-166);
+//   1 vacancies (empty, coordination 4);
+//   2 non-epitaxial bulk atoms: coordination 4 off the cubic diamond sites
+//     (the _d frame, :77-121).
+// and two header counts over z = 1..LenZ-1 (:93-110): 3 atoms of
+// coordination <= 3, 4 covered points.  Each block owns a contiguous chunk
+// of points; k_frame_count counts, the host scans, k_frame_write writes in
+// order with a block-wide prefix scan.
+constexpr int FRAME_THREADS = 256;
+__device__ inline int frame_cat(const Lattice &L, int64_t p, const Site &sp, int *code, bool hdr[2]) {
+  int x, y, z;
+  pxyz(L.box, p, x, y, z);
+  hdr[0] = hdr[1] = false;
+  const int kind = s_kind(sp), coor = s_coor(sp);
+  if (kind == KIND_WALL || z > L.box.lz - 1) return -1;
+  const bool inner = z >= 1;
+  if (kind_species(kind)) {
+    if (coor <= 3) {
+      *code = kind;
+      hdr[0] = inner;
+      return 0;
+    }
+    if (!dia_type(x, y, z)) {
+      *code = kind;
+      return 2;
+    }
+    return -1;
+  }
+  if (kind_cov_slot(kind)) {
+    *code = -kind_cov_slot(kind);  // coverage slot, negated; the host maps it to ListCov
+    hdr[1] = inner;
+    return coor == 4 ? 1 : 0;
+  }
+  return coor == 4 ? 1 : -1;  // an empty point with coordination 4 is a vacancy
+}
+
+__global__ void __launch_bounds__(FRAME_THREADS) k_frame_count(Lattice L, int64_t chunk, unsigned long long *counts) {
+  __shared__ unsigned long long sc[5][FRAME_THREADS / WAVE];
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(lo + chunk, L.box.npts);
+  unsigned long long c[5] = {0, 0, 0, 0, 0};
+  for (int64_t p = lo + threadIdx.x; p < hi; p += FRAME_THREADS) {
+    int code;
+    bool hdr[2];
+    const int cat = frame_cat(L, p, L.site[p], &code, hdr);
+    if (cat >= 0) c[cat]++;
+    if (cat == 1 && code < 0) c[0]++;  // a covered vacancy is also listed in the main frame
+    c[3] += hdr[0];
+    c[4] += hdr[1];
+  }
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  for (int k = 0; k < 5; k++) {
+    unsigned long long v = c[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    if (lane == 0) sc[k][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    unsigned long long v = 0;
+    for (int q = 0; q < FRAME_THREADS / WAVE; q++) v += sc[threadIdx.x][q];
+    counts[(int64_t)blockIdx.x * 5 + threadIdx.x] = v;
+  }
+}
+
+__global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_t chunk, const unsigned long long *offs,
+                                                               int4 *out0, int4 *out1, int4 *out2) {
+  __shared__ unsigned int wsum[3][FRAME_THREADS / WAVE];
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(lo + chunk, L.box.npts);
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  unsigned long long base[3] = {offs[(int64_t)blockIdx.x * 3], offs[(int64_t)blockIdx.x * 3 + 1],
+                                offs[(int64_t)blockIdx.x * 3 + 2]};
+  int4 *outs[3] = {out0, out1, out2};
+  for (int64_t p0 = lo; p0 < hi; p0 += FRAME_THREADS) {
+    const int64_t p = p0 + threadIdx.x;
+    int code = 0, cat = -1;
+    bool hdr[2];
+    if (p < hi) cat = frame_cat(L, p, L.site[p], &code, hdr);
+    int x = 0, y = 0, z = 0;
+    if (cat >= 0) pxyz(L.box, p, x, y, z);
+    bool in[3] = {cat == 0 || (cat == 1 && code < 0), cat == 1, cat == 2};
+    unsigned int rank[3];
+    for (int k = 0; k < 3; k++) {
+      const unsigned long long m = __ballot(in[k]);
+      rank[k] = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) wsum[k][w] = __popcll(m);
+    }
+    __syncthreads();
+    for (int k = 0; k < 3; k++) {
+      unsigned int before = 0, total = 0;
+      for (int q = 0; q < FRAME_THREADS / WAVE; q++) {
+        if (q < w) before += wsum[k][q];
+        total += wsum[k][q];
+      }
+      if (in[k]) outs[k][base[k] + before + rank[k]] = make_int4(x, y, z, k == 1 ? 0 : code);
+      base[k] += total;
+    }
+    __syncthreads();
+  }
+}
+
 // ---- host side ---------------------------------------------------------------
 #define HIPCHK(e, x)                                                                  \
   do {                                                                                \
@@ -2055,6 +2159,72 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
   }
   cleanup();
   return rc_all;
+}
+
+int mskg_frame(mskg_engine *e, int32_t *main_rec, int64_t cap_main, int32_t *void_rec, int64_t cap_void,
+               int32_t *noepi_rec, int64_t cap_noepi, int64_t counts[5]) {
+  if (!e || !e->loaded || !counts) return fail_(e, MSKG_EINVAL, "frame: no state loaded");
+  const int64_t n = e->box.npts;
+  const int nblk = (int)std::min<int64_t>(std::max<int64_t>(1, (n + 65535) / 65536), 65536);
+  const int64_t chunk = (n + nblk - 1) / nblk;
+  Lattice L = make_lattice(e);
+  unsigned long long *d_counts = nullptr, *d_offs = nullptr;
+  int4 *d_out[3] = {nullptr, nullptr, nullptr};
+  auto cleanup = [&]() {
+    hipFree(d_counts);
+    hipFree(d_offs);
+    for (int k = 0; k < 3; k++) hipFree(d_out[k]);
+  };
+  if (hipMalloc(&d_counts, (size_t)nblk * 5 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&d_offs, (size_t)nblk * 3 * sizeof(unsigned long long)) != hipSuccess) {
+    cleanup();
+    return fail_(e, MSKG_ENOMEM, "frame: no device memory for the block counts");
+  }
+  hipLaunchKernelGGL(k_frame_count, dim3(nblk), dim3(FRAME_THREADS), 0, e->stream, L, chunk, d_counts);
+  std::vector<unsigned long long> cnt((size_t)nblk * 5), offs((size_t)nblk * 3);
+  hipError_t ce = hipGetLastError();
+  if (ce == hipSuccess)
+    ce = hipMemcpyAsync(cnt.data(), d_counts, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        e->stream);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
+  if (ce != hipSuccess) {
+    cleanup();
+    return fail_(e, MSKG_EDEVICE, "frame count: %s", hipGetErrorString(ce));
+  }
+  unsigned long long tot[5] = {0, 0, 0, 0, 0};
+  for (int b = 0; b < nblk; b++)
+    for (int k = 0; k < 5; k++) {
+      if (k < 3) offs[(size_t)b * 3 + k] = tot[k];
+      tot[k] += cnt[(size_t)b * 5 + k];
+    }
+  for (int k = 0; k < 5; k++) counts[k] = (int64_t)tot[k];
+  if ((int64_t)tot[0] > cap_main || (int64_t)tot[1] > cap_void || (int64_t)tot[2] > cap_noepi) {
+    cleanup();
+    if ((main_rec && cap_main) || (void_rec && cap_void) || (noepi_rec && cap_noepi))
+      return fail_(e, MSKG_ECAPACITY, "frame: %lld / %lld / %lld records do not fit", (long long)tot[0],
+                   (long long)tot[1], (long long)tot[2]);
+    return MSKG_OK;  // sizing call
+  }
+  for (int k = 0; k < 3; k++)
+    if (hipMalloc(&d_out[k], std::max<unsigned long long>(1, tot[k]) * sizeof(int4)) != hipSuccess) {
+      cleanup();
+      return fail_(e, MSKG_ENOMEM, "frame: no device memory for the records");
+    }
+  ce = hipMemcpyAsync(d_offs, offs.data(), offs.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                      e->stream);
+  if (ce == hipSuccess) {
+    hipLaunchKernelGGL(k_frame_write, dim3(nblk), dim3(FRAME_THREADS), 0, e->stream, L, chunk, d_offs, d_out[0],
+                       d_out[1], d_out[2]);
+    ce = hipGetLastError();
+  }
+  int32_t *hosts[3] = {main_rec, void_rec, noepi_rec};
+  for (int k = 0; k < 3 && ce == hipSuccess; k++)
+    if (tot[k] && hosts[k])
+      ce = hipMemcpyAsync(hosts[k], d_out[k], tot[k] * sizeof(int4), hipMemcpyDeviceToHost, e->stream);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
+  cleanup();
+  if (ce != hipSuccess) return fail_(e, MSKG_EDEVICE, "frame write: %s", hipGetErrorString(ce));
+  return MSKG_OK;
 }
 
 }  // extern "C"

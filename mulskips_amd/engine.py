@@ -122,6 +122,18 @@ class Engine:
         """The SetSi slab as a random alloy: species 2 with probability fraction."""
         self._check(self.L.mskg_build_alloy_slab(self.h, int(length), float(fraction)))
 
+    def frame(self):
+        """Device frame compaction: (main, voids, non_epitaxial) record arrays
+        of (x, y, z, code) rows in grid order and the header counts
+        (atoms of coordination <= 3, covered points; both for 1 <= z < LenZ)."""
+        c = np.zeros(5, np.int64)
+        z = None
+        self._check(self.L.mskg_frame(self.h, z, 0, z, 0, z, 0, ptr(c)))
+        bufs = [np.zeros((max(int(c[k]), 1), 4), np.int32) for k in range(3)]
+        self._check(self.L.mskg_frame(self.h, ptr(bufs[0]), int(c[0]), ptr(bufs[1]), int(c[1]), ptr(bufs[2]),
+                                      int(c[2]), ptr(c)))
+        return bufs[0][:c[0]], bufs[1][:c[1]], bufs[2][:c[2]], int(c[3]), int(c[4])
+
     def track_classes(self, on=True):
         """Keep the rate-class keys current through runs (likelihood segments)."""
         self._check(self.L.mskg_track_classes(self.h, 1 if on else 0))

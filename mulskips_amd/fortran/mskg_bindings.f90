@@ -179,6 +179,26 @@ module mskg_bindings
       real(c_double), intent(out) :: S(*), A(*), scalars(*)
     end function
 
+    integer(c_int) function mskg_track_classes(eng, on) bind(c, name="mskg_track_classes")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: eng
+      integer(c_int32_t), value :: on
+    end function
+
+    integer(c_int) function mskg_copy_state(dst, src) bind(c, name="mskg_copy_state")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dst, src
+    end function
+
+    ! frame compaction: (x,y,z,code) int32 records in grid order; counts(5)
+    integer(c_int) function mskg_frame(eng, main_rec, cap_main, void_rec, cap_void, noepi_rec, cap_noepi, counts) &
+        bind(c, name="mskg_frame")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: eng, main_rec, void_rec, noepi_rec
+      integer(c_int64_t), value :: cap_main, cap_void, cap_noepi
+      integer(c_int64_t), intent(out) :: counts(5)
+    end function
+
     integer(c_int) function mskg_last_timing(eng, run_ms, sweep_ms, sweep_points, sweeps_done) &
         bind(c, name="mskg_last_timing")
       import :: c_int, c_int64_t, c_ptr, c_double
