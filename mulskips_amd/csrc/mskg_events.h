@@ -322,18 +322,6 @@ struct EvResult {
   int err;
 };
 
-// Get_Prob followed by UpdateTree / AddMC / EraseMC+Add{Bulk,Void}: a
-// zero rate takes the site out of the MC population, its list stays.
-MSKG_HD void refresh(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &err,
-                     const Ovr *ovr = nullptr, float *out_rate = nullptr, uint8_t *out_cls = nullptr) {
-  int cls = 0, key = -1;
-  double r = 0.0;
-  get_prob(L, rtot, p, sp, coor, nn, cls, r, err, ovr, &key);
-  set_rate(L, p, cls, r, key);
-  if (out_rate) *out_rate = cls ? (float)r : 0.0f;
-  if (out_cls) *out_cls = (uint8_t)cls;
-}
-
 MSKG_HD bool put_list(const Lattice &L, Site &s, const P3 &p, const NL &l, EvResult &res) {
   if (!set_list(L.box, s, p.x, p.y, p.z, l)) { res.err = ERR_GEOM; return false; }
   return true;
@@ -358,6 +346,7 @@ struct EvScratch {
   int found[4];     //   ... and found its reciprocal partner (NBuff) at found_pos[i]
   int found_pos[4][3];
   int seq;          // visit the neighbours in order on lane 0
+  int nvisit;       // neighbours the event visits (desorption at coordination 1: one)
   int abort;
   P3 sec[12];       // second-neighbour refresh: the point refreshed by lane k (i = -1: none)
   int err2[12];
@@ -423,28 +412,35 @@ __device__ inline void second_setup(const Lattice &L, const P3 &s, int k, EvScra
   f.nn = list_of(L, q, sq);
 }
 
+// A rate evaluation the neighbour phase asks for: the point, its record
+// (already written back unless list_after) and the coordination and list
+// the evaluation uses.  list_after: the record is written after the
+// evaluation, with the list stored only when the point gets a rate (AddMC).
+struct RefreshReq {
+  bool need, list_after;
+  P3 p;
+  Site sp;
+  int coor;
+  NL nn;
+};
+
 // deposition.f:78-235, the update of neighbour i of the new atom at s.
-__device__ void dep_neighbour(const Lattice &L, double rtot, const P3 &s, int i, EvScratch &sc, int &err) {
+__device__ void dep_prepare(const Lattice &L, const P3 &s, int i, EvScratch &sc, int &err, RefreshReq &q) {
   const Geometry &g = *L.geo;
   const int lx = L.box.lx, ly = L.box.ly;
   const P3 p = sc.pts[i];
   Site sp = rd(L, p.i);
   int kind = s_kind(sp), coornn = s_coor(sp);
+  NL nnn = list_of(L, p, sp);
+  bool need = false, list_after = false;
   if (!kind_occupied(kind)) {
     if (coornn == 0) {
       coornn = 1;
       s_set_coor(sp, coornn);
-      NL nnn;
       for (int j = 0; j < 4; j++) nnn.x[j] = nnn.y[j] = nnn.z[j] = 0;
       nnn.x[0] = s.x; nnn.y[0] = s.y; nnn.z[0] = s.z;
-      int cls = 0, key = -1;
-      double r = 0.0;
-      get_prob(L, rtot, p, sp, coornn, nnn, cls, r, err, nullptr, &key);
-      if (cls && !set_list(L.box, sp, p.x, p.y, p.z, nnn)) { err = ERR_GEOM; return; }  // AddMC stores the list
-      wr(L, p.i, sp);
-      set_rate(L, p, cls, r, key);
+      need = list_after = true;
     } else if (coornn == 1) {
-      NL nnn = list_of(L, p, sp);
       int dx = nnn.x[0] - s.x, dy = nnn.y[0] - s.y, dz = nnn.z[0] - s.z;
       dx = dx < 0 ? -dx : dx;
       dx = dx < lx - dx ? dx : lx - dx;
@@ -455,147 +451,114 @@ __device__ void dep_neighbour(const Lattice &L, double rtot, const P3 &s, int i,
         nnn.x[1] = s.x; nnn.y[1] = s.y; nnn.z[1] = s.z;
         int a1[3] = {nnn.x[0], nnn.y[0], nnn.z[0]}, a2[3] = {s.x, s.y, s.z}, pc[3] = {p.x, p.y, p.z}, ns[2][3];
         find_nn(g, lx, ly, a1, a2, pc, ns);
-        P3 q[2];
+        P3 qq[2];
         Site sq[2];
         for (int j = 2; j < 4; j++) {
           nnn.x[j] = ns[j - 2][0]; nnn.y[j] = ns[j - 2][1]; nnn.z[j] = ns[j - 2][2];
-          q[j - 2] = entry(L, nnn, j);
+          qq[j - 2] = entry(L, nnn, j);
         }
-        sq[0] = rd(L, q[0].i);
-        sq[1] = rd(L, q[1].i);
+        sq[0] = rd(L, qq[0].i);
+        sq[1] = rd(L, qq[1].i);
         for (int j = 0; j < 2; j++) {
           int kq = s_kind(sq[j]);
           if (kind_occupied(kq) && s_coor(sq[j]) <= 3 && kq != KIND_WALL) {
-            if (recip(list_of(L, q[j], sq[j]), p)) coornn++;
+            if (recip(list_of(L, qq[j], sq[j]), p)) coornn++;
           }
         }
         s_set_coor(sp, coornn);
         if (!set_list(L.box, sp, p.x, p.y, p.z, nnn)) { err = ERR_GEOM; return; }
         wr(L, p.i, sp);
-        refresh(L, rtot, p, sp, coornn, nnn, err);
+        need = true;
       }
     } else if (coornn == 2 || coornn == 3) {
-      NL nnn = list_of(L, p, sp);
       if (recip(nnn, s)) {
         coornn++;
         s_set_coor(sp, coornn);
         wr(L, p.i, sp);
-        refresh(L, rtot, p, sp, coornn, nnn, err);
+        need = true;
       }
     }
   } else if (kind != KIND_WALL) {
-    if (coornn <= 3) {
-      NL nnn = list_of(L, p, sp);
-      if (recip(nnn, s)) {
-        coornn++;
-        s_set_coor(sp, coornn);
-        wr(L, p.i, sp);
-        refresh(L, rtot, p, sp, coornn, nnn, err);
-        if (sc.coor_old == 1 && i >= 1) sc.inc[i] = 1;
-      }
+    if (coornn <= 3 && recip(nnn, s)) {
+      coornn++;
+      s_set_coor(sp, coornn);
+      wr(L, p.i, sp);
+      need = true;
+      if (sc.coor_old == 1 && i >= 1) sc.inc[i] = 1;
     }
   } else {
     sc.inc[i] = 1;  // wall neighbour
   }
+  q.need = need;
+  q.list_after = list_after;
+  q.p = p;
+  q.sp = sp;
+  q.coor = coornn;
+  q.nn = nnn;
 }
 
-// Deposition, mcevent/deposition.f:24-251.
-__device__ void deposition(const Lattice &L, double rtot, const P3 &s, int species, const Choice &ch, EvResult &res,
-                           EvScratch &sc, int lane) {
-  const Geometry &g = *L.geo;
-  if (lane == 0) {
-    sc.abort = 0;
-    Site ss = rd(L, s.i);
-    int coor = s_coor(ss);
-    sc.coor_old = coor;
-    s_set_kind(ss, species);
-    wr(L, s.i, ss);
-    res.arm = -1;
-    NL nn_site = list_of(L, s, ss);
-    if (coor == 1) {  // :49-74, FIND_SN + IF_Arm_Choice
-      P3 c = entry(L, nn_site, 0);
-      Site scr = rd(L, c.i);
-      NL lc = list_of(L, c, scr);
-      int sn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, j = 0;
-      for (int i = 0; i < 4; i++)
-        if (!same(lc, i, s)) {
-          if (j < 3) { sn[j][0] = lc.x[i]; sn[j][1] = lc.y[i]; sn[j][2] = lc.z[i]; }
-          j++;
-        }
-      int site[3] = {s.x, s.y, s.z}, cc[3] = {c.x, c.y, c.z}, zig[3][3], arm[3][3];
-      find_sn(g, L.box.lx, L.box.ly, site, cc, sn, zig, arm);
-      Site sa[3], sz[3];
-      for (int k = 0; k < 3; k++) {
-        sa[k] = rd(L, at(L, arm[k][0], arm[k][1], arm[k][2]).i);
-        sz[k] = rd(L, at(L, zig[k][0], zig[k][1], zig[k][2]).i);
-      }
-      int carm = 0, czig = 0;  // IF_Arm_Choice.f:35-54
-      for (int k = 0; k < 3; k++) {
-        carm += s_coor(sa[k]);
-        czig += s_coor(sz[k]);
-      }
-      int is_arm;
-      if (ch.arm >= 0) is_arm = ch.arm;
-      else if (czig > carm) is_arm = 0;
-      else if (czig == carm) is_arm = (ch.u_arm <= L.tab->ptrans_zig) ? 0 : 1;
-      else is_arm = 1;
-      res.arm = is_arm;
-      for (int k = 0; k < 3; k++) {
-        nn_site.x[k + 1] = is_arm ? arm[k][0] : zig[k][0];
-        nn_site.y[k + 1] = is_arm ? arm[k][1] : zig[k][1];
-        nn_site.z[k + 1] = is_arm ? arm[k][2] : zig[k][2];
-      }
-      if (!set_list(L.box, ss, s.x, s.y, s.z, nn_site)) sc.abort = ERR_GEOM;
-      else wr(L, s.i, ss);
-    }
-    sc.nn = nn_site;
-    resolve_neighbours(L, sc, 4);
-    touched_list(s, sc, 4);
-  }
-  MSKG_SYNC();
-  if (sc.abort) {
-    if (lane == 0) res.err = sc.abort;
-    return;
-  }
+// The neighbours lane `lane` visits: neighbour `lane` (lanes 0..n-1), or all
+// n in order on lane 0 when two list entries are the same point.
+__device__ inline void visit_range(const EvScratch &sc, int lane, int n, int &i0, int &i1) {
+  i0 = lane;
+  i1 = lane < n ? lane + 1 : 0;
   if (sc.seq) {
-    if (lane == 0)
-      for (int i = 0; i < 4; i++) dep_neighbour(L, rtot, s, i, sc, sc.err[i]);
-  } else if (lane < 4) {
-    dep_neighbour(L, rtot, s, lane, sc, sc.err[lane]);
+    i0 = 0;
+    i1 = lane == 0 ? n : 0;
   }
-  MSKG_SYNC();
-  // last phase: lane 0 refreshes the site and, with the second-neighbour
-  // term, lanes 1..12 the atoms of its second shell -- one call site, so the
-  // lanes run the evaluation together
-  FinalRefresh f;
-  f.active = false;
-  if (lane == 0) {
-    int coor = sc.coor_old;
-    for (int i = 0; i < 4; i++) {
-      coor += sc.inc[i];
-      if (sc.err[i]) res.err = sc.err[i];
+}
+
+// Deposition, mcevent/deposition.f:24-74: the site takes the species and,
+// at coordination 1, its stacking (FIND_SN + IF_Arm_Choice).  Lane 0.
+__device__ void dep_site(const Lattice &L, const P3 &s, int species, const Choice &ch, EvResult &res,
+                         EvScratch &sc) {
+  const Geometry &g = *L.geo;
+  Site ss = rd(L, s.i);
+  int coor = s_coor(ss);
+  sc.coor_old = coor;
+  s_set_kind(ss, species);
+  wr(L, s.i, ss);
+  res.arm = -1;
+  NL nn_site = list_of(L, s, ss);
+  if (coor == 1) {
+    P3 c = entry(L, nn_site, 0);
+    Site scr = rd(L, c.i);
+    NL lc = list_of(L, c, scr);
+    int sn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, j = 0;
+    for (int i = 0; i < 4; i++)
+      if (!same(lc, i, s)) {
+        if (j < 3) { sn[j][0] = lc.x[i]; sn[j][1] = lc.y[i]; sn[j][2] = lc.z[i]; }
+        j++;
+      }
+    int site[3] = {s.x, s.y, s.z}, cc[3] = {c.x, c.y, c.z}, zig[3][3], arm[3][3];
+    find_sn(g, L.box.lx, L.box.ly, site, cc, sn, zig, arm);
+    Site sa[3], sz[3];
+    for (int k = 0; k < 3; k++) {
+      sa[k] = rd(L, at(L, arm[k][0], arm[k][1], arm[k][2]).i);
+      sz[k] = rd(L, at(L, zig[k][0], zig[k][1], zig[k][2]).i);
     }
-    Site ss = rd(L, s.i);
-    if (coor != sc.coor_old) {
-      s_set_coor(ss, coor);
-      wr(L, s.i, ss);
+    int carm = 0, czig = 0;  // IF_Arm_Choice.f:35-54
+    for (int k = 0; k < 3; k++) {
+      carm += s_coor(sa[k]);
+      czig += s_coor(sz[k]);
     }
-    f.active = true;
-    f.p = s;
-    f.sp = ss;
-    f.coor = coor;
-    f.nn = sc.nn;
-  } else if (lane <= 12 && L.tab->use_f2) {
-    second_setup(L, s, lane - 1, sc, f);
+    int is_arm;
+    if (ch.arm >= 0) is_arm = ch.arm;
+    else if (czig > carm) is_arm = 0;
+    else if (czig == carm) is_arm = (ch.u_arm <= L.tab->ptrans_zig) ? 0 : 1;
+    else is_arm = 1;
+    res.arm = is_arm;
+    for (int k = 0; k < 3; k++) {
+      nn_site.x[k + 1] = is_arm ? arm[k][0] : zig[k][0];
+      nn_site.y[k + 1] = is_arm ? arm[k][1] : zig[k][1];
+      nn_site.z[k + 1] = is_arm ? arm[k][2] : zig[k][2];
+    }
+    if (!set_list(L.box, ss, s.x, s.y, s.z, nn_site)) sc.abort = ERR_GEOM;
+    else wr(L, s.i, ss);
   }
-  if (f.active) {
-    int e = ERR_NONE;
-    refresh(L, rtot, f.p, f.sp, f.coor, f.nn, e);
-    if (e) {
-      if (lane == 0) res.err = e;
-      else sc.err2[lane - 1] = e;
-    }
-  }
+  sc.nn = nn_site;
+  resolve_neighbours(L, sc, 4);
+  touched_list(s, sc, 4);
 }
 
 // The scan of Evaporation.f:82-128 / 225-259 that unfixes a list when a site
@@ -624,63 +587,43 @@ __device__ inline bool one_entry_list(const Lattice &L, Site &sp, const P3 &p, c
 }
 
 // Evaporation.f:36-208, the update of neighbour i of the atom leaving s.
-// NBuff is resolved afterwards (evap_nbuff) because the reference carries
-// it from one neighbour to the next.
-__device__ void evap_neighbour(const Lattice &L, double rtot, const P3 &s, int i, EvScratch &sc, int &err) {
+// NBuff is resolved afterwards (evap_nbuff_one) because the reference
+// carries it from one neighbour to the next.  A neighbour that keeps a rate
+// drops one coordination and asks for one rate evaluation.
+__device__ void evap_prepare(const Lattice &L, const P3 &s, int i, EvScratch &sc, int &err, RefreshReq &q) {
   const P3 p = sc.pts[i];
   Site sp = rd(L, p.i);
-  int kind = s_kind(sp), coornn = s_coor(sp);
-  NL nnn = list_of(L, p, sp);
-  if (!kind_occupied(kind)) {
-    if (coornn == 1) {
-      if (recip(nnn, s)) {
-        Site z = rd(L, -1);
-        z.temp = sp.temp;  // EraseMC + LattCoo=0, keeping the temperature bits
-        wr(L, p.i, z);
-        set_rate(L, p, CLS_NONE, 0.0);
-      }
-    } else if (coornn == 2) {
-      if (recip(nnn, s)) {
-        int pos[3] = {0, 0, 0};
-        sc.unfixed[i] = 1;
-        sc.found[i] = nbuff_of(L, p, nnn, pos) ? 1 : 0;
-        for (int k = 0; k < 3; k++) sc.found_pos[i][k] = pos[k];
-        s_set_coor(sp, 1);
-        if (!one_entry_list(L, sp, p, pos)) { err = ERR_GEOM; return; }
-        wr(L, p.i, sp);
-        refresh(L, rtot, p, sp, 1, nnn, err);  // old list copy, Evaporation.f:121
-      }
-    } else if (coornn == 3) {
-      if (recip(nnn, s)) {
-        s_set_coor(sp, 2);
-        wr(L, p.i, sp);
-        refresh(L, rtot, p, sp, 2, nnn, err);
-      }
-    } else if (coornn == 4) {
-      if (recip(nnn, s)) {
-        s_set_coor(sp, 3);
-        wr(L, p.i, sp);
-        refresh(L, rtot, p, sp, 3, nnn, err);  // EraseVoid + AddMC
-      }
-    }
-  } else if (kind != KIND_WALL) {
-    if (coornn == 4) {
-      if (recip(nnn, s)) {
-        s_set_coor(sp, 3);
-        wr(L, p.i, sp);
-        refresh(L, rtot, p, sp, 3, nnn, err);  // EraseBulk + AddMC
-      }
-    } else {
-      if (recip(nnn, s)) {
-        coornn--;
-        s_set_coor(sp, coornn);
-        wr(L, p.i, sp);
-        refresh(L, rtot, p, sp, coornn, nnn, err);
-      }
-    }
-  } else {
+  const int kind = s_kind(sp), coornn = s_coor(sp);
+  const NL nnn = list_of(L, p, sp);
+  if (kind == KIND_WALL) {
     sc.inc[i] = -1;  // wall neighbour
+    return;
   }
+  const bool occ = kind_occupied(kind);
+  if (!(occ || (coornn >= 1 && coornn <= 4)) || !recip(nnn, s)) return;
+  if (!occ && coornn == 1) {  // EraseMC + LattCoo=0, keeping the temperature bits
+    Site z = rd(L, -1);
+    z.temp = sp.temp;
+    wr(L, p.i, z);
+    set_rate(L, p, CLS_NONE, 0.0);
+    return;
+  }
+  if (!occ && coornn == 2) {  // unfixed: the list keeps the reciprocal partner (NBuff)
+    int pos[3] = {0, 0, 0};
+    sc.unfixed[i] = 1;
+    sc.found[i] = nbuff_of(L, p, nnn, pos) ? 1 : 0;
+    for (int k = 0; k < 3; k++) sc.found_pos[i][k] = pos[k];
+    if (!one_entry_list(L, sp, p, pos)) { err = ERR_GEOM; return; }
+  }
+  // coordination 4 -> 3 is EraseBulk/EraseVoid + AddMC; the rate of an
+  // unfixed site comes from its old list copy (Evaporation.f:121)
+  s_set_coor(sp, coornn - 1);
+  wr(L, p.i, sp);
+  q.need = true;
+  q.p = p;
+  q.sp = sp;
+  q.coor = coornn - 1;
+  q.nn = nnn;
 }
 
 // The NBuff the reference carries across the neighbours: a neighbour that
@@ -699,144 +642,46 @@ __device__ void evap_nbuff_one(const Lattice &L, EvScratch &sc, int i, int carri
   }
 }
 
-// Evaporation, mcevent/Evaporation.f:25-262.
-__device__ void evaporation(const Lattice &L, double rtot, const P3 &s, EvResult &res, EvScratch &sc, int lane) {
-  if (lane == 0) {
-    sc.abort = 0;
-    Site ss = rd(L, s.i);
-    sc.coor_old = s_coor(ss);
-    s_set_kind(ss, KIND_EMPTY);
-    wr(L, s.i, ss);
-    sc.nn = list_of(L, s, ss);
-    resolve_neighbours(L, sc, 4);
-    touched_list(s, sc, 4);
-  }
-  MSKG_SYNC();
-  int nbuff[3] = {0, 0, 0};
-  if (sc.seq) {
-    if (lane == 0)
-      for (int i = 0; i < 4; i++) {  // in order, NBuff applied as each neighbour is visited
-        evap_neighbour(L, rtot, s, i, sc, sc.err[i]);
-        evap_nbuff_one(L, sc, i, nbuff, sc.err[i]);
-      }
-  } else if (lane < 4) {
-    evap_neighbour(L, rtot, s, lane, sc, sc.err[lane]);
-  }
-  MSKG_SYNC();
-  // last phase, as in deposition: lane 0 the site, lanes 1..12 the second
-  // shell, one Get_Prob call site
-  FinalRefresh f;
-  f.active = false;
-  Site ss;
-  int coor = 0;
-  if (lane == 0) {
-    coor = sc.coor_old;
-    for (int i = 0; i < 4; i++) {
-      coor += sc.inc[i];
-      if (sc.err[i]) res.err = sc.err[i];
-    }
-    if (!sc.seq)
-      for (int i = 0; i < 4; i++) evap_nbuff_one(L, sc, i, nbuff, res.err);
-    ss = rd(L, s.i);
-    s_set_coor(ss, coor);
-    if (coor == 0) {  // EraseMC zeroes the record
-      for (int a = 0; a < 4; a++) ss.d[a][0] = ss.d[a][1] = ss.d[a][2] = 0;
-      ss.valid = 0;
-      wr(L, s.i, ss);
-      set_rate(L, s, CLS_NONE, 0.0);
-    } else {
-      wr(L, s.i, ss);
-      f.active = true;
-      f.p = s;
-      f.sp = ss;
-      f.coor = coor;
-      f.nn = sc.nn;
-    }
-  } else if (lane <= 12 && L.tab->use_f2) {
-    second_setup(L, s, lane - 1, sc, f);
-  }
-  int cls = 0, key = -1, e = ERR_NONE;
-  double r = 0.0;
-  if (f.active) get_prob(L, rtot, f.p, f.sp, f.coor, f.nn, cls, r, e, nullptr, &key);
-  if (lane == 0 && f.active && coor == 1) {  // the site keeps one reciprocal bond (NBuff)
-    int pos[3];
-    if (nbuff_of(L, s, sc.nn, pos))
-      for (int k = 0; k < 3; k++) nbuff[k] = pos[k];
-    if (!one_entry_list(L, ss, s, nbuff)) e = ERR_GEOM;
-    else wr(L, s.i, ss);
-  }
-  if (f.active) {
-    set_rate(L, f.p, cls, r, key);
-    if (e) {
-      if (lane == 0) res.err = e;
-      else sc.err2[lane - 1] = e;
-    }
-  }
+// Evaporation, mcevent/Evaporation.f:25-35: the site empties.  Lane 0.
+__device__ inline void evap_site(const Lattice &L, const P3 &s, EvScratch &sc) {
+  Site ss = rd(L, s.i);
+  sc.coor_old = s_coor(ss);
+  s_set_kind(ss, KIND_EMPTY);
+  wr(L, s.i, ss);
+  sc.nn = list_of(L, s, ss);
+  resolve_neighbours(L, sc, 4);
+  touched_list(s, sc, 4);
 }
 
-// absorption, mcevent/absorption.f:25-78 (slot = ListCov index 1..3).
-__device__ void absorption(const Lattice &L, double rtot, const P3 &s, int slot, EvResult &res, EvScratch &sc,
-                           int lane) {
-  if (lane == 0) {
-    Site ss = rd(L, s.i);
-    sc.coor_old = s_coor(ss);
-    s_set_kind(ss, KIND_COV0 + slot - 1);
-    wr(L, s.i, ss);
-    sc.nn = list_of(L, s, ss);
-    resolve_neighbours(L, sc, 4);
-    touched_list(s, sc, 4);
-  }
-  MSKG_SYNC();
-  for (int i = sc.seq ? 0 : lane; i < 4; i += sc.seq ? 1 : 64) {
-    if (sc.seq && lane != 0) break;
-    const P3 p = sc.pts[i];
-    Site sp = rd(L, p.i);
-    int k = s_kind(sp);
-    if (kind_occupied(k) && k != KIND_WALL && s_coor(sp) <= 3) {
-      NL nnn = list_of(L, p, sp);
-      if (recip(nnn, s)) refresh(L, rtot, p, sp, s_coor(sp), nnn, sc.err[i]);
-    }
-  }
-  MSKG_SYNC();
-  if (lane == 0) {
-    for (int i = 0; i < 4; i++)
-      if (sc.err[i]) res.err = sc.err[i];
-    Site ss = rd(L, s.i);
-    refresh(L, rtot, s, ss, sc.coor_old, sc.nn, res.err);
-  }
+// absorption.f:25-78 / desorption.f:25-117: the site takes (absorption,
+// slot = ListCov index 1..3) or loses (slot 0) its coverage species; at
+// coordination 1 desorption visits only the first neighbour.  Lane 0.
+__device__ inline void cov_site(const Lattice &L, const P3 &s, int slot, EvScratch &sc) {
+  Site ss = rd(L, s.i);
+  sc.coor_old = s_coor(ss);
+  s_set_kind(ss, slot ? KIND_COV0 + slot - 1 : KIND_EMPTY);
+  wr(L, s.i, ss);
+  sc.nn = list_of(L, s, ss);
+  sc.nvisit = (!slot && sc.coor_old == 1) ? 1 : 4;
+  resolve_neighbours(L, sc, sc.nvisit);
+  touched_list(s, sc, sc.nvisit);
 }
 
-// desorption, mcevent/desorption.f:25-117.
-__device__ void desorption(const Lattice &L, double rtot, const P3 &s, EvResult &res, EvScratch &sc, int lane) {
-  if (lane == 0) {
-    Site ss = rd(L, s.i);
-    sc.coor_old = s_coor(ss);
-    s_set_kind(ss, KIND_EMPTY);
-    wr(L, s.i, ss);
-    sc.nn = list_of(L, s, ss);
-    const int nloop = sc.coor_old == 1 ? 1 : 4;
-    resolve_neighbours(L, sc, nloop);
-    touched_list(s, sc, nloop);
-  }
-  MSKG_SYNC();
-  const int nloop = sc.coor_old == 1 ? 1 : 4;
-  for (int i = sc.seq ? 0 : lane; i < nloop; i += sc.seq ? 1 : 64) {
-    if (sc.seq && lane != 0) break;
-    const P3 p = sc.pts[i];
-    Site sp = rd(L, p.i);
-    int k = s_kind(sp);
-    if (kind_occupied(k) && k != KIND_WALL) {
-      NL nnn = list_of(L, p, sp);
-      if (recip(nnn, s)) refresh(L, rtot, p, sp, s_coor(sp), nnn, sc.err[i]);
-    }
-  }
-  MSKG_SYNC();
-  if (lane == 0) {
-    for (int i = 0; i < 4; i++)
-      if (sc.err[i]) res.err = sc.err[i];
-    Site ss = rd(L, s.i);
-    refresh(L, rtot, s, ss, sc.coor_old, sc.nn, res.err);
-  }
+// Their neighbour i: an atom bonded back to s gets a fresh rate
+// (absorption only for coordination <= 3, absorption.f:45-60;
+// desorption.f:45-105).
+__device__ inline void cov_prepare(const Lattice &L, const P3 &s, int i, bool absorb, EvScratch &sc, RefreshReq &q) {
+  const P3 p = sc.pts[i];
+  const Site sp = rd(L, p.i);
+  const int k = s_kind(sp);
+  if (!kind_occupied(k) || k == KIND_WALL || (absorb && s_coor(sp) > 3)) return;
+  const NL nnn = list_of(L, p, sp);
+  if (!recip(nnn, s)) return;
+  q.need = true;
+  q.p = p;
+  q.sp = sp;
+  q.coor = s_coor(sp);
+  q.nn = nnn;
 }
 
 // The reference event an MC particle of class `cls` executes.  For an
@@ -874,22 +719,116 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
 }
 
 // Execute one event of type itrans at site s.  Called by every lane of the
-// wave with the same arguments; res is complete on lane 0.
+// wave with the same arguments; res is complete on lane 0.  Three phases:
+// lane 0 updates the site; lanes 0..3 update its neighbours (one rate
+// evaluation each); lane 0 gives the site its final rate while, with the
+// second-neighbour term, lanes 1..12 refresh the atoms of its second shell.
+// Each phase has one rate-evaluation call site for every event type, which
+// keeps the inlined kernel small.
 __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, const Choice &ch, EvResult &res,
                         EvScratch &sc, int lane) {
   res.err = ERR_NONE;
   res.arm = -1;
-  if (lane == 0) sc.ntouched = 0;
+  const bool dep = itrans <= 3, evap = itrans == 4, absorb = itrans >= 6;
   if (lane < 12) {
     sc.sec[lane].i = -1;
     sc.err2[lane] = 0;
   }
-  if (itrans <= 3) deposition(L, rtot, s, itrans, ch, res, sc, lane);
-  else if (itrans == 4) evaporation(L, rtot, s, res, sc, lane);
-  else if (itrans == 5) desorption(L, rtot, s, res, sc, lane);
-  else absorption(L, rtot, s, itrans - 5, res, sc, lane);
-  if (L.tab->use_f2 && itrans <= 4) {
-    MSKG_SYNC();  // the second-shell refreshes (run in the event's last phase) are done
+  if (lane == 0) {
+    sc.ntouched = 0;
+    sc.abort = 0;
+    sc.nvisit = 4;
+    if (dep) dep_site(L, s, itrans, ch, res, sc);
+    else if (evap) evap_site(L, s, sc);
+    else cov_site(L, s, absorb ? itrans - 5 : 0, sc);
+  }
+  MSKG_SYNC();
+  if (sc.abort) {
+    if (lane == 0) res.err = sc.abort;
+    return;
+  }
+  // phase 2: the neighbours
+  int nbuff[3] = {0, 0, 0};  // Evaporation.f's NBuff, carried on lane 0
+  {
+    int i0, i1;
+    visit_range(sc, lane, sc.nvisit, i0, i1);
+#pragma unroll 1
+    for (int i = i0; i < i1; i++) {
+      RefreshReq q;
+      q.need = q.list_after = false;
+      if (dep) dep_prepare(L, s, i, sc, sc.err[i], q);
+      else if (evap) evap_prepare(L, s, i, sc, sc.err[i], q);
+      else cov_prepare(L, s, i, absorb, sc, q);
+      if (q.need) {
+        int cls = 0, key = -1;
+        double r = 0.0;
+        get_prob(L, rtot, q.p, q.sp, q.coor, q.nn, cls, r, sc.err[i], nullptr, &key);
+        bool ok = true;
+        if (q.list_after) {
+          if (cls && !set_list(L.box, q.sp, q.p.x, q.p.y, q.p.z, q.nn)) {
+            sc.err[i] = ERR_GEOM;
+            ok = false;
+          } else {
+            wr(L, q.p.i, q.sp);
+          }
+        }
+        if (ok) set_rate(L, q.p, cls, r, key);
+      }
+      if (evap && sc.seq) evap_nbuff_one(L, sc, i, nbuff, sc.err[i]);  // in order, as each neighbour is visited
+    }
+  }
+  MSKG_SYNC();
+  // phase 3: the site's final rate and the second shell
+  FinalRefresh f;
+  f.active = false;
+  Site ss;
+  bool unfix = false;
+  if (lane == 0) {
+    int coor = sc.coor_old;
+    for (int i = 0; i < 4; i++) {
+      if (dep || evap) coor += sc.inc[i];
+      if (sc.err[i]) res.err = sc.err[i];
+    }
+    if (evap && !sc.seq)
+      for (int i = 0; i < 4; i++) evap_nbuff_one(L, sc, i, nbuff, res.err);
+    ss = rd(L, s.i);
+    if (coor != sc.coor_old) s_set_coor(ss, coor);
+    if (evap && coor == 0) {  // EraseMC zeroes the record
+      for (int a = 0; a < 4; a++) ss.d[a][0] = ss.d[a][1] = ss.d[a][2] = 0;
+      ss.valid = 0;
+      wr(L, s.i, ss);
+      set_rate(L, s, CLS_NONE, 0.0);
+    } else {
+      if (coor != sc.coor_old || evap) wr(L, s.i, ss);
+      f.active = true;
+      f.p = s;
+      f.sp = ss;
+      f.coor = coor;
+      f.nn = sc.nn;
+      unfix = evap && coor == 1;  // the site keeps one reciprocal bond (NBuff)
+    }
+  } else if (lane <= 12 && L.tab->use_f2 && (dep || evap)) {
+    second_setup(L, s, lane - 1, sc, f);
+  }
+  int cls = 0, key = -1, e = ERR_NONE;
+  double r = 0.0;
+  if (f.active) get_prob(L, rtot, f.p, f.sp, f.coor, f.nn, cls, r, e, nullptr, &key);
+  if (unfix) {
+    int pos[3];
+    if (nbuff_of(L, s, sc.nn, pos))
+      for (int k = 0; k < 3; k++) nbuff[k] = pos[k];
+    if (!one_entry_list(L, ss, s, nbuff)) e = ERR_GEOM;
+    else wr(L, s.i, ss);
+  }
+  if (f.active) {
+    set_rate(L, f.p, cls, r, key);
+    if (e) {
+      if (lane == 0) res.err = e;
+      else sc.err2[lane - 1] = e;
+    }
+  }
+  if (L.tab->use_f2 && (dep || evap)) {
+    MSKG_SYNC();  // the second-shell refreshes are done
     if (lane == 0)
       for (int k = 0; k < 12; k++) {
         if (sc.err2[k]) res.err = sc.err2[k];
