@@ -333,11 +333,13 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
   int32_t *lk = (int32_t *)smem;                  // tile-local point offsets
   float *lr = (float *)(lk + cap);                // rates
   uint8_t *lc = (uint8_t *)(lr + cap);            // classes
-  __shared__ int s_n, s_stop, s_nt;
+  __shared__ int s_stop, s_nt;
   __shared__ P3 s_tp[MAX_TOUCHED];
   __shared__ int s_tk[MAX_TOUCHED];
   __shared__ float s_tr[MAX_TOUCHED];
   __shared__ uint8_t s_tc[MAX_TOUCHED];
+  __shared__ int s_found[MAX_TOUCHED];
+  __shared__ unsigned long long s_cnt[8];  // events by type (lane 0)
   __shared__ EvScratch sc;
 
   RunState *st = P.st;
@@ -396,7 +398,8 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
   // 2. rejection-free KMC inside the tile for the window tau
   const double rtot = st->rtot;
   double tloc = 0.0;
-  unsigned long long nev = 0, cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long nev = 0;
+  if (lane < 8) s_cnt[lane] = 0;
   U4 ctr;
   ctr.x = (uint32_t)t;
   ctr.y = (uint32_t)st->sweeps;
@@ -406,9 +409,12 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
   // every wave ends its pass: the window tau bounds the expected number of
   // events, and this cap bounds the worst case
   for (int guard = 0; guard < (1 << 16); guard++) {
+    // lane-major order: lane l owns entries l, l + 64, ...; one wave scan of
+    // the lanes' partial sums gives the total and the lane that holds the pick
     double part = 0.0;
     for (int j = lane; j < n; j += WAVE) part += (double)lr[j];
-    const double R = wave_sum(part);
+    const double inc = wave_incl_scan(part, lane);
+    const double R = __shfl(inc, WAVE - 1, WAVE);
     if (!(R > 0.0)) break;
     U4 d = philox4x32_10(ctr, P.key0, P.key1);
     ctr.w++;
@@ -417,23 +423,31 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
     if (tloc + dt > tau) break;
     tloc += dt;
     const double target = u01_open(d.z, d.w) * R;
-    // select the first particle whose inclusive prefix exceeds target
-    double acc = 0.0;
-    int sel = -1, last_nz = -1;
-    for (int base = 0; base < n; base += WAVE) {
-      int j = base + lane;
-      double v = j < n ? (double)lr[j] : 0.0;
-      unsigned long long nzm = __ballot(v > 0.0);
-      if (nzm) last_nz = base + 63 - __clzll((long long)nzm);
-      double inc = wave_incl_scan(v, lane) + acc;
-      unsigned long long hit = __ballot(j < n && v > 0.0 && inc > target);
-      if (hit) {
-        sel = base + __ffsll((long long)hit) - 1;
-        break;
+    // the first lane whose inclusive prefix exceeds target (the last lane
+    // with a rate when rounding leaves target at the top end), then the
+    // first of its entries whose running prefix exceeds target
+    const unsigned long long hit = __ballot(part > 0.0 && inc > target);
+    const unsigned long long nzl = __ballot(part > 0.0);
+    if (!nzl) break;
+    const int owner = hit ? __ffsll((long long)hit) - 1 : 63 - __clzll((long long)nzl);
+    int sel = -1;
+    if (lane == owner) {
+      double acc = inc - part;
+      int last = -1;
+      for (int j = lane; j < n; j += WAVE) {
+        const double v = (double)lr[j];
+        if (v > 0.0) {
+          last = j;
+          acc += v;
+          if (acc > target) {
+            sel = j;
+            break;
+          }
+        }
       }
-      acc = __shfl(inc, WAVE - 1, WAVE);
+      if (sel < 0) sel = last;
     }
-    if (sel < 0) sel = last_nz;  // rounding at the top end of the scan
+    sel = __shfl(sel, owner, WAVE);
     if (sel < 0) break;
     {
       const P3 s = tile_point(L.box, g, L.tile, lk[sel]);
@@ -464,7 +478,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
           }
         }
         int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
-        if (slot >= 0 && slot < 8) cnt[slot]++;
+        if (slot >= 0 && slot < 8) s_cnt[slot]++;
         for (int q = 0; q < sc.ntouched; q++) s_tp[q] = sc.touched[q];
         s_nt = sc.ntouched;
         s_stop = err;
@@ -483,45 +497,55 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
       const P3 tp = s_tp[lane];
       const int kl = tile_local(g, L.tile, tp);
       s_tk[lane] = kl;
+      s_found[lane] = -1;
       if (kl >= 0) {
         s_tr[lane] = L.rate[tp.i];
         s_tc[lane] = L.cls[tp.i];
       }
     }
     __syncthreads();
-    // refresh the on-chip list for touched points inside the tile
-    for (int q = 0; q < nt; q++) {
-      const int kl = s_tk[q];
-      if (kl < 0) continue;
-      int found = -1;
-      for (int base = 0; base < n; base += WAVE) {
-        int j = base + lane;
-        unsigned long long h = __ballot(j < n && lk[j] == kl);
-        if (h) { found = base + __ffsll((long long)h) - 1; break; }
-      }
-      if (lane == 0) {
-        if (found >= 0) {
-          lr[found] = s_tr[q];
-          lc[found] = s_tc[q];
-          s_n = n;
-        } else if (s_tc[q]) {
-          if (n < cap) {
-            lk[n] = kl;
-            lr[n] = s_tr[q];
-            lc[n] = s_tc[q];
-            s_n = n + 1;
-          } else {
-            s_n = -1;
-          }
-        } else {
-          s_n = n;
+    // refresh the on-chip list: every lane matches its entries against the
+    // touched offsets (list entries are distinct, so each touched point has
+    // at most one slot; a point touched twice finds the same slot) ...
+    for (int j = lane; j < n; j += WAVE) {
+      const int kj = lk[j];
+      for (int q = 0; q < nt; q++)
+        if (s_tk[q] == kj) s_found[q] = j;
+    }
+    __syncthreads();
+    // ... updates in place, and appends the new MC particles (first
+    // occurrence of each) after the list in touched order
+    bool app = false;
+    int kl = -1;
+    if (lane < nt) {
+      kl = s_tk[lane];
+      if (kl >= 0) {
+        const int f = s_found[lane];
+        if (f >= 0) {
+          lr[f] = s_tr[lane];
+          lc[f] = s_tc[lane];
+        } else if (s_tc[lane]) {
+          app = true;
+          for (int q = 0; q < lane; q++)
+            if (s_tk[q] == kl) app = false;
         }
       }
-      __syncthreads();
-      if (s_n < 0) { err = ERR_CAPACITY; break; }
-      n = s_n;
     }
-    if (err) break;
+    const unsigned long long am = __ballot(app);
+    if (app) {
+      const int slot = n + __popcll(am & ((1ull << lane) - 1ull));
+      if (slot < cap) {
+        lk[slot] = kl;
+        lr[slot] = s_tr[lane];
+        lc[slot] = s_tc[lane];
+      }
+    }
+    if (n + __popcll(am) > cap) {  // the list keeps its first cap entries
+      err = ERR_CAPACITY;
+      break;
+    }
+    n += __popcll(am);
+    __syncthreads();
   }
   // 3. write the list back (active entries only), tile totals, counters
   double part = 0.0;
@@ -539,7 +563,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
   const double R = wave_sum(part);
   if (lane == 0) {
     L.tl_n[t] = nout;
-    slot_add(st, t, R, (unsigned long long)nout, nev, cnt);
+    slot_add(st, t, R, (unsigned long long)nout, nev, s_cnt);
     if (nout) {
       if (g.tz < L.win[4 * P.colour + 2]) atomic_min_i32(&L.win[4 * P.colour + 2], g.tz);
       if (g.tz > L.win[4 * P.colour + 3]) atomic_max_i32(&L.win[4 * P.colour + 3], g.tz);
