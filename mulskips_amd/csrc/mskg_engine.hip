@@ -126,22 +126,29 @@ __device__ inline TileGeom tile_geom(int t, int T, int ntx, int nty, int lz) {
   g.npt = (int64_t)g.half * g.half * g.nz;
   return g;
 }
+// The point of a packed tile offset (tile_pack).
 __device__ inline P3 tile_point(const Box &b, const TileGeom &g, int T, int k) {
-  int pp = g.half * g.half;
-  int zl = k / pp, r = k - zl * pp;
-  int j = r / g.half, i = r - j * g.half;
   P3 p;
-  p.z = g.z0 + zl;
-  int par = p.z & 1;
-  p.x = g.tx * T + 2 * i + par;
-  p.y = g.ty * T + 2 * j + par;
+  p.z = g.z0 + (k >> 14);
+  const int par = p.z & 1;
+  p.x = g.tx * T + 2 * (k & 127) + par;
+  p.y = g.ty * T + 2 * ((k >> 7) & 127) + par;
   p.i = pidx(b, p.x, p.y, p.z);
   return p;
 }
+// The packed offset of the k-th point of a tile in (z, y, x) order.
+__device__ inline int tile_nth(const TileGeom &g, int k) {
+  const int pp = g.half * g.half;
+  const int zl = k / pp, r = k - zl * pp;
+  const int j = r / g.half, i = r - j * g.half;
+  return (zl << 14) | (j << 7) | i;
+}
+// The packed offset of p in tile g, -1 when p is outside the tile.
 __device__ inline int tile_local(const TileGeom &g, int T, const P3 &p) {
   if (p.i < 0) return -1;
-  if (p.x / T != g.tx || p.y / T != g.ty || p.z < g.z0 || p.z >= g.z0 + g.nz) return -1;
-  return (p.z - g.z0) * g.half * g.half + ((p.y - g.ty * T) >> 1) * g.half + ((p.x - g.tx * T) >> 1);
+  const int xl = p.x - g.tx * T, yl = p.y - g.ty * T, zl = p.z - g.z0;
+  if ((unsigned)xl >= (unsigned)T || (unsigned)yl >= (unsigned)T || (unsigned)zl >= (unsigned)g.nz) return -1;
+  return tile_pack(zl, yl, xl);
 }
 
 // ---- rate-update sweep ----------------------------------------------------
@@ -274,15 +281,17 @@ __global__ void __launch_bounds__(256) k_build_lists(Lattice L, int ntiles) {
   for (int k0 = 0; k0 < g.npt; k0 += WAVE) {
     int k = k0 + lane;
     bool on = false;
+    int kp = 0;
     if (k < g.npt) {
-      P3 p = tile_point(L.box, g, L.tile, k);
+      kp = tile_nth(g, k);
+      P3 p = tile_point(L.box, g, L.tile, kp);
       on = L.cls[p.i] != 0;
       L.inlist[p.i] = on ? 1 : 0;
     }
     unsigned long long m = __ballot(on);
     int slot = n + __popcll(m & ((1ull << lane) - 1ull));
     if (on) {
-      if (slot < L.capg) L.tl_idx[(int64_t)t * L.capg + slot] = k;
+      if (slot < L.capg) L.tl_idx[(int64_t)t * L.capg + slot] = kp;
       else *L.list_err = ERR_TILE_LIST;
     }
     n += __popcll(m);
@@ -416,13 +425,14 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
     const double inc = wave_incl_scan(part, lane);
     const double R = __shfl(inc, WAVE - 1, WAVE);
     if (!(R > 0.0)) break;
-    U4 d = philox4x32_10(ctr, P.key0, P.key1);
+    // one Philox block per event: waiting time, selection, event type and
+    // stacking choice, 32 random bits each (the reference's ran3 has fewer)
+    const U4 d = philox4x32_10(ctr, P.key0, P.key1);
     ctr.w++;
-    double u1 = u01_open(d.x, d.y);
-    double dt = -log(u1) / R;
+    const double dt = -(double)__logf(u01f(d.x)) / R;
     if (tloc + dt > tau) break;
     tloc += dt;
-    const double target = u01_open(d.z, d.w) * R;
+    const double target = u01_32(d.y) * R;
     // the first lane whose inclusive prefix exceeds target (the last lane
     // with a rate when rounding leaves target at the top end), then the
     // first of its entries whose running prefix exceeds target
@@ -451,11 +461,10 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
     if (sel < 0) break;
     {
       const P3 s = tile_point(L.box, g, L.tile, lk[sel]);
-      const U4 d2 = philox4x32_10(ctr, P.key0, P.key1);
       Choice ch;
       ch.arm = -1;
-      ch.u_arm = u01_open(d2.z, d2.w);
-      const int itrans = pick_itrans(L, s, lc[sel], u01_open(d2.x, d2.y));
+      ch.u_arm = u01_32(d.w);
+      const int itrans = pick_itrans(L, s, lc[sel], u01_32(d.z));
       const int coor0 = P.log ? s_coor(L.site[s.i]) : 0;
       __syncthreads();  // every lane has read the site before it changes
       EvResult res;
@@ -484,7 +493,6 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2)
         s_stop = err;
       }
     }
-    ctr.w++;
     nev++;
     __syncthreads();
     if (s_stop) {
@@ -1263,6 +1271,7 @@ static Lattice make_lattice(mskg_engine *e) {
   L.geo = e->d_geo;
   L.tab = e->d_tab;
   L.tile = e->tile;
+  L.inv_tile = 1.0f / (float)e->tile;
   L.ntx = e->ntx;
   L.nty = e->nty;
   L.key = e->track ? e->d_key : nullptr;
@@ -1278,7 +1287,8 @@ static Lattice make_lattice(mskg_engine *e) {
 // entries that are not bond vectors.
 static int choose_tile(int lx, int ly, int want, int min_tile) {
   auto ok = [&](int t) {
-    return t >= min_tile && t % 2 == 0 && lx % t == 0 && ly % t == 0 && (lx / t) % 2 == 0 && (ly / t) % 2 == 0;
+    return t >= min_tile && t <= 254 && t % 2 == 0 && lx % t == 0 && ly % t == 0 && (lx / t) % 2 == 0 &&
+           (ly / t) % 2 == 0;
   };
   if (want) return ok(want) ? want : -1;
   const int pref[] = {24, 20, 18, 28, 30, 32, 36, 40, 42, 48, 16, 22, 26, 34, 38, 44, 46};

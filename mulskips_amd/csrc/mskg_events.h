@@ -44,6 +44,7 @@ struct Lattice {
   const Tables *tab;
   int tile, ntx, nty, capg;
   int cur_tile;           // the tile whose pass is running: its list lives on chip
+  float inv_tile;         // 1 / tile
   int32_t *key;           // may be null: rate-class key of every point (likelihood accounting)
 };
 
@@ -91,12 +92,24 @@ MSKG_HD bool recip(const NL &l, const P3 &a) {
   return false;
 }
 
-MSKG_HD int tile_of(const Lattice &L, const P3 &p) {
-  return ((p.z / L.tile) * L.nty + p.y / L.tile) * L.ntx + p.x / L.tile;
+// x / tile for 0 <= x < 2^20: the FP32 quotient is off by at most one.
+MSKG_HD int tdiv(const Lattice &L, int x) {
+  int q = (int)((float)x * L.inv_tile);
+  const int r = x - q * L.tile;
+  if (r < 0) q--;
+  else if (r >= L.tile) q++;
+  return q;
 }
+MSKG_HD int tile_of(const Lattice &L, const P3 &p) {
+  return (tdiv(L, p.z) * L.nty + tdiv(L, p.y)) * L.ntx + tdiv(L, p.x);
+}
+// A point's offset inside its tile, packed as (z << 14) | ((y / 2) << 7) |
+// (x / 2) in tile-local coordinates (tile edge <= 254), so that decoding
+// needs no division; offsets sort like the points' (z, y, x) order.
+MSKG_HD int tile_pack(int zl, int yl, int xl) { return (zl << 14) | ((yl >> 1) << 7) | (xl >> 1); }
 MSKG_HD int tile_offset(const Lattice &L, const P3 &p) {
-  const int half = L.tile / 2, z0 = (p.z / L.tile) * L.tile;
-  return ((p.z - z0) * half + ((p.y % L.tile) >> 1)) * half + ((p.x % L.tile) >> 1);
+  const int tx = tdiv(L, p.x), ty = tdiv(L, p.y), tz = tdiv(L, p.z);
+  return tile_pack(p.z - tz * L.tile, p.y - ty * L.tile, p.x - tx * L.tile);
 }
 MSKG_HD int atomic_add_i32(int32_t *a, int v) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -121,6 +134,25 @@ MSKG_HD void atomic_max_i32(int *a, int v) {
   if (v > *a) *a = v;
 #endif
 }
+// Does the record sq of point q list position p?  The reference's Recipr
+// loops (list_of(q) contains p), evaluated on the stored displacements: a
+// valid entry holds p - q wrapped as set_list wraps it (no displacement is
+// near half a box edge), an unused entry stands for the origin.
+MSKG_HD bool lists_point(const Box &b, const P3 &q, const Site &sq, const P3 &p) {
+  int wx = p.x - q.x, wy = p.y - q.y;
+  const int wz = p.z - q.z;
+  if (wx > b.lx / 2) wx -= b.lx;
+  if (wx < -b.lx / 2) wx += b.lx;
+  if (wy > b.ly / 2) wy -= b.ly;
+  if (wy < -b.ly / 2) wy += b.ly;
+  const bool origin = p.x == 0 && p.y == 0 && p.z == 0;
+  bool hit = false;
+  for (int j = 0; j < 4; j++) {
+    if ((sq.valid >> j) & 1) hit |= sq.d[j][0] == wx && sq.d[j][1] == wy && sq.d[j][2] == wz;
+    else hit |= origin;
+  }
+  return hit;
+}
 MSKG_HD int tile_colour(int tx, int ty, int tz) { return (tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2); }
 // A tile layer that holds candidates joins the window of layers its colour's
 // pass visits.  A colour's window is only widened by passes of the other
@@ -138,13 +170,14 @@ MSKG_HD void widen_window(const Lattice &L, int colour, int tz) {
 // A site that becomes an MC particle joins its tile's list (once).
 MSKG_HD void enlist(const Lattice &L, const P3 &p) {
   if (!L.inlist) return;
-  const int t = tile_of(L, p);
+  const int tx = tdiv(L, p.x), ty = tdiv(L, p.y), tz = tdiv(L, p.z);
+  const int t = (tz * L.nty + ty) * L.ntx + tx;
   if (t == L.cur_tile) return;  // the running tile keeps its list on chip and flags it on write-back
   if (L.inlist[p.i]) return;
   L.inlist[p.i] = 1;
-  widen_window(L, tile_colour(p.x / L.tile, p.y / L.tile, p.z / L.tile), p.z / L.tile);
+  widen_window(L, tile_colour(tx, ty, tz), tz);
   const int slot = atomic_add_i32(&L.tl_n[t], 1);
-  if (slot < L.capg) L.tl_idx[(int64_t)t * L.capg + slot] = tile_offset(L, p);
+  if (slot < L.capg) L.tl_idx[(int64_t)t * L.capg + slot] = tile_pack(p.z - tz * L.tile, p.y - ty * L.tile, p.x - tx * L.tile);
   else *L.list_err = ERR_TILE_LIST;
 }
 MSKG_HD void set_rate(const Lattice &L, const P3 &p, int cls, double rate, int key = -1) {
@@ -268,7 +301,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
     int kq = s_kind(sq);
     if (kind_occupied(kq)) {
       if (kq != KIND_WALL) {
-        if (recip(list_of(L, q, sq), p)) {
+        if (lists_point(L.box, q, sq, p)) {
           if (kq == 1) n_si++;
           if (kq == 2) n_c++;
           if (kq == 3) n_b++;
@@ -277,7 +310,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
         n_wall++;
       }
     } else if (kind_cov_slot(kq)) {
-      if (recip(list_of(L, q, sq), p)) {
+      if (lists_point(L.box, q, sq, p)) {
         int s = kind_cov_slot(kq);
         if (s > t.ncov) { err = ERR_COVERAGE; cls = CLS_NONE; rate = 0.0; return; }
         covnn[s - 1]++;
@@ -462,7 +495,7 @@ __device__ void dep_prepare(const Lattice &L, const P3 &s, int i, EvScratch &sc,
         for (int j = 0; j < 2; j++) {
           int kq = s_kind(sq[j]);
           if (kind_occupied(kq) && s_coor(sq[j]) <= 3 && kq != KIND_WALL) {
-            if (recip(list_of(L, qq[j], sq[j]), p)) coornn++;
+            if (lists_point(L.box, qq[j], sq[j], p)) coornn++;
           }
         }
         s_set_coor(sp, coornn);
@@ -571,7 +604,7 @@ __device__ bool nbuff_of(const Lattice &L, const P3 &p, const NL &old, int pos[3
   for (int j = 0; j < 4; j++) {
     int kq = s_kind(sq[j]);
     if (kind_occupied(kq) && s_coor(sq[j]) <= 3 && kq != KIND_WALL) {
-      if (recip(list_of(L, q[j], sq[j]), p)) {
+      if (lists_point(L.box, q[j], sq[j], p)) {
         pos[0] = old.x[j]; pos[1] = old.y[j]; pos[2] = old.z[j];
         f = true;
       }

@@ -13,8 +13,17 @@ difference:
   regression deck (PtransZig 0.8) on a 144^3 grid;
 * growth and coverage: the CVD-Si-SiH2 regression deck (H coverage).
 Both window sizes the engine is run with are covered: about 1 and 4 events
-per tile and colour pass (tau policy 1).  The seeds are fixed, so the test
-is deterministic.
+per tile and colour pass (tau policy 1).  The seeds are fixed; the serial
+side is deterministic, the engine's window tau comes from FP64 rate sums
+accumulated in arrival order, so a rerun can move a rare event across a
+window edge.
+
+A case compares several observables, and the six cases hold about 24
+comparisons, so a correct engine puts one of them outside 2 standard
+errors in most realisations.  The test is therefore sequential: an
+observable outside 2 standard errors at 32 seeds is measured again on 128
+fresh seeds of both sides and must lie within 2 standard errors there.  A
+real bias that reaches 2 standard errors at 32 seeds reaches about 4 at 128.
 """
 import multiprocessing as mp
 import os
@@ -55,39 +64,58 @@ def case(tmp_path, name):
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("epp", [1.0, 4.0])
-@pytest.mark.parametrize("name", ["si-pvd", "cube-zigzag", "CVD-Si-SiH2"])
-def test_sublattice_matches_serial_statistics(tmp_path, name, epp):
-    path, d, box, z0, frozen, t_end, keys = case(tmp_path, name)
-    # serial reference: the oracle, one process per seed
+def serial_runs(path, box, t_end, z0, frozen, seeds):
     ctx = mp.get_context("spawn")
-    with ctx.Pool(min(8, SEEDS)) as pool:
-        serial = pool.map(oracle_observables, [(path, box, 1001 + 2 * s, t_end, z0, frozen) for s in range(SEEDS)])
-    # the engine from the same initial lattice
+    with ctx.Pool(min(8, len(seeds))) as pool:
+        return pool.map(oracle_observables, [(path, box, s, t_end, z0, frozen) for s in seeds])
+
+
+def parallel_runs(path, d, box, t_end, z0, frozen, epp, seeds):
     o = Oracle(*box)
     o.load_deck(path)
     if frozen:
         o.set_frozen(frozen)
     lat0, mc0, bulk0, void0 = oracle_state(o)
     o.close()
-    parallel = []
-    for s in range(SEEDS):
-        eng = Engine(box, d.tables(), seed=7000 + s, tile=24, tau_policy=1, events_per_pass=epp,
-                     frozen_below_z=frozen)
+    out = []
+    for s in seeds:
+        eng = Engine(box, d.tables(), seed=s, tile=24, tau_policy=1, events_per_pass=epp, frozen_below_z=frozen)
         eng.load_state(lat0, mc0, bulk0, void0)
         n, t, stop = eng.run(10 ** 12, max_time=t_end)
         assert stop == 0 and t >= t_end
         lat, _, _, _ = eng.store_state()
         r = observables.film(lat, box, z0)
         r["events"] = n
-        parallel.append(r)
+        out.append(r)
         eng.close()
-    res = observables.compare(parallel, serial, keys + ("events",), nsigma=2.0)
-    print(f"\n[{name}, {epp} events per tile and pass] t_end={t_end}, {SEEDS} seeds each (parallel vs serial):",
-          flush=True)
+        if len(out) % 32 == 0:
+            print(f"  engine runs: {len(out)} of {len(seeds)}", flush=True)
+    return out, lat0
+
+
+def report(title, res):
+    print(title, flush=True)
     for k, (ma, mb, dd, bound, ok) in res.items():
         print(f"  {k:10s} {ma:14.5g} {mb:14.5g}  diff {dd:+.4g}  2se {bound:.4g}  {'ok' if ok else 'OUT'}", flush=True)
-    for k in keys:
-        assert res[k][4], f"{name}: {k} differs by more than 2 standard errors: {res[k]}"
+
+
+@pytest.mark.parametrize("epp", [1.0, 4.0])
+@pytest.mark.parametrize("name", ["si-pvd", "cube-zigzag", "CVD-Si-SiH2"])
+def test_sublattice_matches_serial_statistics(tmp_path, name, epp):
+    path, d, box, z0, frozen, t_end, keys = case(tmp_path, name)
+    # serial reference: the oracle, one process per seed; the engine from the same initial lattice
+    serial = serial_runs(path, box, t_end, z0, frozen, [1001 + 2 * s for s in range(SEEDS)])
+    parallel, lat0 = parallel_runs(path, d, box, t_end, z0, frozen, epp, [7000 + s for s in range(SEEDS)])
+    res = observables.compare(parallel, serial, keys + ("events",), nsigma=2.0)
+    report(f"\n[{name}, {epp} events per tile and pass] t_end={t_end}, {SEEDS} seeds each (parallel vs serial):", res)
+    out = [k for k in keys if not res[k][4]]
+    if out:  # second stage on fresh seeds
+        n2 = 4 * SEEDS
+        serial2 = serial_runs(path, box, t_end, z0, frozen, [50001 + 2 * s for s in range(n2)])
+        parallel2, _ = parallel_runs(path, d, box, t_end, z0, frozen, epp, [17000 + s for s in range(n2)])
+        res2 = observables.compare(parallel2, serial2, tuple(out), nsigma=2.0)
+        report(f"[{name}, {epp}] second stage, {n2} fresh seeds each, for {', '.join(out)}:", res2)
+        for k in out:
+            assert res2[k][4], f"{name}: {k} differs by more than 2 standard errors at 32 and at {n2} seeds: {res2[k]}"
     # the comparison is not vacuous: the films changed
     assert np.mean([r["atoms"] for r in serial]) != observables.film(lat0, box, z0)["atoms"]
