@@ -194,15 +194,18 @@ MSKG_HD double PEv(const Tables &t, int sp, int a, int b, int c, int d, int e, i
 }
 // The reference's kb (Get_Prob.f:70).
 #define MSKG_KB 0.00008617330350
+// LA = false compiles the laser-annealing branches out (tables with la_mode 0).
+template <bool LA = true>
 MSKG_HD double pd_of(const Tables &t, int sp, int coor, int temperature) {
   double e = t.pd[(sp - 1) + 3 * (coor - 1)];
-  if (!t.la_mode) return e;
+  if (!LA || !t.la_mode) return e;
   return 0.5 * t.p0 * exp(-e / (MSKG_KB * t.tm)) * (1.0 + erf((temperature - t.tflex) / t.sigma)) *
          (e != 0.0 ? 1.0 : 0.0);
 }
+template <bool LA = true>
 MSKG_HD double pe_of(const Tables &t, int sp, int a, int b, int c, int d, int e, int f, int temperature) {
   double v = PEv(t, sp, a, b, c, d, e, f);
-  if (!t.la_mode) return v;
+  if (!LA || !t.la_mode) return v;
   return t.p0 * exp(-v / (MSKG_KB * temperature)) * (v != 0.0 ? 1.0 : 0.0);
 }
 
@@ -238,6 +241,7 @@ MSKG_HD P3 second_pos(const Lattice &L, const P3 &p, int k) {
   if (y < 0) y += L.box.ly;
   return at(L, x, y, p.z + dz);
 }
+template <bool LA = true>
 MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &cls,
                       double &rate, int &err, const Ovr *ovr = nullptr, int *key = nullptr) {
   const Tables &t = *L.tab;
@@ -246,7 +250,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   *key = -1;
   int kind = s_kind(sp);
   int temperature = 0;
-  if (t.la_mode) {
+  if (LA && t.la_mode) {
     temperature = (int)sp.temp;
     if (temperature == 0) { err = ERR_TEMPERATURE; cls = CLS_NONE; rate = 0.0; return; }
   }
@@ -262,7 +266,8 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
       bool ok = true;
       if (coor == 1) ok = s_coor(rd_o(L, entry(L, nn, 0).i, ovr)) >= 2;
       if (ok) {
-        double pdep = pd_of(t, 1, coor, temperature) + pd_of(t, 2, coor, temperature) + pd_of(t, 3, coor, temperature);
+        double pdep = pd_of<LA>(t, 1, coor, temperature) + pd_of<LA>(t, 2, coor, temperature) +
+                      pd_of<LA>(t, 3, coor, temperature);
         double pabs = t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor];
         cls = CLS_ADSORB;
         rate = pdep + pabs;
@@ -329,7 +334,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   } else {
     int c1 = t.ncov >= 1 ? covnn[0] : 0, c2 = t.ncov >= 2 ? covnn[1] : 0, c3 = t.ncov >= 3 ? covnn[2] : 0;
     cls = CLS_EVAP;
-    rate = pe_of(t, kind, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
+    rate = pe_of<LA>(t, kind, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
     int n2 = 0;
     if (t.use_f2) {
       for (int k = 0; k < 12; k++) {
@@ -723,6 +728,7 @@ __device__ inline void cov_prepare(const Lattice &L, const P3 &s, int i, bool ab
 // execution instead of at rate-update time (same distribution: the class
 // is refreshed whenever the coordination changes).  Returns itrans in the
 // reference's numbering (1..3 deposit, 4 evaporate, 5 desorb, 6.. absorb).
+template <bool LA = true>
 MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   const Tables &t = *L.tab;
   if (cls == CLS_PLACEHOLDER) return 1;
@@ -730,8 +736,9 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   if (cls == CLS_DESORB) return 5;
   Site ss = rd(L, s.i);
   int coor = s_coor(ss);
-  int temperature = t.la_mode ? (int)ss.temp : 0;
-  double p1 = pd_of(t, 1, coor, temperature), p2 = pd_of(t, 2, coor, temperature), p3 = pd_of(t, 3, coor, temperature);
+  int temperature = (LA && t.la_mode) ? (int)ss.temp : 0;
+  double p1 = pd_of<LA>(t, 1, coor, temperature), p2 = pd_of<LA>(t, 2, coor, temperature),
+         p3 = pd_of<LA>(t, 3, coor, temperature);
   double pdep = p1 + p2 + p3;
   double pabs = t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor];
   double pcurr = u * (pdep + pabs);
@@ -758,6 +765,7 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
 // second-neighbour term, lanes 1..12 refresh the atoms of its second shell.
 // Each phase has one rate-evaluation call site for every event type, which
 // keeps the inlined kernel small.
+template <bool LA = true>
 __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, const Choice &ch, EvResult &res,
                         EvScratch &sc, int lane) {
   res.err = ERR_NONE;
@@ -795,7 +803,7 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
       if (q.need) {
         int cls = 0, key = -1;
         double r = 0.0;
-        get_prob(L, rtot, q.p, q.sp, q.coor, q.nn, cls, r, sc.err[i], nullptr, &key);
+        get_prob<LA>(L, rtot, q.p, q.sp, q.coor, q.nn, cls, r, sc.err[i], nullptr, &key);
         bool ok = true;
         if (q.list_after) {
           if (cls && !set_list(L.box, q.sp, q.p.x, q.p.y, q.p.z, q.nn)) {
@@ -845,7 +853,7 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
   }
   int cls = 0, key = -1, e = ERR_NONE;
   double r = 0.0;
-  if (f.active) get_prob(L, rtot, f.p, f.sp, f.coor, f.nn, cls, r, e, nullptr, &key);
+  if (f.active) get_prob<LA>(L, rtot, f.p, f.sp, f.coor, f.nn, cls, r, e, nullptr, &key);
   if (unfix) {
     int pos[3];
     if (nbuff_of(L, s, sc.nn, pos))
