@@ -164,7 +164,7 @@ __device__ inline int tile_local(const TileGeom &g, int T, const P3 &p) {
 // classes.  Nearly every point is not an MC particle (a few per cent of the
 // stored points are), so the Get_Prob branch is rare and its registers
 // (spilled at four waves per SIMD) cost little.
-template <bool LA>
+template <bool LA, bool COV>
 __device__ inline void sweep_point(const Lattice &L, double rtot, int64_t p, const Site &sp, int atoms_only,
                                    int *err, float &r_out, uint8_t &c_out, bool &keep) {
   const int kind = s_kind(sp), coor = s_coor(sp);
@@ -178,7 +178,7 @@ __device__ inline void sweep_point(const Lattice &L, double rtot, int64_t p, con
     pt.i = p;
     pxyz(L.box, p, pt.x, pt.y, pt.z);
     int e = ERR_NONE;
-    get_prob<LA>(L, rtot, pt, sp, coor, list_of(L, pt, sp), cls, r, e, nullptr, &key);
+    get_prob<LA, COV>(L, rtot, pt, sp, coor, list_of(L, pt, sp), cls, r, e, nullptr, &key);
     if (e) atomicExch(err, e);
   }
   r_out = cls ? (float)r : 0.0f;
@@ -186,7 +186,8 @@ __device__ inline void sweep_point(const Lattice &L, double rtot, int64_t p, con
   if (L.key && !keep) L.key[p] = cls ? key : -1;
 }
 
-template <bool LA>
+// LA, COV: the rate model's laser-annealing and coverage branches are compiled in.
+template <bool LA, bool COV>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_rate_sweep(Lattice L, double rtot, int *err, int atoms_only) {
   const int64_t n = L.box.npts;
@@ -198,7 +199,7 @@ k_rate_sweep(Lattice L, double rtot, int *err, int atoms_only) {
       float r[4];
       uint8_t c[4];
       bool keep[4];
-      for (int k = 0; k < 4; k++) sweep_point<LA>(L, rtot, p0 + k, sp[k], atoms_only, err, r[k], c[k], keep[k]);
+      for (int k = 0; k < 4; k++) sweep_point<LA, COV>(L, rtot, p0 + k, sp[k], atoms_only, err, r[k], c[k], keep[k]);
       if (!(keep[0] || keep[1] || keep[2] || keep[3])) {
         *(float4 *)(L.rate + p0) = make_float4(r[0], r[1], r[2], r[3]);
         *(uchar4 *)(L.cls + p0) = make_uchar4(c[0], c[1], c[2], c[3]);
@@ -214,7 +215,7 @@ k_rate_sweep(Lattice L, double rtot, int *err, int atoms_only) {
         float r;
         uint8_t c;
         bool keep;
-        sweep_point<LA>(L, rtot, q, L.site[q], atoms_only, err, r, c, keep);
+        sweep_point<LA, COV>(L, rtot, q, L.site[q], atoms_only, err, r, c, keep);
         if (!keep) {
           L.rate[q] = r;
           L.cls[q] = c;
@@ -225,7 +226,7 @@ k_rate_sweep(Lattice L, double rtot, int *err, int atoms_only) {
 }
 
 // ---- event-log replay -------------------------------------------------------
-template <bool LA>
+template <bool LA, bool COV>
 __global__ void __launch_bounds__(WAVE) k_replay(Lattice L, const mskg_event *log, int64_t n, int clock_mode,
                                                  RunState *st) {
   __shared__ EvScratch sc;
@@ -250,7 +251,7 @@ __global__ void __launch_bounds__(WAVE) k_replay(Lattice L, const mskg_event *lo
     ch.arm = ev.arm;
     ch.u_arm = 0.0;
     EvResult res;
-    execute<LA>(L, ev.rtot, s, ev.itrans, ch, res, sc, lane);
+    execute<LA, COV>(L, ev.rtot, s, ev.itrans, ch, res, sc, lane);
     if (lane == 0) s_bad = res.err;
     __syncthreads();
     if (s_bad) {
@@ -337,7 +338,7 @@ __global__ void __launch_bounds__(256) k_tile_rates(Lattice L, int ntiles, RunSt
 
 // Two waves per SIMD: the window of one colour pass holds about 2048 tiles
 // at the 512^3 workload, all of them in flight at once on 256 CUs.
-template <bool LA>
+template <bool LA, bool COV>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) k_pass(PassParams P) {
   extern __shared__ unsigned char smem[];
   const int lane = threadIdx.x;
@@ -468,11 +469,11 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
       Choice ch;
       ch.arm = -1;
       ch.u_arm = u01_32(d.w);
-      const int itrans = pick_itrans<LA>(L, s, lc[sel], u01_32(d.z));
+      const int itrans = pick_itrans<LA, COV>(L, s, lc[sel], u01_32(d.z));
       const int coor0 = P.log ? s_coor(L.site[s.i]) : 0;
       __syncthreads();  // every lane has read the site before it changes
       EvResult res;
-      execute<LA>(L, rtot, s, itrans, ch, res, sc, lane);
+      execute<LA, COV>(L, rtot, s, itrans, ch, res, sc, lane);
       __syncthreads();  // neighbour updates by lanes 1..3 are visible to lane 0
       if (lane == 0) {
         err = res.err;
@@ -804,7 +805,7 @@ __device__ int scan_pick(const double *v, int n, double target, int lane, double
   return last_nz;
 }
 
-template <bool LA>
+template <bool LA, bool COV>
 __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
   __shared__ EvScratch sc;
   __shared__ int s_tiles[MAX_TOUCHED], s_ntl, s_err;
@@ -878,11 +879,11 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
     Choice ch;
     ch.arm = -1;
     ch.u_arm = u01_open(d2.z, d2.w);
-    const int itrans = pick_itrans<LA>(L, s, L.cls[s.i], u01_open(d2.x, d2.y));
+    const int itrans = pick_itrans<LA, COV>(L, s, L.cls[s.i], u01_open(d2.x, d2.y));
     const int coor0 = s_coor(L.site[s.i]);
     __syncthreads();
     EvResult res;
-    execute<LA>(L, R, s, itrans, ch, res, sc, lane);
+    execute<LA, COV>(L, R, s, itrans, ch, res, sc, lane);
     __syncthreads();
     clock += dt;
     if (lane == 0) {
@@ -1019,7 +1020,7 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
     ch.arm = ev.arm;
     ch.u_arm = 0.0;
     EvResult res;
-    execute<false>(L, ev.rtot, s, ev.itrans, ch, res, sc, lane);  // the likelihood excludes LA tables
+    execute<false, false>(L, ev.rtot, s, ev.itrans, ch, res, sc, lane);  // the likelihood excludes LA and coverage tables
     __syncthreads();
     if (lane == 0) {
       int bad = res.err;
@@ -1184,6 +1185,13 @@ __global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_
   }
 }
 
+// The kernel instance for a rate model (all four share one signature).
+template <class F>
+F pick_variant(bool la, bool cov, F tt, F tf, F ft, F ff) {
+  return la ? (cov ? tt : tf) : (cov ? ft : ff);
+}
+#define MSKG_VARIANT(K, la, cov) pick_variant((la) != 0, (cov) != 0, K<true, true>, K<true, false>, K<false, true>, K<false, false>)
+
 // ---- host side ---------------------------------------------------------------
 #define HIPCHK(e, x)                                                                  \
   do {                                                                                \
@@ -1195,6 +1203,7 @@ __global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_
 
 struct mskg_engine {
   mskg_config cfg;
+  bool cov_model = true;  // the kernels with the coverage-species branches
   Box box;
   Geometry geo;
   Tables tab;
@@ -1377,6 +1386,8 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   e->tab.ncrystal = cfg->n_crystal;
   e->tab.ncov = cfg->n_cov;
   e->tab.la_mode = cfg->la_mode;
+  e->cov_model = cfg->n_cov > 0;  // coverage species, or absorption / desorption rates in the tables
+  for (int i = 0; i < 12; i++) e->cov_model |= e->tab.pabs[i] != 0.0 || e->tab.pdes[i] != 0.0;
   e->tab.p0 = cfg->p0;
   e->tab.tm = cfg->tm;
   e->tab.sigma = cfg->sigma;
@@ -1560,8 +1571,7 @@ int mskg_refresh_rates(mskg_engine *e) {
   e->lists_dirty = true;
   int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
   HIPCHK(e, hipEventRecord(e->ev0, e->stream));
-  if (e->tab.la_mode) hipLaunchKernelGGL(k_rate_sweep<true>, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
-  else hipLaunchKernelGGL(k_rate_sweep<false>, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
+  hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
   HIPCHK(e, hipGetLastError());
   if (e->track) e->keys_valid = true;
   HIPCHK(e, hipEventRecord(e->ev1, e->stream));
@@ -1589,8 +1599,7 @@ int mskg_replay(mskg_engine *e, const mskg_event *log, int64_t n, int32_t clock_
   Lattice L = make_lattice(e);
   L.inlist = nullptr;
   e->lists_dirty = true;
-  if (e->tab.la_mode) hipLaunchKernelGGL(k_replay<true>, dim3(1), dim3(WAVE), 0, e->stream, L, d_log, n, clock_mode, e->d_st);
-  else hipLaunchKernelGGL(k_replay<false>, dim3(1), dim3(WAVE), 0, e->stream, L, d_log, n, clock_mode, e->d_st);
+  hipLaunchKernelGGL(MSKG_VARIANT(k_replay, e->tab.la_mode, e->cov_model), dim3(1), dim3(WAVE), 0, e->stream, L, d_log, n, clock_mode, e->d_st);
   hipError_t le = hipGetLastError();
   RunState st;
   int rc = read_state(e, st);
@@ -1664,8 +1673,7 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
       P.key0 = (uint32_t)e->cfg.seed;
       P.key1 = (uint32_t)(e->cfg.seed >> 32);
       P.colour = c;
-      if (e->tab.la_mode) hipLaunchKernelGGL(k_pass<true>, dim3(grid), dim3(WAVE), shm, e->stream, P);
-      else hipLaunchKernelGGL(k_pass<false>, dim3(grid), dim3(WAVE), shm, e->stream, P);
+      hipLaunchKernelGGL(MSKG_VARIANT(k_pass, e->tab.la_mode, e->cov_model), dim3(grid), dim3(WAVE), shm, e->stream, P);
     }
     hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err);
     HIPCHK(e, hipStreamEndCapture(e->stream, &e->graph));
@@ -1855,8 +1863,7 @@ static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction
   HIPCHK(e, hipGetLastError());
   if (e->tab.use_f2 || mode == 2) {  // the surface atoms' rates from their neighbours (species, second shell)
     HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
-    if (e->tab.la_mode) hipLaunchKernelGGL(k_rate_sweep<true>, dim3(blocks), dim3(256), 0, e->stream, L, 0.0, e->d_err, 1);
-    else hipLaunchKernelGGL(k_rate_sweep<false>, dim3(blocks), dim3(256), 0, e->stream, L, 0.0, e->d_err, 1);
+    hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, 0.0, e->d_err, 1);
     HIPCHK(e, hipGetLastError());
   }
   // the total rate of the initial population seeds the isolated-atom rule
@@ -1999,11 +2006,12 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
   if (!e0) return MSKG_EINVAL;
   std::vector<SerialDesc> descs(n);
   std::vector<unsigned long long> ev0(n), log0(n);
-  bool any_la = false;
+  bool any_la = false, any_cov = false;
   for (int i = 0; i < n; i++) {
     mskg_engine *e = engines[i];
     if (!e || !e->loaded) return fail_(e0, MSKG_EINVAL, "serial run: replica %d has no state loaded", i);
     any_la |= e->tab.la_mode != 0;
+    any_cov |= e->cov_model;
     if (max_events[i] < 0) return fail_(e, MSKG_EINVAL, "max_events < 0");
     for (int j = 0; j < i; j++)
       if (engines[j] == e) return fail_(e0, MSKG_EINVAL, "serial run: replica %d appears twice", i);
@@ -2049,8 +2057,7 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
   hipError_t ce = hipMemcpyAsync(d_descs, descs.data(), n * sizeof(SerialDesc), hipMemcpyHostToDevice, e0->stream);
   if (ce == hipSuccess) {
     HIPCHK(e0, hipEventRecord(e0->ev0, e0->stream));
-    if (any_la) hipLaunchKernelGGL(k_serial<true>, dim3(n), dim3(WAVE), 0, e0->stream, d_descs);
-    else hipLaunchKernelGGL(k_serial<false>, dim3(n), dim3(WAVE), 0, e0->stream, d_descs);
+    hipLaunchKernelGGL(MSKG_VARIANT(k_serial, any_la, any_cov), dim3(n), dim3(WAVE), 0, e0->stream, d_descs);
     ce = hipGetLastError();
     if (ce == hipSuccess) ce = hipEventRecord(e0->ev1, e0->stream);
     if (ce == hipSuccess) ce = hipStreamSynchronize(e0->stream);
@@ -2139,8 +2146,7 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
     int blocks = (int)std::min<int64_t>((np + 255) / 256, 256 * 64);
     HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
     if (fresh) {
-      if (e->tab.la_mode) hipLaunchKernelGGL(k_rate_sweep<true>, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
-      else hipLaunchKernelGGL(k_rate_sweep<false>, dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
+      hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
       HIPCHK(e, hipGetLastError());
     }
     HIPCHK(e, hipMemsetAsync(d_cnt, 0, (KEY_N + 1) * sizeof(int), e->stream));

@@ -241,7 +241,19 @@ MSKG_HD P3 second_pos(const Lattice &L, const P3 &p, int k) {
   if (y < 0) y += L.box.ly;
   return at(L, x, y, p.z + dz);
 }
-template <bool LA = true>
+// Index step from a point to its second-neighbour position k when neither
+// wraps nor leaves the box: (dz * plane) + (dy / 2) * hx + dx / 2 with the
+// vectors of second_pos.
+MSKG_HD int64_t second_step(const Box &b, int k) {
+  const int a = (k & 2) ? -6 : 6, c = (k & 1) ? -6 : 6, g = k >> 2;
+  const int dx = g == 2 ? 0 : a, dy = g == 0 ? c : (g == 1 ? 0 : a), dz = g == 0 ? 0 : c;
+  return (int64_t)dz * b.plane + (int64_t)(dy / 2) * b.hx + dx / 2;
+}
+// COV = false compiles the coverage-species branches out (tables without
+// coverage species and without absorption or desorption rates).  STEP: the
+// second-shell gather of interior points takes fixed index steps (fewer
+// instructions, more registers: for the event kernels, not the sweep).
+template <bool LA = true, bool COV = true, bool STEP = false>
 MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &cls,
                       double &rate, int &err, const Ovr *ovr = nullptr, int *key = nullptr) {
   const Tables &t = *L.tab;
@@ -254,7 +266,8 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
     temperature = (int)sp.temp;
     if (temperature == 0) { err = ERR_TEMPERATURE; cls = CLS_NONE; rate = 0.0; return; }
   }
-  int slot = kind_cov_slot(kind);
+  if (!COV && kind >= KIND_COV0) { err = ERR_COVERAGE; cls = CLS_NONE; rate = 0.0; return; }
+  int slot = COV ? kind_cov_slot(kind) : 0;
   if (slot) {  // desorption, :93-123
     if (slot > t.ncov) { err = ERR_COVERAGE; cls = CLS_NONE; rate = 0.0; return; }
     if (coor <= 3) { cls = CLS_DESORB; rate = t.pdes[(slot - 1) + 3 * coor]; *key = KEY_UNSUPPORTED; }
@@ -268,7 +281,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
       if (ok) {
         double pdep = pd_of<LA>(t, 1, coor, temperature) + pd_of<LA>(t, 2, coor, temperature) +
                       pd_of<LA>(t, 3, coor, temperature);
-        double pabs = t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor];
+        double pabs = COV ? t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor] : 0.0;
         cls = CLS_ADSORB;
         rate = pdep + pabs;
         *key = t.ncov ? KEY_UNSUPPORTED : KEY_ADS + coor - 1;
@@ -295,11 +308,16 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   uint8_t kc2[12];
   for (int i = 0; i < 4; i++) qs[i] = entry(L, nn, i);
   for (int i = 0; i < 4; i++) sqs[i] = rd_o(L, qs[i].i, ovr);  // four independent gathers in flight
-  if (t.use_f2)  // ... and the twelve second-shell kinds with them
+  if (t.use_f2) {  // ... and the twelve second-shell kinds with them
+    // at least 6 units from every face the shell neither wraps nor leaves
+    // the box, and its indices are p's plus fixed steps
+    const bool inner = STEP && p.x >= 6 && p.x < L.box.lx - 6 && p.y >= 6 && p.y < L.box.ly - 6 && p.z >= 6 &&
+                       p.z <= L.box.lz - 6;
     for (int k = 0; k < 12; k++) {
-      const int64_t i2 = second_pos(L, p, k).i;
+      const int64_t i2 = inner ? p.i + second_step(L.box, k) : second_pos(L, p, k).i;
       kc2[k] = i2 < 0 ? (uint8_t)0 : (ovr ? rd_o(L, i2, ovr).kc : L.site[i2].kc);
     }
+  }
   for (int i = 0; i < 4; i++) {
     const P3 &q = qs[i];
     const Site &sq = sqs[i];
@@ -314,7 +332,7 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
       } else {
         n_wall++;
       }
-    } else if (kind_cov_slot(kq)) {
+    } else if (COV && kind_cov_slot(kq)) {
       if (lists_point(L.box, q, sq, p)) {
         int s = kind_cov_slot(kq);
         if (s > t.ncov) { err = ERR_COVERAGE; cls = CLS_NONE; rate = 0.0; return; }
@@ -728,7 +746,7 @@ __device__ inline void cov_prepare(const Lattice &L, const P3 &s, int i, bool ab
 // execution instead of at rate-update time (same distribution: the class
 // is refreshed whenever the coordination changes).  Returns itrans in the
 // reference's numbering (1..3 deposit, 4 evaporate, 5 desorb, 6.. absorb).
-template <bool LA = true>
+template <bool LA = true, bool COV = true>
 MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   const Tables &t = *L.tab;
   if (cls == CLS_PLACEHOLDER) return 1;
@@ -740,9 +758,9 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   double p1 = pd_of<LA>(t, 1, coor, temperature), p2 = pd_of<LA>(t, 2, coor, temperature),
          p3 = pd_of<LA>(t, 3, coor, temperature);
   double pdep = p1 + p2 + p3;
-  double pabs = t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor];
+  double pabs = COV ? t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor] : 0.0;
   double pcurr = u * (pdep + pabs);
-  if (pcurr <= pdep) {
+  if (!COV || pcurr <= pdep) {
     if (pcurr <= p1) return 1;
     if (pcurr > p1 + p2) return 3;
     return 2;
@@ -765,7 +783,7 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
 // second-neighbour term, lanes 1..12 refresh the atoms of its second shell.
 // Each phase has one rate-evaluation call site for every event type, which
 // keeps the inlined kernel small.
-template <bool LA = true>
+template <bool LA = true, bool COV = true>
 __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, const Choice &ch, EvResult &res,
                         EvScratch &sc, int lane) {
   res.err = ERR_NONE;
@@ -781,7 +799,7 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
     sc.nvisit = 4;
     if (dep) dep_site(L, s, itrans, ch, res, sc);
     else if (evap) evap_site(L, s, sc);
-    else cov_site(L, s, absorb ? itrans - 5 : 0, sc);
+    else if (COV) cov_site(L, s, absorb ? itrans - 5 : 0, sc);
   }
   MSKG_SYNC();
   if (sc.abort) {
@@ -799,11 +817,11 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
       q.need = q.list_after = false;
       if (dep) dep_prepare(L, s, i, sc, sc.err[i], q);
       else if (evap) evap_prepare(L, s, i, sc, sc.err[i], q);
-      else cov_prepare(L, s, i, absorb, sc, q);
+      else if (COV) cov_prepare(L, s, i, absorb, sc, q);
       if (q.need) {
         int cls = 0, key = -1;
         double r = 0.0;
-        get_prob<LA>(L, rtot, q.p, q.sp, q.coor, q.nn, cls, r, sc.err[i], nullptr, &key);
+        get_prob<LA, COV, true>(L, rtot, q.p, q.sp, q.coor, q.nn, cls, r, sc.err[i], nullptr, &key);
         bool ok = true;
         if (q.list_after) {
           if (cls && !set_list(L.box, q.sp, q.p.x, q.p.y, q.p.z, q.nn)) {
@@ -853,7 +871,7 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
   }
   int cls = 0, key = -1, e = ERR_NONE;
   double r = 0.0;
-  if (f.active) get_prob<LA>(L, rtot, f.p, f.sp, f.coor, f.nn, cls, r, e, nullptr, &key);
+  if (f.active) get_prob<LA, COV, true>(L, rtot, f.p, f.sp, f.coor, f.nn, cls, r, e, nullptr, &key);
   if (unfix) {
     int pos[3];
     if (nbuff_of(L, s, sc.nn, pos))
