@@ -202,11 +202,44 @@ def adjoint_leg(n_traj, n_events, segment):
                         "recorded by the serial kernel, segments replayed in one launch"}
 
 
+def cpu_baseline(tables, substrate_len, seconds):
+    """The oracle (C restatement of the reference engine, serial, one core) on
+    the same model and substrate at a 480 x 480 x 240 grid, for about
+    `seconds` of wall time after its initialisation."""
+    import tempfile
+
+    from mulskips_amd import deck as deckmod
+    from oracle.oracle import Oracle
+    box = (480, 480, 240)
+    with tempfile.TemporaryDirectory() as tmp:
+        d = deckmod.deck_from_tables(tables, "F", [substrate_len, 120, 120])
+        path = os.path.join(tmp, "start.dat")
+        deckmod.write_start_dat(d, path)
+        o = Oracle(*box)
+        o.load_deck(path)
+        o.set_frozen(substrate_len + 1)
+        if tables.get("second_factor") is not None:
+            o.set_second_factor(tables["second_factor"])
+        o.step(20000, record=False)  # leave the initial transient
+        n = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            k = len(o.step(100000, record=False))
+            n += k
+            if k < 100000:
+                break
+        dt = time.perf_counter() - t0
+        o.close()
+    return {"value": n / dt, "unit": "events/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (reference engine restated in C, serial BKL, no 98 KB table copy), same SiC model "
+                      f"and frozen 40-ML substrate on a 480x480x240 grid, {n} events in {dt:.1f} s"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported (this engine is a single-GPU build; "
-              "see DESIGN.md for how the lattice would shard)", file=sys.stderr)
+        print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported by this single-GPU build",
+              file=sys.stderr)
         return 2
     import torch  # device selection and synchronisation only
 
