@@ -42,9 +42,13 @@ constexpr int NSLOT = 128;
 constexpr int STOP_AT_LIMIT = 100;
 struct Slot {
   unsigned long long rmax;  // bit pattern of the largest (non-negative) tile rate
-  double rsum;
   unsigned long long n, tiles, events, counts[8];
 };
+// The FP64 total rate is summed without atomics, in a fixed order: every
+// tile keeps its total (Lattice::tile_r), NPART blocks each sum a fixed
+// chunk of tiles, and k_sweep_end sums the NPART partials.  A rerun with
+// the same seed therefore repeats the same windows.
+constexpr int NPART = 2 * NSLOT;
 
 // Device-resident run state (one per engine).
 struct RunState {
@@ -63,6 +67,7 @@ struct RunState {
   double max_time;                  // the run's limits: the sweep that reaches one stops the run
   unsigned long long max_events;
   Slot slot[NSLOT];
+  double rpart[NPART];  // partial sums of the tile totals (k_rsum)
   int err;
   int err_site[3];
   int stop;
@@ -99,7 +104,6 @@ __device__ inline void slot_add(RunState *st, int b, double r, unsigned long lon
   if (n) {
     const unsigned long long rb = (unsigned long long)__double_as_longlong(r);
     if (rb > sl.rmax) atomicMax(&sl.rmax, rb);
-    atomicAdd(&sl.rsum, r);
     atomicAdd(&sl.n, n);
     atomicAdd(&sl.tiles, 1ull);
   }
@@ -302,6 +306,7 @@ __global__ void __launch_bounds__(256) k_build_lists(Lattice L, int ntiles) {
   }
   if (lane == 0) {
     L.tl_n[t] = n;
+    L.tl_own[t] = min(n, L.capg);
     if (n) widen_window(L, tile_colour(g.tx, g.ty, g.tz), g.tz);
   }
 }
@@ -333,11 +338,64 @@ __global__ void __launch_bounds__(256) k_tile_rates(Lattice L, int ntiles, RunSt
   }
   acc = wave_sum(acc);
   for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, WAVE);
-  if (lane == 0 && cnt) slot_add(st, t, acc, (unsigned long long)cnt, 0, nullptr);
+  if (lane == 0) {
+    L.tile_r[t] = cnt ? acc : 0.0;
+    if (cnt) slot_add(st, t, acc, (unsigned long long)cnt, 0, nullptr);
+  }
 }
 
 // Two waves per SIMD: the window of one colour pass holds about 2048 tiles
 // at the 512^3 workload, all of them in flight at once on 256 CUs.
+// Sorts lk/lr/lc[lo, hi) by offset (the offsets are distinct): a bitonic
+// network across the lanes for up to 64 entries, odd-even transposition
+// rounds beyond (rare: appends from other tiles between two passes).
+__device__ inline void sort_by_offset(int32_t *lk, float *lr, uint8_t *lc, int lo, int hi, int lane) {
+  const int m = hi - lo;
+  if (m <= 1) return;
+  if (m <= WAVE) {
+    int key = lane < m ? lk[lo + lane] : 0x7fffffff;
+    int idx = lane;
+    for (int k = 2; k <= WAVE; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int ok = __shfl_xor(key, j, WAVE), oi = __shfl_xor(idx, j, WAVE);
+        const bool keep_min = ((lane & k) == 0) == ((lane & j) == 0);
+        if (keep_min ? ok < key : ok > key) {
+          key = ok;
+          idx = oi;
+        }
+      }
+    float r = 0.0f;
+    uint8_t c = 0;
+    if (lane < m) {
+      r = lr[lo + idx];
+      c = lc[lo + idx];
+    }
+    __syncthreads();
+    if (lane < m) {
+      lk[lo + lane] = key;
+      lr[lo + lane] = r;
+      lc[lo + lane] = c;
+    }
+    __syncthreads();
+    return;
+  }
+  for (int round = 0; round < m; round++) {
+    for (int i = lo + 2 * lane + (round & 1); i + 1 < hi; i += 2 * WAVE)
+      if (lk[i] > lk[i + 1]) {
+        const int tk = lk[i];
+        lk[i] = lk[i + 1];
+        lk[i + 1] = tk;
+        const float tr = lr[i];
+        lr[i] = lr[i + 1];
+        lr[i + 1] = tr;
+        const uint8_t tc = lc[i];
+        lc[i] = lc[i + 1];
+        lc[i + 1] = tc;
+      }
+    __syncthreads();
+  }
+}
+
 template <bool LA, bool COV>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) k_pass(PassParams P) {
   extern __shared__ unsigned char smem[];
@@ -375,9 +433,12 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   L.cur_tile = t;
   TileGeom g = tile_geom(t, L.tile, L.ntx, L.nty, L.box.lz);
 
-  // 1. the tile's candidate list -> LDS, dropping entries no longer active
+  // 1. the tile's candidate list -> LDS, dropping entries no longer active;
+  // the entries other tiles appended since the tile's last pass (an order
+  // that depends on timing) are then sorted by offset
   const int m_in = min(n_in, L.capg);
-  int n = 0;
+  const int own = L.tl_own[t];
+  int n = 0, nh = 0;
   for (int j0 = 0; j0 < m_in; j0 += WAVE) {
     int j = j0 + lane;
     uint8_t c = 0;
@@ -398,6 +459,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
       lc[slot] = c;
     }
     n += __popcll(m);
+    nh += __popcll(__ballot(c != 0 && j < own));
   }
   if (n > cap) {
     if (lane == 0) {
@@ -408,6 +470,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
     return;
   }
   __syncthreads();
+  sort_by_offset(lk, lr, lc, nh, n, lane);
 
   // 2. rejection-free KMC inside the tile for the window tau
   const double rtot = st->rtot;
@@ -576,6 +639,8 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   const double R = wave_sum(part);
   if (lane == 0) {
     L.tl_n[t] = nout;
+    L.tl_own[t] = nout;
+    L.tile_r[t] = R;
     slot_add(st, t, R, (unsigned long long)nout, nev, s_cnt);
     if (nout) {
       if (g.tz < L.win[4 * P.colour + 2]) atomic_min_i32(&L.win[4 * P.colour + 2], g.tz);
@@ -592,6 +657,23 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   }  // window loop
 }
 
+// The fixed-order sum of the tile totals: block b sums its chunk of tiles
+// (each thread a fixed stride, then a tree in LDS) into rpart[b].
+__global__ void __launch_bounds__(256) k_rsum(const double *tile_r, int ntiles, RunState *st) {
+  __shared__ double s[256];
+  const int chunk = (ntiles + NPART - 1) / NPART;
+  const int lo = blockIdx.x * chunk, hi = min(lo + chunk, ntiles);
+  double acc = 0.0;
+  for (int t = lo + (int)threadIdx.x; t < hi; t += 256) acc += tile_r[t];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) st->rpart[blockIdx.x] = s[0];
+}
+
 // After the eight colour passes: the clock advances by the window just used
 // (operator splitting of the master equation over the colours), the next
 // window is events_per_pass / (largest tile rate), and the reference's
@@ -601,9 +683,10 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
   __shared__ unsigned long long srmax[NSLOT], sn[NSLOT], stl[NSLOT], sev[NSLOT], scnt[8][NSLOT];
   const int i = threadIdx.x;
   Slot &sl = st->slot[i];
-  srsum[i] = sl.rsum; srmax[i] = sl.rmax; sn[i] = sl.n; stl[i] = sl.tiles; sev[i] = sl.events;
+  srsum[i] = st->rpart[i] + st->rpart[i + NSLOT];
+  srmax[i] = sl.rmax; sn[i] = sl.n; stl[i] = sl.tiles; sev[i] = sl.events;
   for (int q = 0; q < 8; q++) scnt[q][i] = sl.counts[q];
-  sl.rsum = 0.0; sl.rmax = 0ull; sl.n = 0ull; sl.tiles = 0ull; sl.events = 0ull;
+  sl.rmax = 0ull; sl.n = 0ull; sl.tiles = 0ull; sl.events = 0ull;
   for (int q = 0; q < 8; q++) sl.counts[q] = 0ull;
   __syncthreads();
   for (int off = NSLOT / 2; off > 0; off >>= 1) {
@@ -1217,6 +1300,8 @@ struct mskg_engine {
   uint8_t *d_inlist = nullptr;
   int32_t *d_tl_idx = nullptr;
   int32_t *d_tl_n = nullptr;
+  int32_t *d_tl_own = nullptr;
+  double *d_tile_r = nullptr;
   int32_t *d_tile_n = nullptr;
   int *d_list_err = nullptr;
   int *d_win = nullptr;
@@ -1277,6 +1362,8 @@ static Lattice make_lattice(mskg_engine *e) {
   L.inlist = e->d_inlist;
   L.tl_idx = e->d_tl_idx;
   L.tl_n = e->d_tl_n;
+  L.tl_own = e->d_tl_own;
+  L.tile_r = e->d_tile_r;
   L.list_err = e->d_list_err;
   L.win = e->d_win;
   L.capg = e->capg;
@@ -1427,6 +1514,8 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   HIPCHK(e, hipMalloc(&e->d_inlist, n));
   HIPCHK(e, hipMalloc(&e->d_tl_idx, (size_t)e->ntiles * e->capg * sizeof(int32_t)));
   HIPCHK(e, hipMalloc(&e->d_tl_n, e->ntiles * sizeof(int32_t)));
+  HIPCHK(e, hipMalloc(&e->d_tl_own, e->ntiles * sizeof(int32_t)));
+  HIPCHK(e, hipMalloc(&e->d_tile_r, e->ntiles * sizeof(double)));
   HIPCHK(e, hipMalloc(&e->d_list_err, sizeof(int)));
   HIPCHK(e, hipMalloc(&e->d_win, 32 * sizeof(int)));
   HIPCHK(e, hipMalloc(&e->d_geo, sizeof(Geometry)));
@@ -1440,6 +1529,8 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   HIPCHK(e, hipMemsetAsync(e->d_cls, 0, n, e->stream));
   HIPCHK(e, hipMemsetAsync(e->d_inlist, 0, n, e->stream));
   HIPCHK(e, hipMemsetAsync(e->d_tl_n, 0, e->ntiles * sizeof(int32_t), e->stream));
+  HIPCHK(e, hipMemsetAsync(e->d_tl_own, 0, e->ntiles * sizeof(int32_t), e->stream));
+  HIPCHK(e, hipMemsetAsync(e->d_tile_r, 0, e->ntiles * sizeof(double), e->stream));
   HIPCHK(e, hipMemsetAsync(e->d_list_err, 0, sizeof(int), e->stream));
   RunState st;
   memset(&st, 0, sizeof st);
@@ -1454,7 +1545,7 @@ void mskg_destroy(mskg_engine *e) {
   hipFree(e->d_site); hipFree(e->d_rate); hipFree(e->d_cls);
   if (e->graph_exec) hipGraphExecDestroy(e->graph_exec);
   if (e->graph) hipGraphDestroy(e->graph);
-  hipFree(e->d_inlist); hipFree(e->d_tl_idx); hipFree(e->d_tl_n); hipFree(e->d_list_err); hipFree(e->d_win);
+  hipFree(e->d_inlist); hipFree(e->d_tl_idx); hipFree(e->d_tl_n); hipFree(e->d_tl_own); hipFree(e->d_tile_r); hipFree(e->d_list_err); hipFree(e->d_win);
   hipFree(e->d_geo); hipFree(e->d_tab); hipFree(e->d_st); hipFree(e->d_err); hipFree(e->d_log);
   hipFree(e->d_tsum); hipFree(e->d_bsum); hipFree(e->d_key); hipFree(e->d_key_seen); hipFree(e->d_rate_seen);
   if (e->ev0) hipEventDestroy(e->ev0);
@@ -1654,6 +1745,7 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
   // tile totals of the current rates, then the first window
   hipLaunchKernelGGL(k_tile_rates, dim3(tblocks), dim3(256), 0, e->stream, L, e->ntiles, e->d_st);
   HIPCHK(e, hipGetLastError());
+  hipLaunchKernelGGL(k_rsum, dim3(NPART), dim3(256), 0, e->stream, e->d_tile_r, e->ntiles, e->d_st);
   hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err);
   HIPCHK(e, hipGetLastError());
   const size_t shm = (size_t)e->cap * (4 + 4 + 1) + 16;
@@ -1675,6 +1767,7 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
       P.colour = c;
       hipLaunchKernelGGL(MSKG_VARIANT(k_pass, e->tab.la_mode, e->cov_model), dim3(grid), dim3(WAVE), shm, e->stream, P);
     }
+    hipLaunchKernelGGL(k_rsum, dim3(NPART), dim3(256), 0, e->stream, e->d_tile_r, e->ntiles, e->d_st);
     hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err);
     HIPCHK(e, hipStreamEndCapture(e->stream, &e->graph));
     HIPCHK(e, hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));

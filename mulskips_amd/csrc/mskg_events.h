@@ -36,6 +36,8 @@ struct Lattice {
   uint8_t *inlist;        // may be null (no list maintenance)
   int32_t *tl_idx;        // [ntiles][capg] tile-local offsets
   int32_t *tl_n;          // [ntiles] entries used
+  int32_t *tl_own;        // [ntiles] entries the tile itself wrote (the rest were appended by others)
+  double *tile_r;         // [ntiles] FP64 rate total of the tile's list (isolated atoms excluded)
   int *list_err;          // set on overflow
   int *win;               // [8][4] per colour: tile-layer window lo, hi (read by that colour's
                           // pass), next lo, next hi (collected for the next sweep)

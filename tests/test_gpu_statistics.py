@@ -13,10 +13,8 @@ difference:
   regression deck (PtransZig 0.8) on a 144^3 grid;
 * growth and coverage: the CVD-Si-SiH2 regression deck (H coverage).
 Both window sizes the engine is run with are covered: about 1 and 4 events
-per tile and colour pass (tau policy 1).  The seeds are fixed; the serial
-side is deterministic, the engine's window tau comes from FP64 rate sums
-accumulated in arrival order, so a rerun can move a rare event across a
-window edge.
+per tile and colour pass (tau policy 1).  The seeds are fixed and both
+sides repeat a run bit for bit, so the test is deterministic.
 
 A case compares several observables, and the six cases hold about 24
 comparisons, so a correct engine puts one of them outside 2 standard
