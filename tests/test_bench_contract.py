@@ -20,6 +20,8 @@ def test_cpu_baseline_leg():
     sys.path.insert(0, ROOT)
     import bench
     from mulskips_amd import models
-    out = bench.cpu_baseline(models.sic_c2_tables(), 120, 0.2)
+    out = bench.cpu_baseline(models.sic_c2_tables(), 120, 0.2, cores=2)
     assert out["kind"] == "port" and out["cores"] == 1 and out["unit"] == "events/s"
     assert out["value"] > 0 and "events in" in out["sample"]
+    agg = out["aggregate"]
+    assert agg["cores"] == 2 and agg["value"] > 0
