@@ -888,26 +888,35 @@ __device__ int scan_pick(const double *v, int n, double target, int lane, double
   return last_nz;
 }
 
+// Replicas of up to SERIAL_LDS_TILES tiles keep their tile and block sums on
+// chip; larger ones use the engine's global arrays.
+constexpr int SERIAL_LDS_TILES = 4096;
 template <bool LA, bool COV>
 __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
   __shared__ EvScratch sc;
   __shared__ int s_tiles[MAX_TOUCHED], s_ntl, s_err;
+  __shared__ double s_tsum[SERIAL_LDS_TILES], s_bsum[SERIAL_LDS_TILES / WAVE];
+  __shared__ unsigned long long s_cnt[8];
   const int lane = threadIdx.x;
   const SerialDesc D = descs[blockIdx.x];
   Lattice L = D.L;
   L.cur_tile = -1;
   RunState *st = D.st;
   if (st->err) return;
+  const bool on_chip = D.ntiles <= SERIAL_LDS_TILES;
+  double *tsum = on_chip ? s_tsum : D.tsum;
+  double *bsum = on_chip ? s_bsum : D.bsum;
+  if (lane < 8) s_cnt[lane] = 0;
   // initial sums
   for (int t = 0; t < D.ntiles; t++) {
     const double ts = tile_sum_compact(L, t, lane);
-    if (lane == 0) D.tsum[t] = ts;
+    if (lane == 0) tsum[t] = ts;
   }
   __syncthreads();
   for (int b = 0; b < D.nblk; b++) {
     const int t = b * WAVE + lane;
-    const double v = wave_sum(t < D.ntiles ? D.tsum[t] : 0.0);
-    if (lane == 0) D.bsum[b] = v;
+    const double v = wave_sum(t < D.ntiles ? tsum[t] : 0.0);
+    if (lane == 0) bsum[b] = v;
   }
   __syncthreads();
   double clock = st->clock;
@@ -917,21 +926,23 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
   ctr.x = 0x5e71a1u;  // the stream depends on the engine's seed only, not on its place in the batch
   while ((long long)nev < D.max_events && clock < D.max_time) {
     double part = 0.0;
-    for (int b = lane; b < D.nblk; b += WAVE) part += D.bsum[b];
+    for (int b = lane; b < D.nblk; b += WAVE) part += bsum[b];
     const double R = wave_sum(part);
     if (!(R > 0.0)) { stop = MSKG_STOP_NO_EVENTS; break; }
     ctr.y = (uint32_t)events;
     ctr.z = (uint32_t)(events >> 32);
     ctr.w = 0;
+    // one Philox block per event: waiting time, selection, event type and
+    // stacking choice, 32 random bits each
     const U4 d = philox4x32_10(ctr, D.key0, D.key1);
-    const double dt = -log(u01_open(d.x, d.y)) / R;
-    double target = u01_open(d.z, d.w) * R, below = 0.0;
+    const double dt = -log(u01_32(d.x)) / R;
+    double target = u01_32(d.y) * R, below = 0.0;
     // block -> tile -> list entry
-    const int b = scan_pick(D.bsum, D.nblk, target, lane, &below);
+    const int b = scan_pick(bsum, D.nblk, target, lane, &below);
     if (b < 0) { stop = MSKG_STOP_NO_EVENTS; break; }
     target -= below;
     const int t0 = b * WAVE;
-    const int tl = scan_pick(D.tsum + t0, min(WAVE, D.ntiles - t0), target, lane, &below);
+    const int tl = scan_pick(tsum + t0, min(WAVE, D.ntiles - t0), target, lane, &below);
     if (tl < 0) { err = ERR_SUMS; break; }
     const int t = t0 + tl;
     target -= below;
@@ -957,12 +968,10 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
     if (sel < 0) sel = last_nz;
     if (sel < 0) { err = ERR_SUMS; break; }
     const P3 s = tile_point(L.box, g, L.tile, idx[sel]);
-    ctr.w = 1;
-    const U4 d2 = philox4x32_10(ctr, D.key0, D.key1);
     Choice ch;
     ch.arm = -1;
-    ch.u_arm = u01_open(d2.z, d2.w);
-    const int itrans = pick_itrans<LA, COV>(L, s, L.cls[s.i], u01_open(d2.x, d2.y));
+    ch.u_arm = u01_32(d.w);
+    const int itrans = pick_itrans<LA, COV>(L, s, L.cls[s.i], u01_32(d.z));
     const int coor0 = s_coor(L.site[s.i]);
     __syncthreads();
     EvResult res;
@@ -986,7 +995,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
         }
       }
       const int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
-      if (slot >= 0 && slot < 8) st->counts[slot]++;
+      if (slot >= 0 && slot < 8) s_cnt[slot]++;
       int nt = 0;
       for (int q = 0; q < sc.ntouched; q++) {
         if (sc.touched[q].i < 0) continue;
@@ -1004,7 +1013,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
     const int ntl = s_ntl;
     for (int q = 0; q < ntl; q++) {
       const double ts = tile_sum_compact(L, s_tiles[q], lane);
-      if (lane == 0) D.tsum[s_tiles[q]] = ts;
+      if (lane == 0) tsum[s_tiles[q]] = ts;
     }
     __syncthreads();
     for (int q = 0; q < ntl; q++) {
@@ -1013,8 +1022,8 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
       for (int r = 0; r < q; r++) dup |= s_tiles[r] / WAVE == bq;
       if (dup) continue;
       const int tt = bq * WAVE + lane;
-      const double v = wave_sum(tt < D.ntiles ? D.tsum[tt] : 0.0);
-      if (lane == 0) D.bsum[bq] = v;
+      const double v = wave_sum(tt < D.ntiles ? tsum[tt] : 0.0);
+      if (lane == 0) bsum[bq] = v;
     }
     __syncthreads();
   }
@@ -1023,6 +1032,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
     st->events = events;
     st->log_n += nev;
     st->stop = stop;
+    for (int q = 0; q < 8; q++) st->counts[q] += s_cnt[q];
     if (err) st->err = err;
   }
 }
