@@ -1071,6 +1071,8 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
   int *cnt = (int *)(tau + KEY_N);        // [KEY_N]
   __shared__ EvScratch sc;
   __shared__ int s_bad;
+  __shared__ int s_o[MAX_TOUCHED], s_nw[MAX_TOUCHED];
+  __shared__ float s_ro[MAX_TOUCHED], s_rn[MAX_TOUCHED];
   const int lane = threadIdx.x;
   const AdjDesc D = descs[blockIdx.x];
   Lattice L = D.L;
@@ -1085,9 +1087,20 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
   double t = 0.0, ttot = 0.0, r_iso = D.iso0, tau_iso = 0.0, iso_int = 0.0, iso_log = 0.0;
   int err = 0;
   long long k = 0;
+  // the log is read one event ahead: its loads overlap the event before
+  mskg_event ev_next;
+  double dt_next = 0.0;
+  if (D.n > 0) {
+    ev_next = D.log[0];
+    dt_next = D.dt[0];
+  }
   for (; k < D.n; k++) {
-    const mskg_event ev = D.log[k];
-    const double dtk = D.dt[k];
+    const mskg_event ev = ev_next;
+    const double dtk = dt_next;
+    if (k + 1 < D.n) {
+      ev_next = D.log[k + 1];
+      dt_next = D.dt[k + 1];
+    }
     t += dtk;
     ttot += dtk;
     const P3 s = at(L, ev.site[0], ev.site[1], ev.site[2]);
@@ -1098,12 +1111,12 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
       else if (ks < 0) bad = ERR_COVERAGE;  // a rate class the likelihood does not cover
       else if (ks >= KEY_ADS && ks < KEY_PH) {
         if (ev.itrans < 1 || ev.itrans > 3) bad = ERR_NOT_MC;
-        else D.S[KEY_N + (ev.itrans - 1) * 3 + (ks - KEY_ADS)] += 1.0;
+        else atomicAdd(&D.S[KEY_N + (ev.itrans - 1) * 3 + (ks - KEY_ADS)], 1.0);
       } else if (ks == KEY_ISO) {
         iso_log += log((double)D.rate_seen[s.i]);
-        D.S[KEY_ISO] += 1.0;
+        atomicAdd(&D.S[KEY_ISO], 1.0);
       } else {
-        D.S[ks] += 1.0;
+        atomicAdd(&D.S[ks], 1.0);  // no return value: the wave does not wait for it
       }
       s_bad = bad;
     }
@@ -1115,27 +1128,48 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
     EvResult res;
     execute<false, false>(L, ev.rtot, s, ev.itrans, ch, res, sc, lane);  // the likelihood excludes LA and coverage tables
     __syncthreads();
+    // the touched points' keys and rates, before and after, one lane each
+    // (a point listed twice is taken once); every class change of the event
+    // happens at time t, so the accounting below does not depend on order
+    const int nt = sc.ntouched;
+    if (lane < nt) {
+      const int64_t p = sc.touched[lane].i;
+      bool first = p >= 0;
+      for (int q = 0; q < lane; q++)
+        if (sc.touched[q].i == p) first = false;
+      int o = -1, nw = -1;
+      float ro = 0.0f, rn = 0.0f;
+      if (first) {
+        o = D.key_seen[p];
+        nw = L.key[p];
+        ro = D.rate_seen[p];
+        rn = L.rate[p];
+        D.key_seen[p] = nw;
+        D.rate_seen[p] = rn;
+      }
+      s_o[lane] = first ? o : -3;
+      s_nw[lane] = nw;
+      s_ro[lane] = ro;
+      s_rn[lane] = rn;
+    }
+    __syncthreads();
     if (lane == 0) {
       int bad = res.err;
-      for (int q = 0; q < sc.ntouched && !bad; q++) {
-        const int64_t p = sc.touched[q].i;
-        if (p < 0) continue;
-        const int o = D.key_seen[p], nw = L.key[p];
+      for (int q = 0; q < nt && !bad; q++) {
+        const int o = s_o[q], nw = s_nw[q];
+        if (o == -3) continue;
         if (nw == KEY_UNSUPPORTED) { bad = ERR_COVERAGE; break; }
         if (o != nw) {
           if (o >= 0 && o != KEY_ISO) { A[o] += cnt[o] * (t - tau[o]); tau[o] = t; cnt[o]--; }
           if (nw >= 0 && nw != KEY_ISO) { A[nw] += cnt[nw] * (t - tau[nw]); tau[nw] = t; cnt[nw]++; }
         }
-        const float rnew = L.rate[p];
-        const double ro = o == KEY_ISO ? (double)D.rate_seen[p] : 0.0;
-        const double rn = nw == KEY_ISO ? (double)rnew : 0.0;
+        const double ro = o == KEY_ISO ? (double)s_ro[q] : 0.0;
+        const double rn = nw == KEY_ISO ? (double)s_rn[q] : 0.0;
         if (ro != rn) {
           iso_int += r_iso * (t - tau_iso);
           tau_iso = t;
           r_iso += rn - ro;
         }
-        D.key_seen[p] = nw;
-        D.rate_seen[p] = rnew;
       }
       s_bad = bad;
     }
