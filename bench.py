@@ -38,6 +38,19 @@ SWEEP_BYTES_PER_POINT = 21
 PMC_PROFILE = os.path.join(ROOT, "profiles", "pmc_rate_sweep_sic512.md")
 
 
+def measured_pass_traffic_per_event():
+    """HBM bytes per event of the pass kernel from the committed counter
+    profile (FETCH_SIZE + WRITE_SIZE over a run's dispatches / its events)."""
+    try:
+        with open(PMC_PROFILE) as f:
+            for line in f:
+                if line.startswith("pass_traffic_bytes_per_event ="):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic_per_point():
     """HBM bytes per lattice point of the sweep kernel from the committed
     counter profile (FETCH_SIZE + WRITE_SIZE, calibrated against a copy)."""
@@ -296,6 +309,7 @@ def main():
     npts = tm["sweep_points"]
     achieved = npts * SWEEP_BYTES_PER_POINT / (sweep_ms * 1e-3) / 1e9
     tpp = measured_traffic_per_point()
+    bpe = measured_pass_traffic_per_event()
     eng.close()
     probe = stream_probe()
     out = {
@@ -324,6 +338,16 @@ def main():
                                        "per point, times the points of this launch)",
                      "points": npts, "bytes_per_point": SWEEP_BYTES_PER_POINT, "ms": sweep_ms,
                      "stream_probe": probe, "frac_of_triad": achieved / probe["triad_GBps"]},
+        # the kernel with most of the device time: bound by the latency of its
+        # dependent per-event work; its measured HBM traffic for scale
+        "pass_kernel": {"kernel": "k_pass (synchronous-sublattice colour pass)", "bound": "latency",
+                        "device_ms": dev_ms, "events_per_s_device": total / (dev_ms * 1e-3),
+                        "traffic_bytes_per_event": bpe,
+                        "achieved": None if bpe is None else bpe * total / (dev_ms * 1e-3) / 1e9,
+                        "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": None if bpe is None else bpe * total / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                        "traffic_source": "profiles/pmc_rate_sweep_sic512.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE "
+                                          "over a run's k_pass dispatches, per event)"},
         "engine": {"device_ms": dev_ms, "events": total, "kmc_time_s": t_kmc,
                    "counts": [int(c) for c in counts]},
     }
