@@ -22,7 +22,7 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
     # every device function inlined into its kernel: no call ABI, whose
     # callee-saved registers pin the event kernels at 256 VGPRs and one wave
-    # per SIMD; with it inlined the pass kernel fits two waves per SIMD
+    # per SIMD; inlined, the pass kernel runs three waves per SIMD
     "-mllvm", "-amdgpu-early-inline-all=true", "-mllvm", "-amdgpu-function-calls=false",
 ]
 

@@ -13,7 +13,13 @@
 //                  one tile for the window tau, selecting events with a
 //                  wavefront prefix scan over the tile's FP32 rates held in
 //                  LDS, FP64 partial sums and Philox-4x32-10 draws.
-//   k_sweep_end    FP64 reductions of tile rates -> next tau and the clock.
+//   k_rsum         fixed-order FP64 sum of the tile rate totals;
+//   k_sweep_end    -> next tau and the clock.
+//   k_serial       serial BKL over a batch of lattices (one wave each).
+//   k_adjoint      log replay with per-class likelihood statistics.
+//   k_build_slab, k_frame_count / k_frame_write: device builder, frame records.
+// The event kernels are templates on the rate model (laser annealing,
+// coverage species), chosen at launch from the tables.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -344,8 +350,6 @@ __global__ void __launch_bounds__(256) k_tile_rates(Lattice L, int ntiles, RunSt
   }
 }
 
-// Two waves per SIMD: the window of one colour pass holds about 2048 tiles
-// at the 512^3 workload, all of them in flight at once on 256 CUs.
 // Sorts lk/lr/lc[lo, hi) by offset (the offsets are distinct): a bitonic
 // network across the lanes for up to 64 entries, odd-even transposition
 // rounds beyond (rare: appends from other tiles between two passes).
@@ -396,6 +400,8 @@ __device__ inline void sort_by_offset(int32_t *lk, float *lr, uint8_t *lc, int l
   }
 }
 
+// Three waves per SIMD (168 VGPRs): 3072 tiles in flight on 256 CUs, against
+// the ~4096 tiles of a colour pass at the 512^3 workload.
 template <bool LA, bool COV>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) k_pass(PassParams P) {
   extern __shared__ unsigned char smem[];
