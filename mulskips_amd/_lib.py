@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmskg.so")
 
 MSKG_OK, MSKG_EINVAL, MSKG_ESTATE, MSKG_ECAPACITY, MSKG_EDEVICE, MSKG_ENOMEM = range(6)
+MSKG_STOP_LIMIT, MSKG_STOP_FEW_MC, MSKG_STOP_NO_EVENTS = 0, 1, 3  # stop_reason of mskg_run
 
 EXPORTS = [
     "mskg_create", "mskg_destroy", "mskg_last_error", "mskg_load_state", "mskg_set_temperature",

@@ -1,0 +1,99 @@
+"""GPU edge cases of the entry points, each forced from the host so that
+every kernel still ends normally:
+* an empty lattice stops at once with the reference's "few MC particles"
+  reason (main.f:307-310: NumAdAtom <= 20);
+* a run of zero events and the replay of an empty log change nothing;
+* a time limit stops the run at the first sweep that reaches it;
+* a logged event at a point off the lattice is reported, not executed;
+* a tile list longer than tile_capacity is reported, and so is a
+  store_state buffer that is too small.
+"""
+import numpy as np
+import pytest
+
+from mulskips_amd import _lib
+from mulskips_amd.engine import Engine, EngineError
+from oracle.oracle import Oracle
+from tests.helpers import SMALL, oracle_state, small_deck
+
+pytestmark = pytest.mark.gpu
+
+
+def deck(tmp_path, key="surface"):
+    base, box, lens = SMALL[key]
+    path, d = small_deck(tmp_path, base, lens)
+    return path, d, box
+
+
+def loaded(tmp_path, **kw):
+    path, d, box = deck(tmp_path)
+    o = Oracle(*box)
+    o.load_deck(path)
+    st = oracle_state(o)
+    o.close()
+    eng = Engine(box, d.tables(), **kw)
+    eng.load_state(*st)
+    return eng, box
+
+
+def empty_lists():
+    e = {"xyz": np.zeros((0, 3), np.int32), "nn": np.zeros((0, 4, 3), np.int32)}
+    return dict(e, event=np.zeros(0, np.int32), prob=np.zeros(0)), dict(e), dict(e)
+
+
+def test_empty_lattice_stops_at_once(tmp_path):
+    _, d, box = deck(tmp_path)
+    lx, ly, lz = box
+    eng = Engine(box, d.tables())
+    eng.load_state(np.zeros((lz + 1, ly, lx), np.int32), *empty_lists())
+    n, t, stop = eng.run(1000)
+    assert (n, t, stop) == (0, 0.0, _lib.MSKG_STOP_FEW_MC)
+    eng.close()
+
+
+def test_zero_events_and_empty_replay_change_nothing(tmp_path):
+    eng, box = loaded(tmp_path, tile=24)
+    lat0, mc0, _, _ = eng.store_state()
+    n, t, stop = eng.run(0)
+    assert n == 0 and t == 0.0
+    eng.replay(np.zeros(0, _lib.EVENT_DTYPE))
+    lat1, mc1, _, _ = eng.store_state()
+    np.testing.assert_array_equal(lat0, lat1)
+    np.testing.assert_array_equal(mc0["xyz"], mc1["xyz"])
+    assert eng.counters(lists=False)[1] == 0.0
+    eng.close()
+
+
+def test_time_limit(tmp_path):
+    eng, box = loaded(tmp_path, tile=24, events_per_pass=4.0, tau_policy=1)
+    n1, t1, s1 = eng.run(10 ** 12, max_time=1e-3)
+    assert s1 == _lib.MSKG_STOP_LIMIT and n1 > 0 and t1 >= 1e-3
+    n2, t2, s2 = eng.run(10 ** 12, max_time=t1 + 1e-3)  # the limit is on the clock, not the call
+    assert s2 == _lib.MSKG_STOP_LIMIT and n2 > 0 and t2 >= t1 + 1e-3
+    assert eng.counters(lists=False)[1] == t2
+    eng.close()
+
+
+def test_off_lattice_event_is_reported(tmp_path):
+    eng, box = loaded(tmp_path, tile=24)
+    log = np.zeros(1, _lib.EVENT_DTYPE)
+    log["site"] = (1, 0, 0)  # x, y, z of mixed parity: no stored point
+    log["itrans"] = 4
+    log["rtot"] = 1.0
+    with pytest.raises(EngineError) as err:
+        eng.replay(log)
+    assert err.value.code == _lib.MSKG_ESTATE
+    eng.close()
+
+
+def test_capacities_are_checked(tmp_path):
+    eng, box = loaded(tmp_path, tile=24, tile_capacity=4)
+    with pytest.raises(EngineError) as err:
+        eng.run(1000)
+    assert err.value.code in (_lib.MSKG_ECAPACITY, _lib.MSKG_ESTATE)
+    eng.close()
+    eng, box = loaded(tmp_path, tile=24)
+    with pytest.raises(EngineError) as err:
+        eng.store_state(capacity=(1, 1, 1))
+    assert err.value.code == _lib.MSKG_ECAPACITY
+    eng.close()
