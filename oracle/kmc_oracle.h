@@ -31,6 +31,13 @@
  *   - LA-mode rates (Get_Prob.f:127-149) are pinned by the closed form only
  *     (the reference LA fixture is stale against the current source):
  *     "parity pinned by closed form, not by a reference run".
+ *   - The engine options with no reference counterpart: the second-neighbour
+ *     factor by a brute-force count of occupied shell points on a grown
+ *     surface (rates with the factor / rates without = f2[count]), the
+ *     frozen substrate by its rule at the z bound (tests/test_oracle_options.py);
+ *     the likelihood statistics (orc_adjoint_stats) by the identity with the
+ *     log-likelihood evaluated directly from the sum tree and by central
+ *     finite differences of it (tests/test_adjoint.py).
  */
 #ifndef MSK_KMC_ORACLE_H
 #define MSK_KMC_ORACLE_H
