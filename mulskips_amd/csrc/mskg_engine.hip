@@ -573,16 +573,23 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
       err = s_stop;
       break;
     }
-    // the touched points' new rates, one lane each
+    // the touched points' new rates, one lane each, as execute() left them
+    // on chip; a point that got no new rate keeps its list entry as it is
     const int nt = s_nt;
     if (lane < nt) {
       const P3 tp = s_tp[lane];
-      const int kl = tile_local(g, L.tile, tp);
+      const int ts = sc.tset[lane];
+      const int kl = ts ? tile_local(g, L.tile, tp) : -1;
       s_tk[lane] = kl;
       s_found[lane] = -1;
       if (kl >= 0) {
-        s_tr[lane] = L.rate[tp.i];
-        s_tc[lane] = L.cls[tp.i];
+        if (ts == 1) {
+          s_tr[lane] = sc.trate[lane];
+          s_tc[lane] = sc.tcls[lane];
+        } else {
+          s_tr[lane] = L.rate[tp.i];
+          s_tc[lane] = L.cls[tp.i];
+        }
       }
     }
     __syncthreads();

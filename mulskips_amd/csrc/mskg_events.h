@@ -217,7 +217,10 @@ MSKG_HD double pe_of(const Tables &t, int sp, int a, int b, int c, int d, int e,
 // isolated-atom rule.  No random number is drawn: the deposition species /
 // absorbed coverage is drawn when the event executes (class CLS_ADSORB).
 // Records that are newer than HBM (updated by this event on another lane):
-// reads of these points take the given record instead.
+// reads of these points take the given record instead.  known_i /
+// known_coor: a point whose coordination the caller already holds (the
+// event's site during the neighbour updates), read by the coordination-1
+// rule of an empty site without a gather.
 struct Ovr {
   int n;
   int64_t idx[5];
@@ -257,7 +260,8 @@ MSKG_HD int64_t second_step(const Box &b, int k) {
 // instructions, more registers: for the event kernels, not the sweep).
 template <bool LA = true, bool COV = true, bool STEP = false>
 MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp, int coor, const NL &nn, int &cls,
-                      double &rate, int &err, const Ovr *ovr = nullptr, int *key = nullptr) {
+                      double &rate, int &err, const Ovr *ovr = nullptr, int *key = nullptr, int64_t known_i = -1,
+                      int known_coor = 0) {
   const Tables &t = *L.tab;
   int kdummy;
   if (!key) key = &kdummy;
@@ -279,7 +283,10 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   if (kind == KIND_EMPTY) {  // :155-214
     if (coor >= 1 && coor <= 3) {
       bool ok = true;
-      if (coor == 1) ok = s_coor(rd_o(L, entry(L, nn, 0).i, ovr)) >= 2;
+      if (coor == 1) {
+        const int64_t i0 = entry(L, nn, 0).i;
+        ok = (i0 >= 0 && i0 == known_i ? known_coor : s_coor(rd_o(L, i0, ovr))) >= 2;
+      }
       if (ok) {
         double pdep = pd_of<LA>(t, 1, coor, temperature) + pd_of<LA>(t, 2, coor, temperature) +
                       pd_of<LA>(t, 3, coor, temperature);
@@ -413,7 +420,22 @@ struct EvScratch {
   // positions; complete after execute() returns
   P3 touched[MAX_TOUCHED];
   int ntouched;
+  // the new rate and class of touched point q as stored in HBM, when tset[q]
+  // is 1; 0: the event gave it no new rate; 2: read it from HBM (a list with
+  // repeated entries, whose points take several rates in turn)
+  float trate[MAX_TOUCHED];
+  uint8_t tcls[MAX_TOUCHED];
+  uint8_t tset[MAX_TOUCHED];
+  float secrate[12];  // the second-shell refresh of lane k + 1
+  uint8_t seccls[12];
 };
+
+// The value set_rate stores, kept on chip for touched point q.
+__device__ inline void note_rate(EvScratch &sc, int q, int cls, double rate) {
+  sc.trate[q] = cls ? (float)rate : 0.0f;
+  sc.tcls[q] = (uint8_t)cls;
+  sc.tset[q] = 1;
+}
 
 #define MSKG_SYNC() __syncthreads()
 
@@ -441,6 +463,7 @@ __device__ inline void touched_list(const P3 &s, EvScratch &sc, int n) {
   sc.ntouched = 0;
   sc.touched[sc.ntouched++] = s;
   for (int i = 0; i < n; i++) sc.touched[sc.ntouched++] = sc.pts[i];
+  for (int q = 0; q <= n; q++) sc.tset[q] = 0;
 }
 
 // Second-neighbour term: after a change of occupancy at s, the evaporating
@@ -664,6 +687,7 @@ __device__ void evap_prepare(const Lattice &L, const P3 &s, int i, EvScratch &sc
     z.temp = sp.temp;
     wr(L, p.i, z);
     set_rate(L, p, CLS_NONE, 0.0);
+    note_rate(sc, 1 + i, CLS_NONE, 0.0);
     return;
   }
   if (!occ && coornn == 2) {  // unfixed: the list keeps the reciprocal partner (NBuff)
@@ -823,7 +847,9 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
       if (q.need) {
         int cls = 0, key = -1;
         double r = 0.0;
-        get_prob<LA, COV, true>(L, rtot, q.p, q.sp, q.coor, q.nn, cls, r, sc.err[i], nullptr, &key);
+        // the site's coordination is final only in phase 3: here it is still coor_old
+        get_prob<LA, COV, true>(L, rtot, q.p, q.sp, q.coor, q.nn, cls, r, sc.err[i], nullptr, &key, s.i,
+                                sc.coor_old);
         bool ok = true;
         if (q.list_after) {
           if (cls && !set_list(L.box, q.sp, q.p.x, q.p.y, q.p.z, q.nn)) {
@@ -833,7 +859,10 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
             wr(L, q.p.i, q.sp);
           }
         }
-        if (ok) set_rate(L, q.p, cls, r, key);
+        if (ok) {
+          set_rate(L, q.p, cls, r, key);
+          note_rate(sc, 1 + i, cls, r);
+        }
       }
       if (evap && sc.seq) evap_nbuff_one(L, sc, i, nbuff, sc.err[i]);  // in order, as each neighbour is visited
     }
@@ -845,6 +874,8 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
   Site ss;
   bool unfix = false;
   if (lane == 0) {
+    if (sc.seq)  // a point listed twice took several rates in turn
+      for (int q = 1; q < sc.ntouched; q++) sc.tset[q] = 2;
     int coor = sc.coor_old;
     for (int i = 0; i < 4; i++) {
       if (dep || evap) coor += sc.inc[i];
@@ -859,6 +890,7 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
       ss.valid = 0;
       wr(L, s.i, ss);
       set_rate(L, s, CLS_NONE, 0.0);
+      note_rate(sc, 0, CLS_NONE, 0.0);
     } else {
       if (coor != sc.coor_old || evap) wr(L, s.i, ss);
       f.active = true;
@@ -883,6 +915,12 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
   }
   if (f.active) {
     set_rate(L, f.p, cls, r, key);
+    if (lane == 0) {
+      note_rate(sc, 0, cls, r);
+    } else {
+      sc.secrate[lane - 1] = cls ? (float)r : 0.0f;
+      sc.seccls[lane - 1] = (uint8_t)cls;
+    }
     if (e) {
       if (lane == 0) res.err = e;
       else sc.err2[lane - 1] = e;
@@ -893,7 +931,12 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
     if (lane == 0)
       for (int k = 0; k < 12; k++) {
         if (sc.err2[k]) res.err = sc.err2[k];
-        if (sc.sec[k].i >= 0) sc.touched[sc.ntouched++] = sc.sec[k];
+        if (sc.sec[k].i >= 0) {
+          sc.trate[sc.ntouched] = sc.secrate[k];
+          sc.tcls[sc.ntouched] = sc.seccls[k];
+          sc.tset[sc.ntouched] = 1;
+          sc.touched[sc.ntouched++] = sc.sec[k];
+        }
       }
   }
 }
