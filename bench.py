@@ -81,6 +81,7 @@ def parse():
     ap.add_argument("--replicas", type=int, default=256, help="batched-ensemble leg (0 = skip)")
     ap.add_argument("--trajectories", type=int, default=64, help="adjoint leg (0 = skip)")
     ap.add_argument("--trajectory-events", type=int, default=1000000)
+    ap.add_argument("--la-cycles", type=int, default=10, help="laser-annealing coupling leg (0 = skip)")
     ap.add_argument("--checkpoint-every", type=int, default=100000)
     return ap.parse_args()
 
@@ -173,6 +174,91 @@ def batched_leg(n):
             "speedup": (float(done.sum()) / wall) / (seq_events / seq_wall),
             "workload": f"{n} SiGe replicas, lateral {sizes} cells, heights 8-16 cells, 20 ML deposited each, "
                         "one launch (one workgroup per replica, serial BKL) vs sequential mskg_run"}
+
+
+def la_leg(cycles, events_per_cycle=100_000, seed=2026):
+    """Laser-annealing mode coupled to a temperature field, the grid of the
+    reference's LA runs, Si(0.7)Ge(0.3) on a 256 x 256 x 384 grid, taken as
+    264 x 264 x 384 (lateral edges must be whole 12-unit cells; a 240-unit
+    random-alloy slab), T(z) = 1700 + 300 tanh((z - z_m) / 5 nm) with the
+    front z_m rising one (001) monolayer (3 grid units) per exchange, from
+    the slab top (a front speed tied to the KMC clock, 2 m/s, would cross
+    the box within the first exchange at these rates).  Each cycle is one exchange
+    with the FEM side: the temperature map in its socket layout from host
+    memory (mskg_set_temperature_field), the rates re-evaluated, 1e5 events,
+    and the two phase maps back to host memory (mskg_phases).  Pair barriers
+    Si-Si / Si-Ge / Ge-Ge uniform in [0.4, 1.2] eV from a fixed seed; an
+    atom's melting barrier is the sum over its bonds, the solidification
+    barrier of a site of coordination c is c times the species' own pair
+    barrier; P0 = 1e13 /s, Tm 1687 K, Tflex 1650 K, sigma 50 K."""
+    import numpy as np
+
+    from mulskips_amd import models
+    from mulskips_amd.engine import Engine
+    rng = np.random.default_rng(seed)
+    e_ss, e_sg, e_gg = (float(v) for v in rng.uniform(0.4, 1.2, 3))
+    pair = {(1, 1): e_ss, (1, 2): e_sg, (2, 1): e_sg, (2, 2): e_gg}
+    tab = models.pair_energy_tables([[0.55, 0.5], [0.5, 0.45]], [1.0, 1.0], 1000.0, 1e13, (14, 32))
+    pe = np.zeros_like(np.asarray(tab["ptrans_e"], np.float64))
+    for sp in (1, 2):
+        for a in range(4):
+            for b in range(4 - a):
+                if a + b:
+                    pe[models.pe_index(sp, a, b, 0)] = a * pair[(sp, 1)] + b * pair[(sp, 2)]
+    pd = np.zeros(9)
+    for sp in (1, 2):
+        for c in (1, 2, 3):
+            pd[(sp - 1) + 3 * (c - 1)] = c * pair[(sp, sp)]
+    tab["ptrans_e"], tab["ptrans_d"] = pe, pd
+    box = (264, 264, 384)
+    lx, ly, lz = box
+    eng = Engine(box, tab, seed=seed, tau_policy=1, events_per_pass=4.0,
+                 la=dict(tm=1687.0, p0=1e13, sigma=50.0, tflex=1650.0))
+    eng.build_alloy_slab(240, 0.3)
+    zs = np.arange(lz, dtype=np.float64)
+    grid_nm = 0.5431 / 12  # the super-lattice step: a(Si) / 12
+
+    def field(zm):
+        t = np.rint(1700.0 + 300.0 * np.tanh((zs - zm) * grid_nm / 5.0)).astype(np.int32)
+        return np.ascontiguousarray(np.broadcast_to(t, (lx * ly, lz)))  # b(1+z, 1+x+LenX*y)
+
+    solid0 = int(((eng.phases("socket")[1] == 1) | (eng.phases("socket")[1] == 2)).sum())
+    zm = 240.0
+    t_set = t_run = t_ph = 0.0
+    events = 0
+    stops = []
+    for k in range(cycles + 1):
+        f = field(zm)
+        t0 = time.perf_counter()
+        eng.set_temperature_field(f, "socket")
+        eng.refresh_rates()
+        t1 = time.perf_counter()
+        n, t_kmc, stop = eng.run(events_per_cycle)
+        stops.append(int(stop))
+        t2 = time.perf_counter()
+        c, d = eng.phases("socket")
+        t3 = time.perf_counter()
+        zm += 3.0
+        if k:  # the first cycle warms up
+            t_set += t1 - t0
+            t_run += t2 - t1
+            t_ph += t3 - t2
+            events += n
+    solid = int(((d == 1) | (d == 2)).sum())
+    eng.close()
+    map_bytes = lx * ly * lz * 4
+    return {"value": events / t_run, "unit": "events/s (KMC only)", "cycles": cycles,
+            "events_per_cycle": events_per_cycle, "set_field_ms": t_set * 1e3 / cycles,
+            "phases_ms": t_ph * 1e3 / cycles, "run_ms": t_run * 1e3 / cycles,
+            "coupling_share": (t_set + t_ph) / (t_set + t_ph + t_run),
+            "pcie_GBps_set_field": map_bytes / (t_set / cycles) / 1e9,
+            "pcie_GBps_phases": 2 * map_bytes / (t_ph / cycles) / 1e9,
+            "solid_points_at_start": solid0, "solid_points_at_end": solid, "front_z_at_end": zm,
+            "kmc_time_s": t_kmc, "stop_reasons": stops,
+            "barriers_eV": {"Si-Si": e_ss, "Si-Ge": e_sg, "Ge-Ge": e_gg},
+            "workload": "laser-annealing mode, SiGe x=0.3, 264x264x384 grid, FEM temperature map in and phase "
+                        "maps out every 1e5 events (set_field_ms includes the rate re-evaluation); window policy 1, "
+                        "k = 4 as in the main leg"}
 
 
 def adjoint_leg(n_traj, n_events, segment):
@@ -358,6 +444,11 @@ def main():
     if a.trajectories > 0:
         out["adjoint"] = adjoint_leg(a.trajectories, a.trajectory_events, a.checkpoint_every)
         print(f"# adjoint leg: {out['adjoint']['value']:.4g} gradients/s", file=sys.stderr, flush=True)
+    if a.la_cycles > 0:
+        out["laser_annealing"] = la_leg(a.la_cycles)
+        print(f"# laser-annealing leg: {out['laser_annealing']['value']:.4g} events/s, coupling "
+              f"{out['laser_annealing']['set_field_ms']:.1f} + {out['laser_annealing']['phases_ms']:.1f} ms per cycle",
+              file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tables, substrate, a.cpu_seconds, max(1, min(a.cpu_cores, os.cpu_count())))
     print(json.dumps(out), flush=True)

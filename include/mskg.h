@@ -127,8 +127,39 @@ int mskg_build_slab(mskg_engine *e, int32_t mode, int32_t len);
 int mskg_build_alloy_slab(mskg_engine *e, int32_t len, double fraction);
 
 /* Per-site temperature in K for laser-annealing mode, (len_x, len_y,
- * len_z+1) like LattCoo's T bits (SetT.f:25-50). */
+ * len_z+1) like LattCoo's T bits (SetT.f:25-50).  Every value at a lattice
+ * point must lie in 0..8191 (else MSKG_EINVAL and nothing changes).  Rates
+ * are not re-evaluated: call mskg_refresh_rates. */
 int mskg_set_temperature(mskg_engine *e, const int32_t *temp_k);
+
+/* Layouts of the whole-grid int32 maps the laser-annealing build exchanges
+ * with its FEM driver (mulskips-source-LaserAnnealing/modules/defsystem.f:88-91,
+ * main.f:211-334):
+ *   MSKG_FIELD_SOCKET  DIMENSION(LenZ, LenX*LenY): element z + LenZ*(x + LenX*y)
+ *   MSKG_FIELD_PLANES  the temperature / Coor files: x + LenX*(y + LenY*z)
+ * both for the planes z = 0..LenZ-1. */
+#define MSKG_FIELD_SOCKET 0
+#define MSKG_FIELD_PLANES 1
+
+/* SetT (mulskips-source-LaserAnnealing/initial/SetT.f:38-48 for the socket
+ * map, :56-66 for the file), on the device:
+ * every point of the planes z < LenZ takes the low 13 bits of its map value
+ * as its temperature, except where the value is 10 (the FEM side's wall
+ * marker), which leaves the point as it is.  `field` is a host pointer, or a
+ * device pointer on the engine's device when on_device != 0 (ready before the
+ * call; it is read on the engine's stream).  Replaces the socket branch of
+ * SetT, whose array b arrives with SENDFIELD (main.f:211-230); like SetT it
+ * leaves the rates as they are (mskg_refresh_rates re-evaluates them). */
+int mskg_set_temperature_field(mskg_engine *e, const int32_t *field, int32_t layout, int32_t on_device);
+
+/* The phase maps the laser-annealing build returns to its FEM driver
+ * (mulskips-source-LaserAnnealing/main.f:796-818, sent with GETPHASES /
+ * GETXGE, :273-335): for the grid points of the
+ * planes z < LenZ, coor = coordination and species = species index (1..3) of
+ * an occupied point, 10 and 10 on a wall (index 113), 0 and 0 elsewhere.
+ * Either output may be null; host pointers, or device pointers when
+ * on_device != 0 (complete when the call returns). */
+int mskg_phases(mskg_engine *e, int32_t *coor, int32_t *species, int32_t layout, int32_t on_device);
 
 /* Rate-update sweep: re-evaluates Get_Prob (Get_Prob.f:54-331) for every
  * MC particle of the lattice from the current state.  */

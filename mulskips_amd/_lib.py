@@ -19,8 +19,9 @@ EXPORTS = [
     "mskg_refresh_rates", "mskg_run", "mskg_replay", "mskg_store_state", "mskg_site_rates",
     "mskg_counters", "mskg_last_timing", "mskg_geometry_find_nn", "mskg_geometry_find_sn",
     "mskg_build_slab", "mskg_set_event_log", "mskg_get_event_log", "mskg_run_serial", "mskg_adjoint", "mskg_build_alloy_slab",
-    "mskg_track_classes", "mskg_copy_state", "mskg_frame",
+    "mskg_track_classes", "mskg_copy_state", "mskg_frame", "mskg_set_temperature_field", "mskg_phases",
 ]
+MSKG_FIELD_SOCKET, MSKG_FIELD_PLANES = 0, 1  # layouts of the laser-annealing maps
 
 # rate-class keys of the trajectory likelihood (include/mskg.h)
 KEY_ADS, KEY_PH, KEY_ISO, KEY_N, ADJ_NS = 2496, 2499, 2500, 2501, 2510
@@ -80,6 +81,8 @@ def lib():
     L.mskg_track_classes.argtypes = [P, I32]
     L.mskg_copy_state.argtypes = [P, P]
     L.mskg_frame.argtypes = [P, P, I64, P, I64, P, I64, P]
+    L.mskg_set_temperature_field.argtypes = [P, P, I32, I32]
+    L.mskg_phases.argtypes = [P, P, P, I32, I32]
     L.mskg_adjoint.argtypes = [P, I32, P, P, P, P, P, P]
     L.mskg_geometry_find_nn.argtypes = [P, P, P, I32, I32, P]
     L.mskg_geometry_find_sn.argtypes = [P, P, P, I32, I32, P, P]

@@ -1325,6 +1325,99 @@ __global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_
   }
 }
 
+// ---- laser-annealing coupling: temperature map in, phase maps out ----------
+// The laser-annealing build exchanges whole-grid int32 maps with its FEM
+// driver: the temperature map b and the phase maps c (coordination) and d
+// (species), DIMENSION(LenZ, LenX*LenY) (mulskips-source-LaserAnnealing/
+// modules/defsystem.f:88-91), or the temperature file's planes
+// (mulskips-source-LaserAnnealing/initial/SetT.f:56-66).  Both are read and
+// written here on the device, so a cycle moves 4 B per grid point over PCIe
+// instead of the lattice.
+enum : int { FIELD_LATTCOO = 2 };  // x fastest over the LenZ+1 planes of LattCoo
+MSKG_HD int64_t field_index(const Box &b, int layout, int x, int y, int z) {
+  if (layout == MSKG_FIELD_SOCKET) return z + (int64_t)b.lz * (x + (int64_t)b.lx * y);
+  return x + (int64_t)b.lx * (y + (int64_t)b.ly * z);
+}
+
+// Range check of a strict temperature map at the stored points (the 13 bits
+// of LattCoo's T field), before anything is written.
+__global__ void __launch_bounds__(256) k_check_field(Lattice L, const int32_t *f, int layout, int *err) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < L.box.npts;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    pxyz(L.box, p, x, y, z);
+    const int32_t v = f[field_index(L.box, layout, x, y, z)];
+    if (v < 0 || v > 8191) atomicExch(err, 1);
+  }
+}
+
+// SetT.f:38-48 (the laser-annealing build): every point of the planes
+// 0..LenZ-1 takes the low LenT = 13 bits of its map value (MVBITS), except
+// where the value is 10, the FEM side's wall marker, which leaves the point
+// as it is.  strict: the map
+// covers LattCoo's LenZ+1 planes and every value is taken as it is (range
+// checked before).  One thread per stored point, records read and written
+// whole.
+__global__ void __launch_bounds__(256) k_set_temperature(Lattice L, const int32_t *f, int layout, int strict) {
+  const int64_t n = L.box.npts;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    pxyz(L.box, p, x, y, z);
+    if (!strict && z >= L.box.lz) continue;
+    const int32_t v = f[field_index(L.box, layout, x, y, z)];
+    if (!strict && v == 10) continue;
+    Site s = L.site[p];
+    s.temp = (uint16_t)(v & 8191);
+    L.site[p] = s;
+  }
+}
+
+// main.f:796-818 of the laser-annealing build: for the points of planes
+// 0..LenZ-1, c = coordination and d = species of an occupied point, 10 and
+// 10 on a wall (index 113), 0 and 0 elsewhere (grid points off the diamond
+// super-lattice are never occupied).  Rows of the map (a z column of the
+// socket layout, an x row of the plane layout) are strided over the grid;
+// writes are in map order.
+__global__ void __launch_bounds__(256) k_phases(Lattice L, int layout, const uint32_t *offwall, int wall_planes,
+                                               int32_t *c, int32_t *d) {
+  const Box &b = L.box;
+  const int64_t rows = layout == MSKG_FIELD_SOCKET ? (int64_t)b.lx * b.ly : (int64_t)b.ly * b.lz;
+  const int len = layout == MSKG_FIELD_SOCKET ? b.lz : b.lx;
+  for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    int x = 0, y = 0, z = 0;
+    if (layout == MSKG_FIELD_SOCKET) {
+      x = (int)(r % b.lx);
+      y = (int)(r / b.lx);
+    } else {
+      y = (int)(r % b.ly);
+      z = (int)(r / b.ly);
+    }
+    for (int k = threadIdx.x; k < len; k += blockDim.x) {
+      if (layout == MSKG_FIELD_SOCKET) z = k;
+      else x = k;
+      int32_t cv = 0, dv = 0;
+      if (valid_parity(x, y, z)) {
+        const Site s = L.site[pidx(b, x, y, z)];
+        const int kind = s_kind(s);
+        if (kind == KIND_WALL) {
+          cv = dv = 10;
+        } else if (kind_species(kind)) {
+          cv = s_coor(s);
+          dv = kind;
+        }
+      } else {
+        const int64_t g = x + (int64_t)b.lx * (y + (int64_t)b.ly * z);
+        const bool wall = (offwall && ((offwall[g >> 5] >> (g & 31)) & 1)) ||
+                          (wall_planes && (z <= 5 || (z >= b.lz - 6 && z <= b.lz - 1)));
+        if (wall) cv = dv = 10;
+      }
+      const int64_t o = r * len + k;
+      if (c) c[o] = cv;
+      if (d) d[o] = dv;
+    }
+  }
+}
+
 // The kernel instance for a rate model (all four share one signature).
 template <class F>
 F pick_variant(bool la, bool cov, F tt, F tf, F ft, F ff) {
@@ -1360,6 +1453,13 @@ struct mskg_engine {
   int32_t *d_tl_own = nullptr;
   double *d_tile_r = nullptr;
   int32_t *d_tile_n = nullptr;
+  int32_t *d_field[2] = {nullptr, nullptr};  // grid-sized maps of the laser-annealing coupling (on first use)
+  // walls on grid points off the diamond super-lattice (the reference's
+  // builders fill whole wall regions), which only the phase maps report:
+  // one bit per grid point from mskg_load_state (null: none), or the wall
+  // planes of the device builders
+  uint32_t *d_offwall = nullptr;
+  bool offwall_planes = false;
   int *d_list_err = nullptr;
   int *d_win = nullptr;
   Geometry *d_geo = nullptr;
@@ -1602,7 +1702,8 @@ void mskg_destroy(mskg_engine *e) {
   hipFree(e->d_site); hipFree(e->d_rate); hipFree(e->d_cls);
   if (e->graph_exec) hipGraphExecDestroy(e->graph_exec);
   if (e->graph) hipGraphDestroy(e->graph);
-  hipFree(e->d_inlist); hipFree(e->d_tl_idx); hipFree(e->d_tl_n); hipFree(e->d_tl_own); hipFree(e->d_tile_r); hipFree(e->d_list_err); hipFree(e->d_win);
+  hipFree(e->d_inlist); hipFree(e->d_tl_idx); hipFree(e->d_tl_n); hipFree(e->d_tl_own); hipFree(e->d_tile_r); hipFree(e->d_list_err);
+  hipFree(e->d_field[0]); hipFree(e->d_field[1]); hipFree(e->d_offwall); hipFree(e->d_win);
   hipFree(e->d_geo); hipFree(e->d_tab); hipFree(e->d_st); hipFree(e->d_err); hipFree(e->d_log);
   hipFree(e->d_tsum); hipFree(e->d_bsum); hipFree(e->d_key); hipFree(e->d_key_seen); hipFree(e->d_rate_seen);
   if (e->ev0) hipEventDestroy(e->ev0);
@@ -1634,6 +1735,24 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt, int32_t n_ad, const int
         s.kc = (uint8_t)(kind | (((v >> 24) & 7) << 3));
         s.temp = (uint16_t)((v >> 9) & 0x1FFF);
       }
+  // walls the caller's LattCoo holds off the lattice (index 113)
+  std::vector<uint32_t> offwall;
+  for (int z = 0; z <= b.lz; z++)
+    for (int y = 0; y < b.ly; y++)
+      for (int x = 0; x < b.lx; x++) {
+        if (valid_parity(x, y, z)) continue;
+        const int64_t g = (int64_t)x + (int64_t)b.lx * ((int64_t)y + (int64_t)b.ly * z);
+        if ((latt[g] & 255) != 113) continue;
+        if (offwall.empty()) offwall.assign((size_t)((b.lx * (int64_t)b.ly * (b.lz + 1) + 31) / 32), 0u);
+        offwall[(size_t)(g >> 5)] |= 1u << (g & 31);
+      }
+  hipFree(e->d_offwall);
+  e->d_offwall = nullptr;
+  e->offwall_planes = false;
+  if (!offwall.empty()) {
+    HIPCHK(e, hipMalloc(&e->d_offwall, offwall.size() * sizeof(uint32_t)));
+    HIPCHK(e, hipMemcpy(e->d_offwall, offwall.data(), offwall.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
   double rsum = 0.0;
   auto put = [&](int which, int32_t n, const int32_t *xyz, const int32_t *nn) -> int {
     for (int32_t i = 0; i < n; i++) {
@@ -1688,22 +1807,81 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt, int32_t n_ad, const int
   return MSKG_OK;
 }
 
+// A grid-sized device map of the coupling (LenX * LenY * (LenZ+1) int32).
+static int32_t *field_buffer(mskg_engine *e, int k) {
+  if (!e->d_field[k]) {
+    const Box &b = e->box;
+    if (hipMalloc(&e->d_field[k], (size_t)b.lx * b.ly * (b.lz + 1) * sizeof(int32_t)) != hipSuccess) {
+      e->d_field[k] = nullptr;
+      fail_(e, MSKG_ENOMEM, "no device memory for a %d x %d x %d map", b.lx, b.ly, b.lz + 1);
+    }
+  }
+  return e->d_field[k];
+}
+
+static int set_temperature_map(mskg_engine *e, const int32_t *map, int layout, int on_device) {
+  const Box &b = e->box;
+  const bool strict = layout == FIELD_LATTCOO;
+  const int64_t n = (int64_t)b.lx * b.ly * (strict ? b.lz + 1 : b.lz);
+  const int32_t *f = map;
+  if (!on_device) {
+    int32_t *buf = field_buffer(e, 0);
+    if (!buf) return MSKG_ENOMEM;
+    HIPCHK(e, hipMemcpyAsync(buf, map, n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream));
+    f = buf;
+  }
+  const int blocks = (int)std::min<int64_t>((std::max<int64_t>(n, b.npts) + 255) / 256, 256 * 64);
+  if (strict) {
+    int bad = 0;
+    HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
+    hipLaunchKernelGGL(k_check_field, dim3(blocks), dim3(256), 0, e->stream, make_lattice(e), f, layout, e->d_err);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(&bad, e->d_err, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (bad) return fail_(e, MSKG_EINVAL, "a temperature is out of the 13-bit range 0..8191 K");
+  }
+  hipLaunchKernelGGL(k_set_temperature, dim3(blocks), dim3(256), 0, e->stream, make_lattice(e), f, layout,
+                     strict ? 1 : 0);
+  HIPCHK(e, hipGetLastError());
+  HIPCHK(e, hipStreamSynchronize(e->stream));
+  return MSKG_OK;
+}
+
 int mskg_set_temperature(mskg_engine *e, const int32_t *temp_k) {
   if (!e || !temp_k) return MSKG_EINVAL;
   if (!e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  return set_temperature_map(e, temp_k, FIELD_LATTCOO, 0);
+}
+
+int mskg_set_temperature_field(mskg_engine *e, const int32_t *field, int32_t layout, int32_t on_device) {
+  if (!e || !field) return MSKG_EINVAL;
+  if (!e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  if (layout != MSKG_FIELD_SOCKET && layout != MSKG_FIELD_PLANES)
+    return fail_(e, MSKG_EINVAL, "field layout %d: 0 (socket) or 1 (planes)", layout);
+  return set_temperature_map(e, field, layout, on_device);
+}
+
+int mskg_phases(mskg_engine *e, int32_t *coor, int32_t *species, int32_t layout, int32_t on_device) {
+  if (!e) return MSKG_EINVAL;
+  if (!e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  if (layout != MSKG_FIELD_SOCKET && layout != MSKG_FIELD_PLANES)
+    return fail_(e, MSKG_EINVAL, "phase map layout %d: 0 (socket) or 1 (planes)", layout);
+  if (!coor && !species) return MSKG_OK;
   const Box &b = e->box;
-  std::vector<Site> site((size_t)b.npts);
-  HIPCHK(e, hipMemcpyAsync(site.data(), e->d_site, site.size() * sizeof(Site), hipMemcpyDeviceToHost, e->stream));
-  HIPCHK(e, hipStreamSynchronize(e->stream));
-  for (int z = 0; z <= b.lz; z++)
-    for (int y = 0; y < b.ly; y++)
-      for (int x = 0; x < b.lx; x++)
-        if (valid_parity(x, y, z)) {
-          int32_t v = temp_k[(int64_t)x + (int64_t)b.lx * ((int64_t)y + (int64_t)b.ly * z)];
-          if (v < 0 || v > 8191) return fail_(e, MSKG_EINVAL, "temperature %d K out of the 13-bit range", v);
-          site[(size_t)pidx(b, x, y, z)].temp = (uint16_t)v;
-        }
-  HIPCHK(e, hipMemcpyAsync(e->d_site, site.data(), site.size() * sizeof(Site), hipMemcpyHostToDevice, e->stream));
+  const int64_t n = (int64_t)b.lx * b.ly * b.lz;
+  int32_t *out[2] = {coor, species};
+  if (!on_device)
+    for (int k = 0; k < 2; k++)
+      if (out[k] && !(out[k] = field_buffer(e, k))) return MSKG_ENOMEM;
+  const int64_t rows = layout == MSKG_FIELD_SOCKET ? (int64_t)b.lx * b.ly : (int64_t)b.ly * b.lz;
+  const int blocks = (int)std::min<int64_t>(rows, 256 * 64);
+  hipLaunchKernelGGL(k_phases, dim3(blocks), dim3(256), 0, e->stream, make_lattice(e), (int)layout, e->d_offwall,
+                     e->offwall_planes ? 1 : 0, out[0], out[1]);
+  HIPCHK(e, hipGetLastError());
+  if (!on_device) {
+    if (coor) HIPCHK(e, hipMemcpyAsync(coor, out[0], n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+    if (species) HIPCHK(e, hipMemcpyAsync(species, out[1], n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+  }
   HIPCHK(e, hipStreamSynchronize(e->stream));
   return MSKG_OK;
 }
@@ -2011,6 +2189,9 @@ static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction
   hipLaunchKernelGGL(k_build_slab, dim3(blocks), dim3(256), 0, e->stream, L, len, mode, fraction,
                      (uint32_t)e->cfg.seed, (uint32_t)(e->cfg.seed >> 32));
   HIPCHK(e, hipGetLastError());
+  hipFree(e->d_offwall);  // the SetSi builders fill their wall planes at every grid point
+  e->d_offwall = nullptr;
+  e->offwall_planes = mode != 1;
   if (e->tab.use_f2 || mode == 2) {  // the surface atoms' rates from their neighbours (species, second shell)
     HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
     hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, 0.0, e->d_err, 1);
@@ -2065,6 +2246,15 @@ int mskg_copy_state(mskg_engine *dst, const mskg_engine *src) {
   HIPCHK(dst, hipMemcpyAsync(dst->d_rate, src->d_rate, np * sizeof(float), hipMemcpyDeviceToDevice, dst->stream));
   HIPCHK(dst, hipMemcpyAsync(dst->d_cls, src->d_cls, np, hipMemcpyDeviceToDevice, dst->stream));
   dst->keys_valid = false;
+  hipFree(dst->d_offwall);
+  dst->d_offwall = nullptr;
+  dst->offwall_planes = src->offwall_planes;
+  if (src->d_offwall) {
+    const size_t words = (size_t)((src->box.lx * (int64_t)src->box.ly * (src->box.lz + 1) + 31) / 32);
+    HIPCHK(dst, hipMalloc(&dst->d_offwall, words * sizeof(uint32_t)));
+    HIPCHK(dst, hipMemcpyAsync(dst->d_offwall, src->d_offwall, words * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                               dst->stream));
+  }
   if (dst->track && src->track && src->keys_valid) {
     HIPCHK(dst, hipMemcpyAsync(dst->d_key, src->d_key, np * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
     dst->keys_valid = true;

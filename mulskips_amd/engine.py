@@ -12,9 +12,28 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import Config, EVENT_DTYPE, ptr
+from ._lib import Config, EVENT_DTYPE, MSKG_FIELD_PLANES, MSKG_FIELD_SOCKET, ptr
 
 CLASS_NAMES = {0: "none", 1: "adsorb", 2: "placeholder", 4: "evaporate", 5: "desorb", 6: "isolated"}
+
+
+def _layout(name):
+    codes = {"socket": MSKG_FIELD_SOCKET, "planes": MSKG_FIELD_PLANES}
+    if name not in codes:
+        raise ValueError(f"layout {name!r}: 'socket' or 'planes'")
+    return codes[name]
+
+
+def _torch():
+    import torch  # device tensors only
+    return torch
+
+
+def _device_tensor(a):
+    """a, if it is a torch tensor in GPU memory, else None."""
+    if type(a).__module__.startswith("torch") and getattr(a, "is_cuda", False):
+        return a
+    return None
 
 
 class EngineError(RuntimeError):
@@ -155,6 +174,49 @@ class Engine:
     def set_temperature(self, temp_k):
         t = np.ascontiguousarray(temp_k, np.int32)
         self._check(self.L.mskg_set_temperature(self.h, ptr(t)))
+
+    def set_temperature_field(self, field, layout="socket"):
+        """SetT of the laser-annealing build on the device: the FEM map in its
+        socket layout (Fortran b(LenZ, LenX*LenY): numpy shape (LenX*LenY,
+        LenZ), z fastest) or its file layout (numpy shape (LenZ, LenY,
+        LenX)); value 10 leaves a point as it is, other values
+        give their low 13 bits.  field: a numpy array, or an int32 torch
+        tensor on the engine's device (read in place).  Rates are left as
+        they are (refresh_rates())."""
+        code = _layout(layout)
+        shape = self._map_shape(code)
+        dev = _device_tensor(field)
+        if dev is not None:
+            assert tuple(dev.shape) == shape and dev.dtype == _torch().int32 and dev.is_contiguous(), dev.shape
+            _torch().cuda.current_stream(dev.device).synchronize()  # the engine reads it on its own stream
+            self._check(self.L.mskg_set_temperature_field(self.h, C.c_void_p(dev.data_ptr()), code, 1))
+            return
+        f = np.ascontiguousarray(field, np.int32)
+        assert f.shape == shape, (f.shape, shape)
+        self._check(self.L.mskg_set_temperature_field(self.h, ptr(f), code, 0))
+
+    def phases(self, layout="socket", device=False):
+        """The phase maps the laser-annealing build returns to its FEM driver:
+        (coor, species) int32 of numpy shape (LenX*LenY, LenZ) for the socket
+        layout (Fortran c(LenZ, LenX*LenY)) or (LenZ, LenY, LenX) for the file
+        layout; coordination and species of
+        occupied points, 10 on walls, 0 elsewhere.  device=True returns torch
+        tensors on the engine's device."""
+        code = _layout(layout)
+        shape = self._map_shape(code)
+        if device:
+            torch = _torch()
+            out = [torch.empty(shape, dtype=torch.int32, device="cuda") for _ in range(2)]
+            self._check(self.L.mskg_phases(self.h, C.c_void_p(out[0].data_ptr()), C.c_void_p(out[1].data_ptr()),
+                                           code, 1))
+            return out[0], out[1]
+        out = [np.empty(shape, np.int32) for _ in range(2)]
+        self._check(self.L.mskg_phases(self.h, ptr(out[0]), ptr(out[1]), code, 0))
+        return out[0], out[1]
+
+    def _map_shape(self, code):
+        lx, ly, lz = self.box
+        return (lx * ly, lz) if code == MSKG_FIELD_SOCKET else (lz, ly, lx)
 
     def refresh_rates(self):
         self._check(self.L.mskg_refresh_rates(self.h))

@@ -16,6 +16,7 @@ module mskg_bindings
   integer(c_int), parameter :: MSKG_OK = 0, MSKG_EINVAL = 1, MSKG_ESTATE = 2, &
                                MSKG_ECAPACITY = 3, MSKG_EDEVICE = 4, MSKG_ENOMEM = 5
   integer(c_int), parameter :: MSKG_STOP_LIMIT = 0, MSKG_STOP_FEW_MC = 1, MSKG_STOP_NO_EVENTS = 3
+  integer(c_int32_t), parameter :: MSKG_FIELD_SOCKET = 0, MSKG_FIELD_PLANES = 1
   integer(c_int), parameter :: MSKG_KEY_ADS = 2496, MSKG_KEY_PH = 2499, MSKG_KEY_ISO = 2500, MSKG_KEY_N = 2501, &
                                MSKG_ADJ_NS = 2510
 
@@ -86,6 +87,21 @@ module mskg_bindings
     integer(c_int) function mskg_set_temperature(eng, temp_k) bind(c, name="mskg_set_temperature")
       import :: c_int, c_ptr
       type(c_ptr), value :: eng, temp_k
+    end function
+
+    ! field / coor / species: DIMENSION(LenZ, LenX*LenY) arrays a, b, c, d of
+    ! the laser-annealing build with layout MSKG_FIELD_SOCKET (0)
+    integer(c_int) function mskg_set_temperature_field(eng, field, layout, on_device) &
+        bind(c, name="mskg_set_temperature_field")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: eng, field
+      integer(c_int32_t), value :: layout, on_device
+    end function
+
+    integer(c_int) function mskg_phases(eng, coor, species, layout, on_device) bind(c, name="mskg_phases")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: eng, coor, species
+      integer(c_int32_t), value :: layout, on_device
     end function
 
     integer(c_int) function mskg_refresh_rates(eng) bind(c, name="mskg_refresh_rates")

@@ -1943,6 +1943,59 @@ int orc_adjoint_stats(orc_sim *s, const orc_event *evs, int64_t n, const double 
   return 0;
 }
 
+/* ---- laser-annealing coupling maps ------------------------------------ */
+/* Index of grid point (x, y, z), z < LenZ, in a coupling map: layout 0 the
+ * socket arrays DIMENSION(LenZ, LenX*LenY) (mulskips-source-LaserAnnealing/
+ * modules/defsystem.f:88-91: element (1+z, 1+x+LenX*y)), layout 1 the
+ * temperature / Coor file planes (x fastest, then y, then z). */
+static int64_t map_index(const orc_sim *s, int layout, int x, int y, int z) {
+  if (layout == 0) return (int64_t)z + (int64_t)s->lz * ((int64_t)x + (int64_t)s->lx * y);
+  return (int64_t)x + (int64_t)s->lx * ((int64_t)y + (int64_t)s->ly * z);
+}
+
+/* SetT, mulskips-source-LaserAnnealing/initial/SetT.f:38-48 (socket map b)
+ * and :56-66 (file planes, read as INT): every grid point of the planes
+ * z = 0..LenZ-1 whose map value is not 10 takes the value's low LenT bits
+ * at PosT (MVBITS). */
+int orc_set_temperature_field(orc_sim *s, const int32_t *field, int layout) {
+  if (layout != 0 && layout != 1) return -1;
+  for (int z = 0; z < s->lz; z++)
+    for (int y = 0; y < s->ly; y++)
+      for (int x = 0; x < s->lx; x++) {
+        const int32_t t = field[map_index(s, layout, x, y, z)];
+        if (t != 10) mvbits(t, LEN_T, &s->latt[gidx(s, x, y, z)], POS_T);
+      }
+  return 0;
+}
+
+/* The phase maps of mulskips-source-LaserAnnealing/main.f:796-818: c and d
+ * zeroed, then for every grid point of the planes z = 0..LenZ-1 that is not
+ * a wall (IQStat /= 113) and occupied, c = Coo and d = IAt (the species);
+ * 10 and 10 on walls.  The laser-annealing build keeps the species one bit
+ * higher (PosSiC = 28, its defsystem.f:99); IAt is the species index in
+ * both layouts. */
+int orc_phases(const orc_sim *s, int32_t *c, int32_t *d, int layout) {
+  if (layout != 0 && layout != 1) return -1;
+  const int64_t n = (int64_t)s->lx * s->ly * s->lz;
+  for (int64_t i = 0; i < n; i++) c[i] = d[i] = 0;
+  for (int k = 0; k < s->lz; k++)
+    for (int j = 0; j < s->ly; j++)
+      for (int i = 0; i < s->lx; i++) {
+        const int32_t w = s->latt[gidx(s, i, j, k)];
+        const int64_t o = map_index(s, layout, i, j, k);
+        if (ibits(w, POS_INDEX, LEN_INDEX) != WALL) {
+          if (ibits(w, POS_OCC, LEN_OCC) == 1) {
+            c[o] = ibits(w, POS_COOR, LEN_COOR);
+            d[o] = ibits(w, POS_SIC, LEN_SIC);
+          }
+        } else {
+          c[o] = 10;
+          d[o] = 10;
+        }
+      }
+  return 0;
+}
+
 int32_t *orc_lattcoo(orc_sim *s) { return s->latt; }
 int32_t *orc_lattind(orc_sim *s) { return s->lind; }
 void orc_dims(const orc_sim *s, int *lx, int *ly, int *lz) { *lx = s->lx; *ly = s->ly; *lz = s->lz; }

@@ -38,6 +38,12 @@
  *     the likelihood statistics (orc_adjoint_stats) by the identity with the
  *     log-likelihood evaluated directly from the sum tree and by central
  *     finite differences of it (tests/test_adjoint.py).
+ *   - The laser-annealing coupling maps (orc_set_temperature_field,
+ *     orc_phases) by known answers built from SetT's and main.f's rules:
+ *     a map whose value names its own grid point (a transposed or shifted
+ *     layout fails), the wall marker 10 and values beyond 13 bits, the top
+ *     plane left alone; phase counts against the particle lists of a
+ *     regression deck (tests/test_oracle_options.py).
  */
 #ifndef MSK_KMC_ORACLE_H
 #define MSK_KMC_ORACLE_H
@@ -137,6 +143,12 @@ int orc_write_frame(orc_sim *s, const char *xyz_path, const char *d_path,
                     int64_t iter);
 
 /* ---- state access (for building device states and comparing) --------- */
+/* The laser-annealing coupling maps (layout 0: DIMENSION(LenZ, LenX*LenY),
+ * z fastest; layout 1: x fastest, then y, then z; planes z < LenZ):
+ * SetT's temperature update and the phase maps c/d sent back to the FEM
+ * side.  Return 0, or -1 for a bad layout. */
+int orc_set_temperature_field(orc_sim *s, const int32_t *field, int layout);
+int orc_phases(const orc_sim *s, int32_t *c, int32_t *d, int layout);
 int32_t *orc_lattcoo(orc_sim *s);   /* (LenX, LenY, LenZ+1), x fastest   */
 int32_t *orc_lattind(orc_sim *s);
 void orc_dims(const orc_sim *s, int *lx, int *ly, int *lz);

@@ -49,6 +49,8 @@ def _load():
         L.orc_run.argtypes = [P, I64, D, P]
         L.orc_adjoint_stats.argtypes = [P, P, I64, P, P, P, P]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
+        L.orc_set_temperature_field.argtypes = [P, P, I]
+        L.orc_phases.argtypes = [P, P, P, I]
         L.orc_lattcoo.restype = C.POINTER(C.c_int32)
         L.orc_lattcoo.argtypes = [P]
         L.orc_dims.argtypes = [P, P, P, P]
@@ -176,6 +178,25 @@ class Oracle:
             rc = self.L.orc_apply(self.h, _p(site), int(ev["itrans"]), int(ev["arm"]))
             if rc:
                 raise RuntimeError(f"event {k}: {self.L.orc_last_error(self.h).decode()}")
+
+    def set_temperature_field(self, field, layout="socket"):
+        """SetT of the laser-annealing build (layout 'socket': numpy shape
+        (LenX*LenY, LenZ), z fastest; 'planes': (LenZ, LenY, LenX))."""
+        code = {"socket": 0, "planes": 1}[layout]
+        f = np.ascontiguousarray(field, np.int32)
+        lx, ly, lz = self.dims
+        assert f.size == lx * ly * lz
+        self._check(self.L.orc_set_temperature_field(self.h, _p(f), code), "orc_set_temperature_field")
+
+    def phases(self, layout="socket"):
+        """The phase maps (coor, species) of the laser-annealing build."""
+        code = {"socket": 0, "planes": 1}[layout]
+        lx, ly, lz = self.dims
+        shape = (lx * ly, lz) if code == 0 else (lz, ly, lx)
+        c = np.empty(shape, np.int32)
+        d = np.empty(shape, np.int32)
+        self._check(self.L.orc_phases(self.h, _p(c), _p(d), code), "orc_phases")
+        return c, d
 
     def lattcoo(self):
         lx, ly, lz = self.dims

@@ -1,0 +1,115 @@
+"""GPU: the laser-annealing coupling maps on the device against the oracle.
+
+mskg_set_temperature_field restates SetT (mulskips-source-LaserAnnealing/
+initial/SetT.f:38-48, :56-66) and mskg_phases the phase maps of main.f:796-818
+of that build; the oracle's restatements are pinned in
+tests/test_oracle_la_coupling.py.  On a regression deck grown by replayed
+oracle events, the engine's maps equal the oracle's at every grid point and
+its LattCoo (T bits included) equals the oracle's at every lattice point,
+bit for bit, for both map layouts and for host and device buffers.
+"""
+import numpy as np
+import pytest
+import torch
+
+from mulskips_amd.engine import Engine, EngineError
+from oracle.oracle import Oracle
+from tests.helpers import SMALL, oracle_state, parity_mask, small_deck
+from tests.test_oracle_la_coupling import self_naming_field, to_socket
+
+pytestmark = pytest.mark.gpu
+
+
+def grown(tmp_path, key, n=3000):
+    base, box, lens = SMALL[key]
+    path, d = small_deck(tmp_path, base, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    eng = Engine(box, d.tables())
+    eng.load_state(*oracle_state(o))
+    log = o.step(n)
+    print(f"[{key}] replaying {len(log)} events", flush=True)
+    eng.replay(log)
+    return o, eng, box
+
+
+def same_lattcoo(o, eng, box):
+    lat_o = o.lattcoo()
+    lat_e = eng.store_state(lattcoo=lat_o.copy())[0]
+    m = parity_mask(box)
+    bad = np.nonzero((lat_o != lat_e) & m)
+    assert bad[0].size == 0, f"LattCoo differs at {bad[0].size} points, first (z,y,x)={[int(a[0]) for a in bad]}"
+
+
+@pytest.mark.parametrize("key", ["surface", "finfet", "CVD-Si-SiH2"])
+def test_phases_match_the_oracle(tmp_path, key):
+    o, eng, box = grown(tmp_path, key)
+    for layout in ("socket", "planes"):
+        ce, de = eng.phases(layout)
+        co, do = o.phases(layout)
+        np.testing.assert_array_equal(ce, co)
+        np.testing.assert_array_equal(de, do)
+    cd, dd = eng.phases("socket", device=True)
+    np.testing.assert_array_equal(cd.cpu().numpy(), o.phases("socket")[0])
+    np.testing.assert_array_equal(dd.cpu().numpy(), o.phases("socket")[1])
+    eng.close()
+
+
+@pytest.mark.parametrize("deck,mode,box,lens", [("Si-SL", 0, (96, 96, 144), (72, 120, 120)),
+                                                 ("surface", 1, (96, 96, 144), (72, 72, 72))])
+def test_phases_of_the_device_builders(tmp_path, deck, mode, box, lens):
+    """SetSi fills its wall planes at every grid point, on and off the
+    diamond super-lattice; SetSiC3C has no walls.  The device-built slab's
+    phase maps equal the oracle builder's everywhere, and survive a state
+    copy into another engine."""
+    path, d = small_deck(tmp_path, deck, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    eng = Engine(box, d.tables())
+    eng.build_slab(mode, lens[0] - lens[0] % 24)
+    co, do = o.phases("socket")
+    assert ((co == 10).sum() > 0) == (mode == 0)
+    ce, de = eng.phases("socket")
+    np.testing.assert_array_equal(ce, co)
+    np.testing.assert_array_equal(de, do)
+    other = Engine(box, d.tables())
+    other.copy_state_from(eng)
+    np.testing.assert_array_equal(other.phases("socket")[0], co)
+    other.close()
+    eng.close()
+
+
+@pytest.mark.parametrize("key", ["surface", "finfet"])
+def test_temperature_field_matches_the_oracle(tmp_path, key):
+    o, eng, box = grown(tmp_path, key)
+    lx, ly, lz = box
+    t = self_naming_field(box) + 1000
+    t[3, 5, :] = 10           # wall marker: the point keeps its temperature
+    t[4, 6, 8] = 8192 + 300   # low 13 bits
+    t[6, 2, 2] = -1
+    for layout, f in (("planes", t), ("socket", to_socket(t[::-1].copy()))):
+        o.set_temperature_field(f, layout)
+        eng.set_temperature_field(f, layout)
+        same_lattcoo(o, eng, box)
+    # a device map, read in place
+    t2 = np.ascontiguousarray(t[:, ::-1, :]) + 7
+    o.set_temperature_field(t2, "planes")
+    eng.set_temperature_field(torch.from_numpy(t2).cuda(), "planes")
+    same_lattcoo(o, eng, box)
+    # the maps do not change what the phase maps read
+    np.testing.assert_array_equal(eng.phases("planes")[0], o.phases("planes")[0])
+    eng.close()
+
+
+def test_strict_temperature_rejects_and_keeps(tmp_path):
+    o, eng, box = grown(tmp_path, "surface", n=500)
+    lx, ly, lz = box
+    full = np.full((lz + 1, ly, lx), 1700, np.int32)
+    eng.set_temperature(full)
+    before = eng.store_state()[0]
+    bad = full.copy()
+    bad[10, 4, 4] = 9000  # a lattice point (all even): beyond 13 bits
+    with pytest.raises(EngineError, match="13-bit"):
+        eng.set_temperature(bad)
+    np.testing.assert_array_equal(eng.store_state()[0], before)
+    eng.close()
