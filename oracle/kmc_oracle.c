@@ -1851,6 +1851,21 @@ int orc_set_second_factor(orc_sim *s, const double *f2) {
 /* Get_Prob for every MC particle (Get_Prob.f:54-331 with its side effects:
  * an adsorption site redraws its species), so that every stored rate and
  * class key comes from the current state. */
+/* Laser-annealing rates (Get_Prob.f:127-149): the tables hold energies,
+ * and each rate follows from the site's temperature bits.  The reference
+ * enters this mode for InitSt 'LA', whose CAD-file initialisation is not
+ * restated here; this switch puts a deck of another InitSt, written with
+ * energies in its tables, into the same mode.  Rates are re-evaluated by
+ * orc_refresh_all once the temperatures are set. */
+int orc_set_la(orc_sim *s, double tm, double p0, double sigma, double tflex) {
+  s->la_mode = 1;
+  s->tm = tm;
+  s->p0 = p0;
+  s->sigma = sigma;
+  s->tflex = tflex;
+  return 0;
+}
+
 int orc_refresh_all(orc_sim *s) {
   if (setjmp(s->jb)) { s->have_jb = 0; return 2; }
   s->have_jb = 1;

@@ -30,7 +30,8 @@
  *     subtractive-generator identity checked in the tests.
  *   - LA-mode rates (Get_Prob.f:127-149) are pinned by the closed form only
  *     (the reference LA fixture is stale against the current source):
- *     "parity pinned by closed form, not by a reference run".
+ *     "parity pinned by closed form, not by a reference run".  orc_set_la
+ *     enters that mode for a deck of another InitSt.
  *   - The engine options with no reference counterpart: the second-neighbour
  *     factor by a brute-force count of occupied shell points on a grown
  *     surface (rates with the factor / rates without = f2[count]), the
@@ -126,6 +127,9 @@ int orc_set_second_factor(orc_sim *s, const double *f2);
 
 /* Get_Prob for every MC particle, storing rate, event and rate-class key
  * (an adsorption site redraws its species, consuming ran3). */
+/* Laser-annealing rates (Get_Prob.f:127-149) for a deck whose tables hold
+ * energies: (Tm, P0, sigma, Tflex) as the LA start.dat gives them. */
+int orc_set_la(orc_sim *s, double tm, double p0, double sigma, double tflex);
 int orc_refresh_all(orc_sim *s);
 
 /* Trajectory-likelihood statistics by brute force (the check for the

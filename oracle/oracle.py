@@ -46,6 +46,7 @@ def _load():
         L.orc_set_frozen.argtypes = [P, I]
         L.orc_set_second_factor.argtypes = [P, P]
         L.orc_refresh_all.argtypes = [P]
+        L.orc_set_la.argtypes = [P, D, D, D, D]
         L.orc_run.argtypes = [P, I64, D, P]
         L.orc_adjoint_stats.argtypes = [P, P, I64, P, P, P, P]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
@@ -152,6 +153,10 @@ class Oracle:
         if rc == 2:
             self._check(rc, "orc_run")
         return done.value, rc
+
+    def set_la(self, tm, p0, sigma, tflex):
+        """Laser-annealing rates from the T bits; the tables hold energies."""
+        self._check(self.L.orc_set_la(self.h, float(tm), float(p0), float(sigma), float(tflex)), "orc_set_la")
 
     def refresh_all(self):
         """Get_Prob for every MC particle (fresh rates and class keys)."""

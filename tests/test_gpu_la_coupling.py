@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 import torch
 
+from mulskips_amd import deck as deckmod
 from mulskips_amd.engine import Engine, EngineError
 from oracle.oracle import Oracle
 from tests.helpers import SMALL, oracle_state, parity_mask, small_deck
@@ -98,6 +99,65 @@ def test_temperature_field_matches_the_oracle(tmp_path, key):
     same_lattcoo(o, eng, box)
     # the maps do not change what the phase maps read
     np.testing.assert_array_equal(eng.phases("planes")[0], o.phases("planes")[0])
+    eng.close()
+
+
+LA = dict(tm=1687.0, p0=1e13, sigma=50.0, tflex=1650.0)
+
+
+def la_deck(tmp_path):
+    """The Si-SL regression deck, small box, with energies in its tables
+    (as an LA start.dat holds them): melting 0.6 eV per bond, solidification
+    0.55 eV per bond of the site."""
+    base, box, lens = SMALL["Si-SL"]
+    _, d = small_deck(tmp_path, base, lens)
+    d.evap = [(sp, digits, 0.6 * int(digits)) for sp, digits, _ in d.evap]
+    d.depo = [(sp, c, 0.55 * c) for sp, c, _ in d.depo]
+    path = str(tmp_path / "start-la.dat")
+    deckmod.write_start_dat(d, path)
+    return path, d, box
+
+
+def la_field(box, shift):
+    lx, ly, lz = box
+    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+    return (1500 + 2 * z + (x % 24) + 3 * ((y + shift) % 12)).astype(np.int32)
+
+
+def test_la_cycle_replays_bit_exact(tmp_path):
+    """One laser-annealing cycle twice over: temperature map in (socket
+    layout), rates re-evaluated, 6000 oracle events replayed by the engine.
+    The lattice (T bits included) equals the oracle's bit for bit and every
+    MC particle's LA rate (Get_Prob.f:127-149) is within 1e-6 of the
+    oracle's FP64 rate, after each cycle."""
+    path, d, box = la_deck(tmp_path)
+    o = Oracle(*box)
+    o.load_deck(path)
+    o.set_la(LA["tm"], LA["p0"], LA["sigma"], LA["tflex"])
+    eng = Engine(box, d.tables(), la=LA)
+    eng.load_state(*oracle_state(o))
+    for cycle in range(2):
+        f = to_socket(la_field(box, cycle))
+        o.set_temperature_field(f, "socket")
+        o.refresh_all()
+        eng.set_temperature_field(f, "socket")
+        eng.refresh_rates()
+        log = o.step(6000)
+        print(f"[LA cycle {cycle}] replaying {len(log)} events", flush=True)
+        eng.replay(log)
+        same_lattcoo(o, eng, box)
+        mc, _, _ = o.lists()
+        cls, rate = eng.site_rates(mc["xyz"])
+        ev = mc["event"]
+        np.testing.assert_array_equal(np.where(cls == 6, 4, np.where(cls == 2, 1, cls)),
+                                      np.where(ev == 4, 4, np.where(ev == 5, 5, 1)))
+        keep = cls != 6
+        rel = np.abs(rate[keep].astype(np.float64) - mc["prob"][keep]) / mc["prob"][keep]
+        assert rel.max() <= 1e-6, rel.max()
+        c_e, d_e = eng.phases("socket")
+        c_o, d_o = o.phases("socket")
+        np.testing.assert_array_equal(c_e, c_o)
+        np.testing.assert_array_equal(d_e, d_o)
     eng.close()
 
 
