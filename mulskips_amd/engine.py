@@ -210,7 +210,11 @@ class Engine:
             self._check(self.L.mskg_phases(self.h, C.c_void_p(out[0].data_ptr()), C.c_void_p(out[1].data_ptr()),
                                            code, 1))
             return out[0], out[1]
-        out = [np.empty(shape, np.int32) for _ in range(2)]
+        # page-locked host buffers (torch's pinned allocator caches them):
+        # the device-to-host copies run at PCIe speed instead of staging
+        # through pageable memory; the arrays keep their buffers alive
+        torch = _torch()
+        out = [torch.empty(shape, dtype=torch.int32, pin_memory=True).numpy() for _ in range(2)]
         self._check(self.L.mskg_phases(self.h, ptr(out[0]), ptr(out[1]), code, 0))
         return out[0], out[1]
 
