@@ -132,6 +132,24 @@ int mskg_build_alloy_slab(mskg_engine *e, int32_t len, double fraction);
  * are not re-evaluated: call mskg_refresh_rates. */
 int mskg_set_temperature(mskg_engine *e, const int32_t *temp_k);
 
+/* The laser-annealing build's rate form (mulskips-source-LaserAnnealing/
+ * mcevent/Get_Prob.f:140-153), for an engine created with la_mode: per-species
+ * P0, Tm, sigma and Tflex (its start.dat's Tm(:NCrystal) ... lines), and
+ * solidification scaled by CountLiquid(k) / SUM(CountLiquid), CountLiquid =
+ * InitCountCrystal - CountCrystal (its main.f:595-637).  init_count and count
+ * are InitCountCrystal and the current CountCrystal.  For SiGe (list_crystal
+ * 14, 32) a deposition of a species whose liquid is used up deposits the
+ * other one (main.f:676-681).  CountCrystal follows the events: exactly, event
+ * by event, in mskg_replay (the reference refreshes CountLiquid once per
+ * iteration); once per sweep in mskg_run, whose passes read the counts of the
+ * sweep's start.  The log of a run records the species deposited.  Loading
+ * or building a new state returns to the main build's form; mskg_run_serial
+ * does not take this form.  Call after mskg_load_state / mskg_build_slab. */
+int mskg_set_la_variant(mskg_engine *e, const double p0[3], const double tm[3], const double sigma[3],
+                        const double tflex[3], const int64_t init_count[3], const int64_t count[3]);
+/* CountCrystal and CountLiquid as the engine holds them now (either may be null). */
+int mskg_la_counts(mskg_engine *e, int64_t count[3], int64_t liquid[3]);
+
 /* Layouts of the whole-grid int32 maps the laser-annealing build exchanges
  * with its FEM driver (mulskips-source-LaserAnnealing/modules/defsystem.f:88-91,
  * main.f:211-334):

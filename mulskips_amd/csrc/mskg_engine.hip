@@ -77,6 +77,12 @@ struct RunState {
   int err;
   int err_site[3];
   int stop;
+  // the laser-annealing build's crystal counts (la_variant): CountCrystal,
+  // InitCountCrystal, the CountLiquid the rates read (refreshed per event in
+  // a replay, per sweep in the parallel engine) and a sweep's changes
+  int la_variant;
+  long long cc[3], cinit[3], cl[3];
+  unsigned long long dcc[3];
 };
 
 struct PassParams {
@@ -275,7 +281,12 @@ __global__ void __launch_bounds__(WAVE) k_replay(Lattice L, const mskg_event *lo
     if (lane == 0) {
       int slot = ev.itrans <= 3 ? ev.itrans - 1 : (ev.itrans == 4 ? 3 : (ev.itrans == 5 ? 4 : ev.itrans - 1));
       if (slot >= 0 && slot < 8) st->counts[slot]++;
+      if (LA && L.cl && res.species) {  // CountCrystal, then the next iteration's CountLiquid
+        st->cc[res.species - 1] += ev.itrans <= 3 ? 1 : -1;
+        for (int q = 0; q < 3; q++) st->cl[q] = st->cinit[q] - st->cc[q];
+      }
     }
+    __syncthreads();  // the next event's rates read the counts
   }
   if (lane == 0) {
     st->events += (unsigned long long)n;
@@ -418,6 +429,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   __shared__ uint8_t s_tc[MAX_TOUCHED];
   __shared__ int s_found[MAX_TOUCHED];
   __shared__ unsigned long long s_cnt[8];  // events by type (lane 0)
+  __shared__ long long s_dcc[3];           // crystal count changes (laser-annealing build's form)
   __shared__ EvScratch sc;
 
   RunState *st = P.st;
@@ -483,6 +495,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   double tloc = 0.0;
   unsigned long long nev = 0;
   if (lane < 8) s_cnt[lane] = 0;
+  if (lane < 3) s_dcc[lane] = 0;
   U4 ctr;
   ctr.x = (uint32_t)t;
   ctr.y = (uint32_t)st->sweeps;
@@ -538,7 +551,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
       Choice ch;
       ch.arm = -1;
       ch.u_arm = u01_32(d.w);
-      const int itrans = pick_itrans<LA, COV>(L, s, lc[sel], u01_32(d.z));
+      const int itrans = la_swap<LA>(L, pick_itrans<LA, COV>(L, s, lc[sel], u01_32(d.z)));
       const int coor0 = P.log ? s_coor(L.site[s.i]) : 0;
       __syncthreads();  // every lane has read the site before it changes
       EvResult res;
@@ -562,6 +575,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
         }
         int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
         if (slot >= 0 && slot < 8) s_cnt[slot]++;
+        if (LA && L.cl && res.species) s_dcc[res.species - 1] += itrans <= 3 ? 1 : -1;
         for (int q = 0; q < sc.ntouched; q++) s_tp[q] = sc.touched[q];
         s_nt = sc.ntouched;
         s_stop = err;
@@ -655,6 +669,9 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
     L.tl_own[t] = nout;
     L.tile_r[t] = R;
     slot_add(st, t, R, (unsigned long long)nout, nev, s_cnt);
+    if (LA && L.cl)
+      for (int k = 0; k < 3; k++)
+        if (s_dcc[k]) atomicAdd(&st->dcc[k], (unsigned long long)s_dcc[k]);
     if (nout) {
       if (g.tz < L.win[4 * P.colour + 2]) atomic_min_i32(&L.win[4 * P.colour + 2], g.tz);
       if (g.tz > L.win[4 * P.colour + 3]) atomic_max_i32(&L.win[4 * P.colour + 3], g.tz);
@@ -719,6 +736,12 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
   const long long nmc = (long long)sn[0];
   st->events += sev[0];
   for (int q = 0; q < 8; q++) st->counts[q] += scnt[q][0];
+  if (st->la_variant)  // CountLiquid for the next sweep (main.f:637 of the laser-annealing build)
+    for (int k = 0; k < 3; k++) {
+      st->cc[k] += (long long)st->dcc[k];
+      st->dcc[k] = 0ull;
+      st->cl[k] = st->cinit[k] - st->cc[k];
+    }
   // the passes of the next sweep visit the layers that held candidates
   for (int c = 0; c < 8; c++) {
     win[4 * c] = win[4 * c + 2];
@@ -1533,6 +1556,7 @@ static Lattice make_lattice(mskg_engine *e) {
   L.ntx = e->ntx;
   L.nty = e->nty;
   L.key = e->track ? e->d_key : nullptr;
+  L.cl = e->tab.la_variant ? (const long long *)((const char *)e->d_st + offsetof(RunState, cl)) : nullptr;
   return L;
 }
 
@@ -1567,6 +1591,7 @@ static const char *err_name(int c) {
     case ERR_TEMPERATURE: return "zero temperature in laser-annealing mode";
     case ERR_TILE_LIST: return "a tile's candidate list overflowed tile_capacity";
     case ERR_SUMS: return "serial selection found no event inside a positive rate sum";
+    case ERR_LIQUID: return "a species' CountLiquid is negative: its solidification rate would be negative";
     default: return "unknown";
   }
 }
@@ -1591,6 +1616,7 @@ static int word_kind(const mskg_engine *e, int32_t v, int *kind) {
   return 0;
 }
 
+static int la_variant_off(mskg_engine *e);
 static int read_state(mskg_engine *e, RunState &st) {
   HIPCHK(e, hipMemcpyAsync(&st, e->d_st, sizeof st, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
@@ -1801,6 +1827,7 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt, int32_t n_ad, const int
   st.events_per_pass = e->cfg.events_per_pass > 0 ? e->cfg.events_per_pass : 1.0;
   HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
+  if (int rc_la = la_variant_off(e)) return rc_la;  // a new state: the counts are the caller's to give again
   e->loaded = true;
   e->lists_dirty = true;
   e->keys_valid = false;
@@ -1883,6 +1910,59 @@ int mskg_phases(mskg_engine *e, int32_t *coor, int32_t *species, int32_t layout,
     if (species) HIPCHK(e, hipMemcpyAsync(species, out[1], n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
   }
   HIPCHK(e, hipStreamSynchronize(e->stream));
+  return MSKG_OK;
+}
+
+// Back to the main build's laser-annealing form (a new state was loaded).
+static int la_variant_off(mskg_engine *e) {
+  if (!e->tab.la_variant) return MSKG_OK;
+  e->tab.la_variant = 0;
+  HIPCHK(e, hipMemcpyAsync(e->d_tab, &e->tab, sizeof(Tables), hipMemcpyHostToDevice, e->stream));
+  return MSKG_OK;
+}
+
+int mskg_set_la_variant(mskg_engine *e, const double p0[3], const double tm[3], const double sigma[3],
+                        const double tflex[3], const int64_t init_count[3], const int64_t count[3]) {
+  if (!e || !p0 || !tm || !sigma || !tflex || !init_count || !count) return MSKG_EINVAL;
+  if (!e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  if (!e->tab.la_mode) return fail_(e, MSKG_EINVAL, "the laser-annealing build's form needs la_mode");
+  for (int k = 0; k < 3; k++)
+    if (k < e->ncrystal && !(sigma[k] != 0.0))
+      return fail_(e, MSKG_EINVAL, "sigma of species %d is zero", k + 1);
+  e->tab.la_variant = 1;
+  e->tab.la_swap = e->ncrystal == 2 && e->cfg.list_crystal[0] == 14 && e->cfg.list_crystal[1] == 32;
+  for (int k = 0; k < 3; k++) {
+    e->tab.la_p0[k] = p0[k];
+    e->tab.la_tm[k] = tm[k];
+    e->tab.la_sigma[k] = k < e->ncrystal ? sigma[k] : 1.0;
+    e->tab.la_tflex[k] = tflex[k];
+  }
+  HIPCHK(e, hipMemcpyAsync(e->d_tab, &e->tab, sizeof(Tables), hipMemcpyHostToDevice, e->stream));
+  RunState st;
+  int rc = read_state(e, st);
+  if (rc) return rc;
+  st.la_variant = 1;
+  for (int k = 0; k < 3; k++) {
+    st.cinit[k] = init_count[k];
+    st.cc[k] = count[k];
+    st.dcc[k] = 0ull;
+    st.cl[k] = st.cinit[k] - st.cc[k];
+  }
+  HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(e, hipStreamSynchronize(e->stream));
+  e->lists_dirty = true;
+  return MSKG_OK;
+}
+
+int mskg_la_counts(mskg_engine *e, int64_t count[3], int64_t liquid[3]) {
+  if (!e || !e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  RunState st;
+  int rc = read_state(e, st);
+  if (rc) return rc;
+  for (int k = 0; k < 3; k++) {
+    if (count) count[k] = st.cc[k] + (long long)st.dcc[k];
+    if (liquid) liquid[k] = st.cinit[k] - (st.cc[k] + (long long)st.dcc[k]);
+  }
   return MSKG_OK;
 }
 
@@ -2207,6 +2287,7 @@ static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction
             nsurf * (t.pd[0 + 3] + t.pd[1 + 3] + t.pd[2 + 3]);
   HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
+  if (int rc_la = la_variant_off(e)) return rc_la;  // a new state: the counts are the caller's to give again
   e->loaded = true;
   e->lists_dirty = true;
   e->keys_valid = false;
@@ -2267,6 +2348,12 @@ int mskg_copy_state(mskg_engine *dst, const mskg_engine *src) {
   s0.clock = st.clock;
   s0.rtot = st.rtot;
   s0.events_per_pass = dst->cfg.events_per_pass > 0 ? dst->cfg.events_per_pass : 1.0;
+  s0.la_variant = dst->tab.la_variant;
+  for (int k = 0; k < 3; k++) {
+    s0.cc[k] = st.cc[k] + (long long)st.dcc[k];
+    s0.cinit[k] = st.cinit[k];
+    s0.cl[k] = s0.cinit[k] - s0.cc[k];
+  }
   HIPCHK(dst, hipMemcpyAsync(dst->d_st, &s0, sizeof s0, hipMemcpyHostToDevice, dst->stream));
   HIPCHK(dst, hipStreamSynchronize(dst->stream));
   dst->loaded = true;
@@ -2351,6 +2438,8 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
     mskg_engine *e = engines[i];
     if (!e || !e->loaded) return fail_(e0, MSKG_EINVAL, "serial run: replica %d has no state loaded", i);
     any_la |= e->tab.la_mode != 0;
+    if (e->tab.la_variant)
+      return fail_(e0, MSKG_EINVAL, "serial run: replica %d uses the laser-annealing build's rate form", i);
     any_cov |= e->cov_model;
     if (max_events[i] < 0) return fail_(e, MSKG_EINVAL, "max_events < 0");
     for (int j = 0; j < i; j++)

@@ -23,7 +23,8 @@ enum : int {
   ERR_CAPACITY = 5,   // tile candidate list overflow
   ERR_TEMPERATURE = 6, // zero temperature in LA mode
   ERR_TILE_LIST = 7,   // a tile's candidate list overflowed
-  ERR_SUMS = 8         // serial selection: the rate sums disagree with the lists
+  ERR_SUMS = 8,        // serial selection: the rate sums disagree with the lists
+  ERR_LIQUID = 9       // negative CountLiquid (laser-annealing build's form)
 };
 
 // Per-tile candidate lists (synchronous-sublattice mode): every MC particle
@@ -48,6 +49,7 @@ struct Lattice {
   int cur_tile;           // the tile whose pass is running: its list lives on chip
   float inv_tile;         // 1 / tile
   int32_t *key;           // may be null: rate-class key of every point (likelihood accounting)
+  const long long *cl;    // may be null: CountLiquid(1..3) the laser-annealing build's rates read
 };
 
 // Rate-class keys for the trajectory likelihood (models without coverage
@@ -198,9 +200,20 @@ MSKG_HD double PEv(const Tables &t, int sp, int a, int b, int c, int d, int e, i
 #define MSKG_KB 0.00008617330350
 // LA = false compiles the laser-annealing branches out (tables with la_mode 0).
 template <bool LA = true>
-MSKG_HD double pd_of(const Tables &t, int sp, int coor, int temperature) {
+MSKG_HD double pd_of(const Tables &t, int sp, int coor, int temperature, const long long *cl = nullptr) {
   double e = t.pd[(sp - 1) + 3 * (coor - 1)];
   if (!LA || !t.la_mode) return e;
+  if (t.la_variant && cl) {
+    // LaserAnnealing Get_Prob.f:148-151, evaluated left to right: 0.5d0 *
+    // P0(k) * FLOAT(CountLiquid(k)) / SUM(CountLiquid) * EXP(..) * (1 + ERF(..)),
+    // FLOAT the REAL(4) conversion.  With no liquid left (the reference
+    // divides by zero) solidification stops.
+    const long long sum = cl[0] + cl[1] + cl[2];
+    if (sum <= 0 || sp > t.ncrystal) return 0.0;  // PD(k,:) is set for k <= NCrystal only
+    return 0.5 * t.la_p0[sp - 1] * (double)(float)(int)cl[sp - 1] / (double)sum *
+           exp(-e / (MSKG_KB * t.la_tm[sp - 1])) * (1.0 + erf((temperature - t.la_tflex[sp - 1]) / t.la_sigma[sp - 1])) *
+           (e != 0.0 ? 1.0 : 0.0);
+  }
   return 0.5 * t.p0 * exp(-e / (MSKG_KB * t.tm)) * (1.0 + erf((temperature - t.tflex) / t.sigma)) *
          (e != 0.0 ? 1.0 : 0.0);
 }
@@ -208,7 +221,18 @@ template <bool LA = true>
 MSKG_HD double pe_of(const Tables &t, int sp, int a, int b, int c, int d, int e, int f, int temperature) {
   double v = PEv(t, sp, a, b, c, d, e, f);
   if (!LA || !t.la_mode) return v;
-  return t.p0 * exp(-v / (MSKG_KB * temperature)) * (v != 0.0 ? 1.0 : 0.0);
+  const double p0 = t.la_variant ? t.la_p0[sp - 1] : t.p0;  // LaserAnnealing Get_Prob.f:144-146
+  return p0 * exp(-v / (MSKG_KB * temperature)) * (v != 0.0 ? 1.0 : 0.0);
+}
+// The laser-annealing build's SiGe rule (its main.f:676-681): a deposition of
+// a species whose liquid is used up (CountCrystal >= InitCountCrystal, i.e.
+// CountLiquid <= 0) deposits the other species.
+template <bool LA = true>
+MSKG_HD int la_swap(const Lattice &L, int itrans) {
+  if (LA && L.cl && L.tab->la_mode && L.tab->la_variant && L.tab->la_swap && (itrans == 1 || itrans == 2) &&
+      L.cl[itrans - 1] <= 0)
+    return itrans % 2 + 1;
+  return itrans;
 }
 
 // Get_Prob, Get_Prob.f:54-331, evaluated for point p whose record is sp,
@@ -281,6 +305,12 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
     return;
   }
   if (kind == KIND_EMPTY) {  // :155-214
+    if (LA && t.la_mode && t.la_variant && L.cl && (L.cl[0] < 0 || L.cl[1] < 0 || L.cl[2] < 0)) {
+      // a negative solidification rate, which a sum tree cannot hold (the
+      // reference's selection goes wrong there, or UpdateTree.f:37-41 stops
+      // on a rate that cancels to zero): the engine stops with an error
+      err = ERR_LIQUID; cls = CLS_NONE; rate = 0.0; return;
+    }
     if (coor >= 1 && coor <= 3) {
       bool ok = true;
       if (coor == 1) {
@@ -288,8 +318,8 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
         ok = (i0 >= 0 && i0 == known_i ? known_coor : s_coor(rd_o(L, i0, ovr))) >= 2;
       }
       if (ok) {
-        double pdep = pd_of<LA>(t, 1, coor, temperature) + pd_of<LA>(t, 2, coor, temperature) +
-                      pd_of<LA>(t, 3, coor, temperature);
+        double pdep = pd_of<LA>(t, 1, coor, temperature, L.cl) + pd_of<LA>(t, 2, coor, temperature, L.cl) +
+                      pd_of<LA>(t, 3, coor, temperature, L.cl);
         double pabs = COV ? t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor] : 0.0;
         cls = CLS_ADSORB;
         rate = pdep + pabs;
@@ -385,6 +415,7 @@ constexpr int MAX_TOUCHED = 17;
 struct EvResult {
   int arm;        // stacking choice made (-1 none)
   int err;
+  int species;    // deposited or evaporated species (1..3), 0 otherwise
 };
 
 MSKG_HD bool put_list(const Lattice &L, Site &s, const P3 &p, const NL &l, EvResult &res) {
@@ -411,6 +442,7 @@ struct EvScratch {
   int found[4];     //   ... and found its reciprocal partner (NBuff) at found_pos[i]
   int found_pos[4][3];
   int seq;          // visit the neighbours in order on lane 0
+  int old_kind;     // evaporation: the species that left the site
   int nvisit;       // neighbours the event visits (desorption at coordination 1: one)
   int abort;
   P3 sec[12];       // second-neighbour refresh: the point refreshed by lane k (i = -1: none)
@@ -728,6 +760,7 @@ __device__ void evap_nbuff_one(const Lattice &L, EvScratch &sc, int i, int carri
 __device__ inline void evap_site(const Lattice &L, const P3 &s, EvScratch &sc) {
   Site ss = rd(L, s.i);
   sc.coor_old = s_coor(ss);
+  sc.old_kind = s_kind(ss);
   s_set_kind(ss, KIND_EMPTY);
   wr(L, s.i, ss);
   sc.nn = list_of(L, s, ss);
@@ -781,8 +814,8 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   Site ss = rd(L, s.i);
   int coor = s_coor(ss);
   int temperature = (LA && t.la_mode) ? (int)ss.temp : 0;
-  double p1 = pd_of<LA>(t, 1, coor, temperature), p2 = pd_of<LA>(t, 2, coor, temperature),
-         p3 = pd_of<LA>(t, 3, coor, temperature);
+  double p1 = pd_of<LA>(t, 1, coor, temperature, L.cl), p2 = pd_of<LA>(t, 2, coor, temperature, L.cl),
+         p3 = pd_of<LA>(t, 3, coor, temperature, L.cl);
   double pdep = p1 + p2 + p3;
   double pabs = COV ? t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor] : 0.0;
   double pcurr = u * (pdep + pabs);
@@ -814,6 +847,7 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
                         EvScratch &sc, int lane) {
   res.err = ERR_NONE;
   res.arm = -1;
+  res.species = 0;
   const bool dep = itrans <= 3, evap = itrans == 4, absorb = itrans >= 6;
   if (lane < 12) {
     sc.sec[lane].i = -1;
@@ -826,6 +860,7 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
     if (dep) dep_site(L, s, itrans, ch, res, sc);
     else if (evap) evap_site(L, s, sc);
     else if (COV) cov_site(L, s, absorb ? itrans - 5 : 0, sc);
+    res.species = dep ? itrans : (evap && kind_species(sc.old_kind) ? sc.old_kind : 0);
   }
   MSKG_SYNC();
   if (sc.abort) {

@@ -81,6 +81,12 @@ struct Tables {
   int ncrystal, ncov;
   int la_mode;          // laser-annealing rates (Get_Prob.f:127-149)
   double p0, tm, sigma, tflex;
+  // the laser-annealing build's form (mulskips-source-LaserAnnealing/mcevent/
+  // Get_Prob.f:140-153): per-species constants and the liquid fraction
+  // CountLiquid(k) / SUM(CountLiquid) on solidification; la_swap: its SiGe
+  // species swap when a species' liquid is used up (main.f:676-681 of that build)
+  int la_variant, la_swap;
+  double la_p0[3], la_tm[3], la_sigma[3], la_tflex[3];
   int frozen_below_z;   // atoms with z below this never evaporate (0: off)
   int use_f2;           // second-neighbour factor on evaporation rates
   double f2[13];        // by the number of occupied second-neighbour positions

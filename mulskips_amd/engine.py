@@ -218,6 +218,25 @@ class Engine:
         self._check(self.L.mskg_phases(self.h, ptr(out[0]), ptr(out[1]), code, 0))
         return out[0], out[1]
 
+    def set_la_variant(self, p0, tm, sigma, tflex, init_count, count):
+        """The laser-annealing build's rate form: per-species (up to 3)
+        constants, InitCountCrystal and the current CountCrystal; see
+        mskg_set_la_variant.  Call after load_state / build_slab."""
+        def d3(v):
+            return np.ascontiguousarray((list(v) + [0.0, 0.0, 0.0])[:3], np.float64)
+
+        def i3(v):
+            return np.ascontiguousarray((list(v) + [0, 0, 0])[:3], np.int64)
+        arrs = [d3(p0), d3(tm), d3(sigma), d3(tflex), i3(init_count), i3(count)]
+        self._check(self.L.mskg_set_la_variant(self.h, *[ptr(a) for a in arrs]))
+
+    def la_counts(self):
+        """(CountCrystal, CountLiquid) of the laser-annealing build's form."""
+        c = np.zeros(3, np.int64)
+        q = np.zeros(3, np.int64)
+        self._check(self.L.mskg_la_counts(self.h, ptr(c), ptr(q)))
+        return c, q
+
     def _map_shape(self, code):
         lx, ly, lz = self.box
         return (lx * ly, lz) if code == MSKG_FIELD_SOCKET else (lz, ly, lx)

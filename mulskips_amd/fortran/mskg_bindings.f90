@@ -104,6 +104,20 @@ module mskg_bindings
       integer(c_int32_t), value :: layout, on_device
     end function
 
+    integer(c_int) function mskg_set_la_variant(eng, p0, tm, sigma, tflex, init_count, count) &
+        bind(c, name="mskg_set_la_variant")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: eng
+      real(c_double), intent(in) :: p0(3), tm(3), sigma(3), tflex(3)
+      integer(c_int64_t), intent(in) :: init_count(3), count(3)
+    end function
+
+    integer(c_int) function mskg_la_counts(eng, count, liquid) bind(c, name="mskg_la_counts")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: eng
+      integer(c_int64_t), intent(out) :: count(3), liquid(3)
+    end function
+
     integer(c_int) function mskg_refresh_rates(eng) bind(c, name="mskg_refresh_rates")
       import :: c_int, c_ptr
       type(c_ptr), value :: eng

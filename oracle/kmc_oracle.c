@@ -60,6 +60,11 @@ struct orc_sim {
   int64_t max_iter, out_molmol;
   double max_time, out_time;
   int la_mode; double tm, p0, sigma, tflex;
+  /* the laser-annealing build's form: per-species constants, InitCountCrystal
+   * and the CountLiquid of the current iteration; la_swap its SiGe rule */
+  int la_variant, la_swap;
+  double la_p0[3], la_tm[3], la_sigma[3], la_tflex[3];
+  int32_t count_init[3], count_liquid[3];
   /* ran3 and the RunType 'T' table */
   orc_ran3 rng; int32_t idum;
   double *numbers; int n_numbers; int64_t counter;
@@ -430,13 +435,45 @@ static double la_pd(const orc_sim *s, double e, int temperature) {
 }
 
 typedef struct { double pd[9]; int la_t; } pdtab;
+/* mulskips-source-LaserAnnealing/mcevent/Get_Prob.f:143-151, the entries of
+ * species k: PE = P0(k) exp(-PtransE/(kb T)) [/= 0]; PD = 0.5d0 * P0(k) *
+ * FLOAT(CountLiquid(k)) / SUM(CountLiquid) * EXP(-PtransD/(kb Tm(k))) *
+ * (1 + ERF((T - Tflex(k)) / sigma(k))) [/= 0], left to right, FLOAT a REAL(4)
+ * conversion of the integer count.  SUM(CountLiquid) = 0 (a division by
+ * zero there) gives no solidification here. */
+static double la_pd_variant(const orc_sim *s, int sp, double e, int temperature) {
+  if (sp > s->ncrystal) return 0.0;  /* the build sets PD(k,:) for k <= NCrystal only */
+  const int32_t sum = s->count_liquid[0] + s->count_liquid[1] + s->count_liquid[2];
+  if (sum <= 0) return 0.0;
+  return 0.5 * s->la_p0[sp - 1] * (double)(float)s->count_liquid[sp - 1] / (double)sum *
+         exp(-e / (KB * s->la_tm[sp - 1])) * (1.0 + erf((temperature - s->la_tflex[sp - 1]) / s->la_sigma[sp - 1])) *
+         (e != 0.0 ? 1.0 : 0.0);
+}
 static double pdv(orc_sim *s, int sp, int coor, int temperature) {
   if (!s->la_mode) return PD(s, sp, coor);
+  if (s->la_variant) return la_pd_variant(s, sp, PD(s, sp, coor), temperature);
   return la_pd(s, PD(s, sp, coor), temperature);
 }
 static double pev(orc_sim *s, int sp, int a, int b, int c, int d, int e, int f, int temperature) {
   if (!s->la_mode) return PE(s, sp, a, b, c, d, e, f);
+  if (s->la_variant) {
+    const double v = PE(s, sp, a, b, c, d, e, f);
+    return s->la_p0[sp - 1] * exp(-v / (KB * temperature)) * (v != 0.0 ? 1.0 : 0.0);
+  }
   return la_pe(s, PE(s, sp, a, b, c, d, e, f), temperature);
+}
+/* main.f:637 of the laser-annealing build, at the top of every iteration. */
+static void la_liquid(orc_sim *s) {
+  if (s->la_variant)
+    for (int k = 0; k < 3; k++) s->count_liquid[k] = s->count_init[k] - s->count_crystal[k];
+}
+/* Its main.f:676-681: a deposition of a species whose liquid is used up
+ * deposits the other one (SiGe). */
+static int32_t la_swap(const orc_sim *s, int32_t itrans) {
+  if (s->la_variant && s->la_swap && (itrans == 1 || itrans == 2) &&
+      s->count_crystal[itrans - 1] >= s->count_init[itrans - 1])
+    return itrans % 2 + 1;
+  return itrans;
 }
 
 /* IndexEvent_Abs, Get_Prob.f:22-52 */
@@ -1668,7 +1705,9 @@ int orc_step(orc_sim *s, orc_event *ev) {
   if (s->n_ad <= 20) { s->have_jb = 0; return 1; }
   double r = s->tree[1];
   int32_t itrans;
+  la_liquid(s);
   int32_t ind = do_pick(s, &itrans);
+  itrans = la_swap(s, itrans);
   if (ev) {
     ev->iter = s->iter;
     cp3(ev->site, s->ad[ind].xyz);
@@ -1702,6 +1741,7 @@ int orc_apply(orc_sim *s, const int32_t site[3], int32_t itrans, int32_t arm) {
   if (ind < 1 || ind > s->n_ad || !eq3(s->ad[ind].xyz, site))
     fail(s, "orc_apply: (%d,%d,%d) is not an MC particle", site[0], site[1], site[2]);
   s->force_arm = arm;
+  la_liquid(s);
   do_execute(s, ind, itrans);
   s->force_arm = -1;
   s->iter++;
@@ -1723,7 +1763,9 @@ int orc_main(orc_sim *s, const char *outdir, orc_event *log, int64_t log_cap, in
     if (s->n_ad <= 20) { *stop_reason = 1; break; }
     double r = s->tree[1];
     int32_t itrans;
+    la_liquid(s);
     int32_t ind = do_pick(s, &itrans);
+    itrans = la_swap(s, itrans);
     if (is_iter) {
       if (s->iter % s->out_molmol == 0) {
         framecounter = iterout / s->out_molmol;
@@ -1864,6 +1906,30 @@ int orc_set_la(orc_sim *s, double tm, double p0, double sigma, double tflex) {
   s->sigma = sigma;
   s->tflex = tflex;
   return 0;
+}
+
+int orc_set_la_variant(orc_sim *s, const double p0[3], const double tm[3], const double sigma[3],
+                       const double tflex[3], const int32_t init_count[3], const int32_t count[3], int swap) {
+  s->la_mode = 1;
+  s->la_variant = 1;
+  s->la_swap = swap ? 1 : 0;
+  for (int k = 0; k < 3; k++) {
+    s->la_p0[k] = p0[k];
+    s->la_tm[k] = tm[k];
+    s->la_sigma[k] = sigma[k];
+    s->la_tflex[k] = tflex[k];
+    s->count_init[k] = init_count[k];
+    s->count_crystal[k] = count[k];
+  }
+  la_liquid(s);
+  return 0;
+}
+
+void orc_la_counts(const orc_sim *s, int32_t count[3], int32_t liquid[3]) {
+  for (int k = 0; k < 3; k++) {
+    count[k] = s->count_crystal[k];
+    liquid[k] = s->count_init[k] - s->count_crystal[k];
+  }
 }
 
 int orc_refresh_all(orc_sim *s) {

@@ -130,6 +130,12 @@ int orc_set_second_factor(orc_sim *s, const double *f2);
 /* Laser-annealing rates (Get_Prob.f:127-149) for a deck whose tables hold
  * energies: (Tm, P0, sigma, Tflex) as the LA start.dat gives them. */
 int orc_set_la(orc_sim *s, double tm, double p0, double sigma, double tflex);
+/* The laser-annealing build's form (mulskips-source-LaserAnnealing/mcevent/
+ * Get_Prob.f:140-153, main.f:637 and :676-681): per-species constants,
+ * InitCountCrystal and CountCrystal; swap: its SiGe species rule applies. */
+int orc_set_la_variant(orc_sim *s, const double p0[3], const double tm[3], const double sigma[3],
+                       const double tflex[3], const int32_t init_count[3], const int32_t count[3], int swap);
+void orc_la_counts(const orc_sim *s, int32_t count[3], int32_t liquid[3]);
 int orc_refresh_all(orc_sim *s);
 
 /* Trajectory-likelihood statistics by brute force (the check for the

@@ -47,6 +47,8 @@ def _load():
         L.orc_set_second_factor.argtypes = [P, P]
         L.orc_refresh_all.argtypes = [P]
         L.orc_set_la.argtypes = [P, D, D, D, D]
+        L.orc_set_la_variant.argtypes = [P, P, P, P, P, P, P, I]
+        L.orc_la_counts.argtypes = [P, P, P]
         L.orc_run.argtypes = [P, I64, D, P]
         L.orc_adjoint_stats.argtypes = [P, P, I64, P, P, P, P]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
@@ -157,6 +159,21 @@ class Oracle:
     def set_la(self, tm, p0, sigma, tflex):
         """Laser-annealing rates from the T bits; the tables hold energies."""
         self._check(self.L.orc_set_la(self.h, float(tm), float(p0), float(sigma), float(tflex)), "orc_set_la")
+
+    def set_la_variant(self, p0, tm, sigma, tflex, init_count, count, swap):
+        """The laser-annealing build's rate form (per-species constants,
+        liquid fraction on solidification, SiGe species swap if swap)."""
+        d3 = [np.ascontiguousarray((list(v) + [0.0] * 3)[:3], np.float64) for v in (p0, tm, sigma, tflex)]
+        i3 = [np.ascontiguousarray((list(v) + [0] * 3)[:3], np.int32) for v in (init_count, count)]
+        self._keep_la = d3 + i3
+        self._check(self.L.orc_set_la_variant(self.h, *[_p(a) for a in d3 + i3], 1 if swap else 0),
+                    "orc_set_la_variant")
+
+    def la_counts(self):
+        c = np.zeros(3, np.int32)
+        q = np.zeros(3, np.int32)
+        self.L.orc_la_counts(self.h, _p(c), _p(q))
+        return c, q
 
     def refresh_all(self):
         """Get_Prob for every MC particle (fresh rates and class keys)."""

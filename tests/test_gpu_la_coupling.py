@@ -161,6 +161,79 @@ def test_la_cycle_replays_bit_exact(tmp_path):
     eng.close()
 
 
+VARIANT = dict(p0=(1e13, 2e13), tm=(1687.0, 1600.0), sigma=(50.0, 70.0), tflex=(1650.0, 1580.0))
+
+
+def variant_pair(tmp_path):
+    """Oracle and engine on the SiC deck with energy tables, in the
+    laser-annealing build's form (per-species constants, liquid fraction)."""
+    from tests.test_oracle_la_variant import field as vfield, la_energy_deck, species_counts
+    path, d, box = la_energy_deck(tmp_path, "surface")
+    o = Oracle(*box)
+    o.load_deck(path)
+    eng = Engine(box, d.tables(), la=LA)
+    eng.load_state(*oracle_state(o))
+    f = vfield(box)
+    o.set_temperature_field(f, "socket")
+    eng.set_temperature_field(f, "socket")
+    cc = species_counts(o)
+    init = [cc[0] + 50000, cc[1] + 30000]  # enough liquid that neither runs out
+    o.set_la_variant(VARIANT["p0"], VARIANT["tm"], VARIANT["sigma"], VARIANT["tflex"], init, cc, False)
+    eng.set_la_variant(VARIANT["p0"], VARIANT["tm"], VARIANT["sigma"], VARIANT["tflex"], init, cc)
+    o.refresh_all()
+    eng.refresh_rates()
+    return o, eng, box
+
+
+def test_la_variant_replays_bit_exact(tmp_path):
+    """The laser-annealing build's rate form (mulskips-source-LaserAnnealing/
+    mcevent/Get_Prob.f:140-153): 8000 oracle events replayed by the engine;
+    lattice bit for bit, every rate within 1e-6, and the same CountCrystal /
+    CountLiquid (refreshed per event, as that build's main.f:637 does)."""
+    o, eng, box = variant_pair(tmp_path)
+    log = o.step(8000)
+    print(f"[LA variant] replaying {len(log)} events, deposits by species "
+          f"{np.bincount(log['itrans'], minlength=5)[1:3]}", flush=True)
+    eng.replay(log)
+    same_lattcoo(o, eng, box)
+    mc, _, _ = o.lists()
+    cls, rate = eng.site_rates(mc["xyz"])
+    keep = cls != 6
+    rel = np.abs(rate[keep].astype(np.float64) - mc["prob"][keep]) / mc["prob"][keep]
+    assert rel.max() <= 1e-6, rel.max()
+    c_o, q_o = o.la_counts()
+    c_e, q_e = eng.la_counts()
+    np.testing.assert_array_equal(c_e[:2], c_o[:2])
+    np.testing.assert_array_equal(q_e[:2], q_o[:2])
+    eng.close()
+
+
+def test_la_variant_parallel_run_counts(tmp_path):
+    """mskg_run in the same form: the crystal counts move by the deposits
+    and evaporations the run counts, and CountLiquid stays consistent."""
+    o, eng, box = variant_pair(tmp_path)
+    c0, q0 = eng.la_counts()
+    n, t, stop = eng.run(20000)
+    counts = eng.counters(lists=False)[0]
+    c1, q1 = eng.la_counts()
+    print(f"[LA variant run] {n} events, stop {stop}, counts {c0} -> {c1}", flush=True)
+    assert n >= 20000
+    assert int((c1 - c0).sum()) == int(counts[0] + counts[1] + counts[2] - counts[3])
+    np.testing.assert_array_equal(c1 + q1, c0 + q0)  # InitCountCrystal is fixed
+    assert (q1[:2] >= 0).all()
+    eng.close()
+
+
+def test_la_variant_needs_la_mode(tmp_path):
+    base, box, lens = SMALL["surface"]
+    _, d = small_deck(tmp_path, base, lens)
+    eng = Engine(box, d.tables())
+    eng.build_slab(1, 72)
+    with pytest.raises(EngineError, match="la_mode"):
+        eng.set_la_variant(*VARIANT.values(), [1, 1], [0, 0])
+    eng.close()
+
+
 def test_strict_temperature_rejects_and_keeps(tmp_path):
     o, eng, box = grown(tmp_path, "surface", n=500)
     lx, ly, lz = box
