@@ -147,6 +147,19 @@ int mskg_set_temperature(mskg_engine *e, const int32_t *temp_k);
  * does not take this form.  Call after mskg_load_state / mskg_build_slab. */
 int mskg_set_la_variant(mskg_engine *e, const double p0[3], const double tm[3], const double sigma[3],
                         const double tflex[3], const int64_t init_count[3], const int64_t count[3]);
+/* InitSt 'ST' (mcevent/SetProbabilityFromField.f:21-171): ntemp reference
+ * temperatures (K, increasing) with their rate tables, each laid out as
+ * mskg_config's tables and stored one after another (pe: ntemp x 12288,
+ * pd: ntemp x 9, pabs and pdes: ntemp x 12).  Every rate evaluation takes
+ * the tables at the site's temperature (the T bits) as
+ * InterpProbFromField.f:22-57 interpolates them (Get_Prob.f:77-90); a
+ * temperature outside the references is an error (the reference indexes
+ * out of its arrays there).  The 'ST' run with one temperature throughout
+ * (fixedT, main.f:114) is an ordinary run with the tables interpolated on
+ * the host.  ntemp = 0 switches it off.  Excludes la_mode; rates are left as
+ * they are (mskg_refresh_rates). */
+int mskg_set_st_tables(mskg_engine *e, int32_t ntemp, const int32_t *temps, const double *pe, const double *pd,
+                       const double *pabs, const double *pdes);
 /* CountCrystal and CountLiquid as the engine holds them now (either may be null). */
 int mskg_la_counts(mskg_engine *e, int64_t count[3], int64_t liquid[3]);
 

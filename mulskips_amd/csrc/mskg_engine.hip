@@ -1483,6 +1483,8 @@ struct mskg_engine {
   // planes of the device builders
   uint32_t *d_offwall = nullptr;
   bool offwall_planes = false;
+  // InitSt 'ST' reference tables (one allocation: temperatures, then the tables)
+  void *d_st_tables = nullptr;
   int *d_list_err = nullptr;
   int *d_win = nullptr;
   Geometry *d_geo = nullptr;
@@ -1592,6 +1594,7 @@ static const char *err_name(int c) {
     case ERR_TILE_LIST: return "a tile's candidate list overflowed tile_capacity";
     case ERR_SUMS: return "serial selection found no event inside a positive rate sum";
     case ERR_LIQUID: return "a species' CountLiquid is negative: its solidification rate would be negative";
+    case ERR_ST_RANGE: return "a site's temperature lies outside the InitSt 'ST' reference temperatures";
     default: return "unknown";
   }
 }
@@ -1729,7 +1732,7 @@ void mskg_destroy(mskg_engine *e) {
   if (e->graph_exec) hipGraphExecDestroy(e->graph_exec);
   if (e->graph) hipGraphDestroy(e->graph);
   hipFree(e->d_inlist); hipFree(e->d_tl_idx); hipFree(e->d_tl_n); hipFree(e->d_tl_own); hipFree(e->d_tile_r); hipFree(e->d_list_err);
-  hipFree(e->d_field[0]); hipFree(e->d_field[1]); hipFree(e->d_offwall); hipFree(e->d_win);
+  hipFree(e->d_field[0]); hipFree(e->d_field[1]); hipFree(e->d_offwall); hipFree(e->d_st_tables); hipFree(e->d_win);
   hipFree(e->d_geo); hipFree(e->d_tab); hipFree(e->d_st); hipFree(e->d_err); hipFree(e->d_log);
   hipFree(e->d_tsum); hipFree(e->d_bsum); hipFree(e->d_key); hipFree(e->d_key_seen); hipFree(e->d_rate_seen);
   if (e->ev0) hipEventDestroy(e->ev0);
@@ -1954,6 +1957,47 @@ int mskg_set_la_variant(mskg_engine *e, const double p0[3], const double tm[3], 
   return MSKG_OK;
 }
 
+int mskg_set_st_tables(mskg_engine *e, int32_t ntemp, const int32_t *temps, const double *pe, const double *pd,
+                       const double *pabs, const double *pdes) {
+  if (!e || ntemp < 0 || (ntemp > 0 && (!temps || !pe || !pd || !pabs || !pdes))) return MSKG_EINVAL;
+  if (!e->d_tab) return fail_(e, MSKG_EINVAL, "engine not created");
+  if (ntemp > 0 && e->tab.la_mode) return fail_(e, MSKG_EINVAL, "InitSt 'ST' tables and la_mode exclude each other");
+  for (int k = 0; k < ntemp; k++) {
+    if (temps[k] <= 0 || temps[k] > 8191) return fail_(e, MSKG_EINVAL, "reference temperature %d K out of range", temps[k]);
+    if (k && temps[k] <= temps[k - 1])
+      return fail_(e, MSKG_EINVAL, "reference temperatures must increase (InterpProbFromField.f:39 assumes it)");
+  }
+  HIPCHK(e, hipStreamSynchronize(e->stream));
+  hipFree(e->d_st_tables);
+  e->d_st_tables = nullptr;
+  e->tab.st_n = 0;
+  e->tab.st_temp = nullptr;
+  e->tab.st_pe = e->tab.st_pd = e->tab.st_pabs = e->tab.st_pdes = nullptr;
+  if (ntemp > 0) {
+    const size_t n = (size_t)ntemp, off_pe = 256 * ((n * sizeof(int32_t) + 255) / 256);
+    const size_t npe = n * 3 * 4096, npd = n * 9, nab = n * 12;
+    const size_t bytes = off_pe + (npe + npd + 2 * nab) * sizeof(double);
+    HIPCHK(e, hipMalloc(&e->d_st_tables, bytes));
+    char *base = (char *)e->d_st_tables;
+    double *dpe = (double *)(base + off_pe), *dpd = dpe + npe, *dab = dpd + npd, *dde = dab + nab;
+    HIPCHK(e, hipMemcpyAsync(base, temps, n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(dpe, pe, npe * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(dpd, pd, npd * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(dab, pabs, nab * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(dde, pdes, nab * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    e->tab.st_n = ntemp;
+    e->tab.st_temp = (const int *)base;
+    e->tab.st_pe = dpe;
+    e->tab.st_pd = dpd;
+    e->tab.st_pabs = dab;
+    e->tab.st_pdes = dde;
+  }
+  HIPCHK(e, hipMemcpyAsync(e->d_tab, &e->tab, sizeof(Tables), hipMemcpyHostToDevice, e->stream));
+  HIPCHK(e, hipStreamSynchronize(e->stream));
+  e->lists_dirty = true;
+  return MSKG_OK;
+}
+
 int mskg_la_counts(mskg_engine *e, int64_t count[3], int64_t liquid[3]) {
   if (!e || !e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
   RunState st;
@@ -1977,7 +2021,7 @@ int mskg_refresh_rates(mskg_engine *e) {
   e->lists_dirty = true;
   int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
   HIPCHK(e, hipEventRecord(e->ev0, e->stream));
-  hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
+  hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
   HIPCHK(e, hipGetLastError());
   if (e->track) e->keys_valid = true;
   HIPCHK(e, hipEventRecord(e->ev1, e->stream));
@@ -2005,7 +2049,7 @@ int mskg_replay(mskg_engine *e, const mskg_event *log, int64_t n, int32_t clock_
   Lattice L = make_lattice(e);
   L.inlist = nullptr;
   e->lists_dirty = true;
-  hipLaunchKernelGGL(MSKG_VARIANT(k_replay, e->tab.la_mode, e->cov_model), dim3(1), dim3(WAVE), 0, e->stream, L, d_log, n, clock_mode, e->d_st);
+  hipLaunchKernelGGL(MSKG_VARIANT(k_replay, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(1), dim3(WAVE), 0, e->stream, L, d_log, n, clock_mode, e->d_st);
   hipError_t le = hipGetLastError();
   RunState st;
   int rc = read_state(e, st);
@@ -2080,7 +2124,7 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
       P.key0 = (uint32_t)e->cfg.seed;
       P.key1 = (uint32_t)(e->cfg.seed >> 32);
       P.colour = c;
-      hipLaunchKernelGGL(MSKG_VARIANT(k_pass, e->tab.la_mode, e->cov_model), dim3(grid), dim3(WAVE), shm, e->stream, P);
+      hipLaunchKernelGGL(MSKG_VARIANT(k_pass, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(grid), dim3(WAVE), shm, e->stream, P);
     }
     hipLaunchKernelGGL(k_rsum, dim3(NPART), dim3(256), 0, e->stream, e->d_tile_r, e->ntiles, e->d_st);
     hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err);
@@ -2274,7 +2318,7 @@ static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction
   e->offwall_planes = mode != 1;
   if (e->tab.use_f2 || mode == 2) {  // the surface atoms' rates from their neighbours (species, second shell)
     HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
-    hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, 0.0, e->d_err, 1);
+    hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, 0.0, e->d_err, 1);
     HIPCHK(e, hipGetLastError());
   }
   // the total rate of the initial population seeds the isolated-atom rule
@@ -2437,7 +2481,7 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
   for (int i = 0; i < n; i++) {
     mskg_engine *e = engines[i];
     if (!e || !e->loaded) return fail_(e0, MSKG_EINVAL, "serial run: replica %d has no state loaded", i);
-    any_la |= e->tab.la_mode != 0;
+    any_la |= e->tab.la_mode != 0 || e->tab.st_n > 0;
     if (e->tab.la_variant)
       return fail_(e0, MSKG_EINVAL, "serial run: replica %d uses the laser-annealing build's rate form", i);
     any_cov |= e->cov_model;
@@ -2575,7 +2619,7 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
     int blocks = (int)std::min<int64_t>((np + 255) / 256, 256 * 64);
     HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
     if (fresh) {
-      hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
+      hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
       HIPCHK(e, hipGetLastError());
     }
     HIPCHK(e, hipMemsetAsync(d_cnt, 0, (KEY_N + 1) * sizeof(int), e->stream));

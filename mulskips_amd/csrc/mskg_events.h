@@ -24,7 +24,8 @@ enum : int {
   ERR_TEMPERATURE = 6, // zero temperature in LA mode
   ERR_TILE_LIST = 7,   // a tile's candidate list overflowed
   ERR_SUMS = 8,        // serial selection: the rate sums disagree with the lists
-  ERR_LIQUID = 9       // negative CountLiquid (laser-annealing build's form)
+  ERR_LIQUID = 9,      // negative CountLiquid (laser-annealing build's form)
+  ERR_ST_RANGE = 10    // a temperature outside the interpolation tables (InitSt 'ST')
 };
 
 // Per-tile candidate lists (synchronous-sublattice mode): every MC particle
@@ -235,6 +236,47 @@ MSKG_HD int la_swap(const Lattice &L, int itrans) {
   return itrans;
 }
 
+// InterpProbFromField.f:28-54: the tables at temperature T are reference
+// table iu when T equals its temperature (the first such), else the blend
+// of tables il = iu - 1 and iu, iu the first of the smallest positive
+// (reference temperature - T), with integer weights wl = |T_iu - T| on il
+// and wu = |T_il - T| on iu.  False when T lies outside the references (the
+// reference indexes out of its arrays there).
+struct STW {
+  int il, iu, wl, wu, exact;
+};
+MSKG_HD bool st_weights(const Tables &t, int temperature, STW &w) {
+  int best = -1;
+  w.exact = 0;
+  for (int k = 0; k < t.st_n; k++) {
+    const int diff = t.st_temp[k] - temperature;
+    if (diff == 0) {
+      w.exact = 1;
+      w.il = w.iu = k;
+      w.wl = w.wu = 0;
+      return true;
+    }
+    if (diff > 0 && (best < 0 || diff < t.st_temp[best] - temperature)) best = k;
+  }
+  if (best <= 0) return false;
+  w.iu = best;
+  w.il = best - 1;
+  const int dl = t.st_temp[w.iu] - temperature, du = t.st_temp[w.il] - temperature;
+  w.wl = dl < 0 ? -dl : dl;
+  w.wu = du < 0 ? -du : du;
+  return true;
+}
+// One interpolated table entry: (wl * ref(il) + wu * ref(iu)) / (wl + wu),
+// the integer weights converted as Fortran does.
+MSKG_HD double st_entry(const double *ref, int stride, int idx, const STW &w) {
+  if (w.exact) return ref[(int64_t)w.iu * stride + idx];
+  return ((double)w.wl * ref[(int64_t)w.il * stride + idx] + (double)w.wu * ref[(int64_t)w.iu * stride + idx]) /
+         (double)(w.wl + w.wu);
+}
+MSKG_HD int pe_flat(int sp, int a, int b, int c, int d, int e, int f) {
+  return (sp - 1) + 3 * (a + 4 * (b + 4 * (c + 4 * (d + 4 * (e + 4 * f)))));
+}
+
 // Get_Prob, Get_Prob.f:54-331, evaluated for point p whose record is sp,
 // with the coordination and neighbour list passed explicitly (Evaporation.f:121
 // passes a stale list copy).  rtot is the total rate R used by the
@@ -292,15 +334,22 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   *key = -1;
   int kind = s_kind(sp);
   int temperature = 0;
-  if (LA && t.la_mode) {
+  STW w;
+  const bool st = LA && t.st_n > 0;  // Get_Prob.f:77-90: tables at the site's temperature
+  if (LA && (t.la_mode || st)) {
     temperature = (int)sp.temp;
     if (temperature == 0) { err = ERR_TEMPERATURE; cls = CLS_NONE; rate = 0.0; return; }
+    if (st && !st_weights(t, temperature, w)) { err = ERR_ST_RANGE; cls = CLS_NONE; rate = 0.0; return; }
   }
   if (!COV && kind >= KIND_COV0) { err = ERR_COVERAGE; cls = CLS_NONE; rate = 0.0; return; }
   int slot = COV ? kind_cov_slot(kind) : 0;
   if (slot) {  // desorption, :93-123
     if (slot > t.ncov) { err = ERR_COVERAGE; cls = CLS_NONE; rate = 0.0; return; }
-    if (coor <= 3) { cls = CLS_DESORB; rate = t.pdes[(slot - 1) + 3 * coor]; *key = KEY_UNSUPPORTED; }
+    if (coor <= 3) {
+      cls = CLS_DESORB;
+      rate = st ? st_entry(t.st_pdes, 12, (slot - 1) + 3 * coor, w) : t.pdes[(slot - 1) + 3 * coor];
+      *key = KEY_UNSUPPORTED;
+    }
     else { cls = CLS_NONE; rate = 0.0; }
     return;
   }
@@ -318,12 +367,21 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
         ok = (i0 >= 0 && i0 == known_i ? known_coor : s_coor(rd_o(L, i0, ovr))) >= 2;
       }
       if (ok) {
-        double pdep = pd_of<LA>(t, 1, coor, temperature, L.cl) + pd_of<LA>(t, 2, coor, temperature, L.cl) +
-                      pd_of<LA>(t, 3, coor, temperature, L.cl);
-        double pabs = COV ? t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor] : 0.0;
+        double pdep, pabs;
+        if (st) {
+          pdep = st_entry(t.st_pd, 9, 3 * (coor - 1), w) + st_entry(t.st_pd, 9, 1 + 3 * (coor - 1), w) +
+                 st_entry(t.st_pd, 9, 2 + 3 * (coor - 1), w);
+          pabs = COV ? st_entry(t.st_pabs, 12, 3 * coor, w) + st_entry(t.st_pabs, 12, 1 + 3 * coor, w) +
+                           st_entry(t.st_pabs, 12, 2 + 3 * coor, w)
+                     : 0.0;
+        } else {
+          pdep = pd_of<LA>(t, 1, coor, temperature, L.cl) + pd_of<LA>(t, 2, coor, temperature, L.cl) +
+                 pd_of<LA>(t, 3, coor, temperature, L.cl);
+          pabs = COV ? t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor] : 0.0;
+        }
         cls = CLS_ADSORB;
         rate = pdep + pabs;
-        *key = t.ncov ? KEY_UNSUPPORTED : KEY_ADS + coor - 1;
+        *key = (t.ncov || st) ? KEY_UNSUPPORTED : KEY_ADS + coor - 1;
       } else {
         cls = CLS_PLACEHOLDER;
         rate = (double)1.e-8f;  // REAL(4) literal, Get_Prob.f:183
@@ -391,7 +449,8 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
   } else {
     int c1 = t.ncov >= 1 ? covnn[0] : 0, c2 = t.ncov >= 2 ? covnn[1] : 0, c3 = t.ncov >= 3 ? covnn[2] : 0;
     cls = CLS_EVAP;
-    rate = pe_of<LA>(t, kind, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
+    rate = st ? st_entry(t.st_pe, 3 * 4096, pe_flat(kind, n_si + n_wall, n_c, n_b, c1, c2, c3), w)
+              : pe_of<LA>(t, kind, n_si + n_wall, n_c, n_b, c1, c2, c3, temperature);
     int n2 = 0;
     if (t.use_f2) {
       for (int k = 0; k < 12; k++) {
@@ -400,7 +459,8 @@ MSKG_HD void get_prob(const Lattice &L, double rtot, const P3 &p, const Site &sp
       }
       rate *= t.f2[n2];
     }
-    *key = (t.ncov || t.la_mode) ? KEY_UNSUPPORTED : (kind - 1) + 3 * (n_si + n_wall + 4 * (n_c + 4 * n_b)) + 192 * n2;
+    *key = (t.ncov || t.la_mode || st) ? KEY_UNSUPPORTED
+                                       : (kind - 1) + 3 * (n_si + n_wall + 4 * (n_c + 4 * n_b)) + 192 * n2;
   }
 }
 
@@ -814,10 +874,23 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   Site ss = rd(L, s.i);
   int coor = s_coor(ss);
   int temperature = (LA && t.la_mode) ? (int)ss.temp : 0;
-  double p1 = pd_of<LA>(t, 1, coor, temperature, L.cl), p2 = pd_of<LA>(t, 2, coor, temperature, L.cl),
-         p3 = pd_of<LA>(t, 3, coor, temperature, L.cl);
+  STW w;
+  // InitSt 'ST': the tables at the site's temperature (its rate was evaluated
+  // with them, so the temperature lies inside the references)
+  const bool st = LA && t.st_n > 0 && st_weights(t, (int)ss.temp, w);
+  auto pabs_of = [&](int i) { return st ? st_entry(t.st_pabs, 12, i + 3 * coor, w) : t.pabs[i + 3 * coor]; };
+  double p1, p2, p3;
+  if (st) {
+    p1 = st_entry(t.st_pd, 9, 3 * (coor - 1), w);
+    p2 = st_entry(t.st_pd, 9, 1 + 3 * (coor - 1), w);
+    p3 = st_entry(t.st_pd, 9, 2 + 3 * (coor - 1), w);
+  } else {
+    p1 = pd_of<LA>(t, 1, coor, temperature, L.cl);
+    p2 = pd_of<LA>(t, 2, coor, temperature, L.cl);
+    p3 = pd_of<LA>(t, 3, coor, temperature, L.cl);
+  }
   double pdep = p1 + p2 + p3;
-  double pabs = COV ? t.pabs[0 + 3 * coor] + t.pabs[1 + 3 * coor] + t.pabs[2 + 3 * coor] : 0.0;
+  double pabs = COV ? pabs_of(0) + pabs_of(1) + pabs_of(2) : 0.0;
   double pcurr = u * (pdep + pabs);
   if (!COV || pcurr <= pdep) {
     if (pcurr <= p1) return 1;
@@ -828,7 +901,7 @@ MSKG_HD int pick_itrans(const Lattice &L, const P3 &s, int cls, double u) {
   int r = 6;
   double lo = pdep;
   for (int i = 1; i <= t.ncov; i++) {
-    double hi = lo + t.pabs[(i - 1) + 3 * coor];
+    double hi = lo + pabs_of(i - 1);
     if (pcurr > lo && pcurr <= hi) r = 5 + i;
     lo = hi;
   }

@@ -87,6 +87,13 @@ struct Tables {
   // species swap when a species' liquid is used up (main.f:676-681 of that build)
   int la_variant, la_swap;
   double la_p0[3], la_tm[3], la_sigma[3], la_tflex[3];
+  // InitSt 'ST' (SetProbabilityFromField.f:21-171, InterpProbFromField.f:22-57):
+  // st_n reference temperatures st_temp[] with their tables (device memory,
+  // table-major: st_pe[k * 12288 + i] ...), interpolated at each site's
+  // temperature; 0: off
+  int st_n;
+  const int *st_temp;
+  const double *st_pe, *st_pd, *st_pabs, *st_pdes;
   int frozen_below_z;   // atoms with z below this never evaporate (0: off)
   int use_f2;           // second-neighbour factor on evaporation rates
   double f2[13];        // by the number of occupied second-neighbour positions

@@ -230,6 +230,21 @@ class Engine:
         arrs = [d3(p0), d3(tm), d3(sigma), d3(tflex), i3(init_count), i3(count)]
         self._check(self.L.mskg_set_la_variant(self.h, *[ptr(a) for a in arrs]))
 
+    def set_st_tables(self, temps, tables):
+        """InitSt 'ST': reference temperatures (increasing) and one table dict
+        per temperature (ptrans_e, ptrans_d, ptrans_abs, ptrans_des as for
+        the constructor); rates interpolate them at each site's temperature.
+        An empty list switches it off."""
+        n = len(temps)
+        t = np.ascontiguousarray(temps, np.int32) if n else np.zeros(1, np.int32)
+
+        def stack(key, size):
+            if not n:
+                return np.zeros(size, np.float64)
+            return np.ascontiguousarray(np.concatenate([np.asarray(tb[key], np.float64).reshape(-1) for tb in tables]))
+        arrs = [stack("ptrans_e", 12288), stack("ptrans_d", 9), stack("ptrans_abs", 12), stack("ptrans_des", 12)]
+        self._check(self.L.mskg_set_st_tables(self.h, n, ptr(t), *[ptr(a) for a in arrs]))
+
     def la_counts(self):
         """(CountCrystal, CountLiquid) of the laser-annealing build's form."""
         c = np.zeros(3, np.int64)

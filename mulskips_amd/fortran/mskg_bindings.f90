@@ -112,6 +112,15 @@ module mskg_bindings
       integer(c_int64_t), intent(in) :: init_count(3), count(3)
     end function
 
+    ! pe(12288, ntemp), pd(9, ntemp), pabs(12, ntemp), pdes(12, ntemp): the
+    ! refPtrans* arrays of SetProbabilityFromField.f with the table index last
+    integer(c_int) function mskg_set_st_tables(eng, ntemp, temps, pe, pd, pabs, pdes) &
+        bind(c, name="mskg_set_st_tables")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: eng, temps, pe, pd, pabs, pdes
+      integer(c_int32_t), value :: ntemp
+    end function
+
     integer(c_int) function mskg_la_counts(eng, count, liquid) bind(c, name="mskg_la_counts")
       import :: c_int, c_int64_t, c_ptr
       type(c_ptr), value :: eng

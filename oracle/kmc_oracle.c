@@ -65,6 +65,11 @@ struct orc_sim {
   int la_variant, la_swap;
   double la_p0[3], la_tm[3], la_sigma[3], la_tflex[3];
   int32_t count_init[3], count_liquid[3];
+  /* InitSt 'ST': reference temperatures and tables (refPtrans*), the working
+   * tables pe/pd/pabs/pdes interpolated at each Get_Prob */
+  int st_n;
+  int32_t *st_temp;
+  double *st_pe, *st_pd, *st_pabs, *st_pdes;
   /* ran3 and the RunType 'T' table */
   orc_ran3 rng; int32_t idum;
   double *numbers; int n_numbers; int64_t counter;
@@ -462,6 +467,38 @@ static double pev(orc_sim *s, int sp, int a, int b, int c, int d, int e, int f, 
   }
   return la_pe(s, PE(s, sp, a, b, c, d, e, f), temperature);
 }
+/* InterpProbFromField.f:22-57: the working tables at temperature T, from
+ * the reference tables (temperatures increasing).  Outside the references
+ * the reference indexes out of its arrays; here that fails. */
+static void st_interp(orc_sim *s, int temperature) {
+  int iu = -1;
+  for (int k = 0; k < s->st_n; k++)
+    if (s->st_temp[k] - temperature == 0) { iu = k; break; }  /* FINDLOC(diff, 0) */
+  if (iu >= 0) {
+    memcpy(s->pe, s->st_pe + (size_t)iu * 3 * 4096, sizeof s->pe);
+    memcpy(s->pd, s->st_pd + (size_t)iu * 9, sizeof s->pd);
+    memcpy(s->pabs, s->st_pabs + (size_t)iu * 12, sizeof s->pabs);
+    memcpy(s->pdes, s->st_pdes + (size_t)iu * 12, sizeof s->pdes);
+    return;
+  }
+  for (int k = 0; k < s->st_n; k++) {  /* MINLOC(diff, MASK = diff > 0) */
+    const int d = s->st_temp[k] - temperature;
+    if (d > 0 && (iu < 0 || d < s->st_temp[iu] - temperature)) iu = k;
+  }
+  if (iu <= 0) fail(s, "temperature %d K outside the 'ST' reference temperatures", temperature);
+  const int il = iu - 1;
+  const int wl = abs(s->st_temp[iu] - temperature), wu = abs(s->st_temp[il] - temperature);
+#define ST_BLEND(dst, ref, n)                                                                          \
+  for (int i = 0; i < (n); i++)                                                                        \
+    (dst)[i] = ((double)wl * (ref)[(size_t)il * (n) + i] + (double)wu * (ref)[(size_t)iu * (n) + i]) / \
+               (double)(wl + wu);
+  ST_BLEND(s->pe, s->st_pe, 3 * 4096)
+  ST_BLEND(s->pd, s->st_pd, 9)
+  ST_BLEND(s->pabs, s->st_pabs, 12)
+  ST_BLEND(s->pdes, s->st_pdes, 12)
+#undef ST_BLEND
+}
+
 /* main.f:637 of the laser-annealing build, at the top of every iteration. */
 static void la_liquid(orc_sim *s) {
   if (s->la_variant)
@@ -538,6 +575,11 @@ static void get_prob_impl(orc_sim *s, int occ, int nsic, int covind, int coor, c
   if (s->la_mode) {
     temperature = ibits(LC(s, site), POS_T, LEN_T);
     if (temperature == 0) fail(s, "Temperature is zero in get_prob()");
+  }
+  if (s->st_n) { /* Get_Prob.f:77-90 */
+    const int t_st = ibits(LC(s, site), POS_T, LEN_T);
+    if (t_st == 0) fail(s, "Temperature is zero in get_prob()");
+    st_interp(s, t_st);
   }
   if (covind > 0 && covind < WALL) { /* desorption, :93-123 */
     if (occ != 0) fail(s, "ERROR Covind!=0 with Occ!=0 in get_prob");
@@ -661,6 +703,11 @@ static void get_prob_ini(orc_sim *s, int occ, int nsic, int coor, const int32_t 
   if (s->la_mode) {
     temperature = ibits(LC(s, site), POS_T, LEN_T);
     if (temperature == 0) fail(s, "Temperature is zero in get_prob_ini()");
+  }
+  if (s->st_n) { /* Get_Prob_Ini.f:42-55 */
+    const int t_st = ibits(LC(s, site), POS_T, LEN_T);
+    if (t_st == 0) fail(s, "Temperature is zero in get_prob_ini()");
+    st_interp(s, t_st);
   }
   if (occ == 0) {
     if (coor == 1 || coor == 2 || coor == 3) {
@@ -877,7 +924,7 @@ static void evaporation(orc_sim *s, int32_t ind_atom) {
         if (recip(nnn, site)) {
           erase_mc(s, p);
           int32_t *wp = LCp(s, p);
-          if (s->la_mode || (s->init_st[0] == 'S' && s->init_st[1] == 'T')) {
+          if (s->la_mode || s->st_n || (s->init_st[0] == 'S' && s->init_st[1] == 'T')) {
             int t = ibits(*wp, POS_T, LEN_T);
             *wp = 0;
             mvbits(t, LEN_T, wp, POS_T);
@@ -1829,6 +1876,7 @@ void orc_destroy(orc_sim *s) {
   if (!s) return;
   free(s->latt); free(s->lind); free(s->ad); free(s->at); free(s->vd); free(s->abuf);
   free(s->tree); free(s->numbers);
+  free(s->st_temp); free(s->st_pe); free(s->st_pd); free(s->st_pabs); free(s->st_pdes);
   free(s);
 }
 
@@ -1905,6 +1953,27 @@ int orc_set_la(orc_sim *s, double tm, double p0, double sigma, double tflex) {
   s->p0 = p0;
   s->sigma = sigma;
   s->tflex = tflex;
+  return 0;
+}
+
+int orc_set_st_tables(orc_sim *s, int ntemp, const int32_t *temps, const double *pe, const double *pd,
+                      const double *pabs, const double *pdes) {
+  free(s->st_temp); free(s->st_pe); free(s->st_pd); free(s->st_pabs); free(s->st_pdes);
+  s->st_temp = NULL; s->st_pe = s->st_pd = s->st_pabs = s->st_pdes = NULL;
+  s->st_n = 0;
+  if (ntemp <= 0) return 0;
+  s->st_temp = malloc((size_t)ntemp * sizeof(int32_t));
+  s->st_pe = malloc((size_t)ntemp * 3 * 4096 * sizeof(double));
+  s->st_pd = malloc((size_t)ntemp * 9 * sizeof(double));
+  s->st_pabs = malloc((size_t)ntemp * 12 * sizeof(double));
+  s->st_pdes = malloc((size_t)ntemp * 12 * sizeof(double));
+  if (!s->st_temp || !s->st_pe || !s->st_pd || !s->st_pabs || !s->st_pdes) return -1;
+  memcpy(s->st_temp, temps, (size_t)ntemp * sizeof(int32_t));
+  memcpy(s->st_pe, pe, (size_t)ntemp * 3 * 4096 * sizeof(double));
+  memcpy(s->st_pd, pd, (size_t)ntemp * 9 * sizeof(double));
+  memcpy(s->st_pabs, pabs, (size_t)ntemp * 12 * sizeof(double));
+  memcpy(s->st_pdes, pdes, (size_t)ntemp * 12 * sizeof(double));
+  s->st_n = ntemp;
   return 0;
 }
 

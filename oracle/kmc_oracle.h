@@ -39,6 +39,10 @@
  *     the likelihood statistics (orc_adjoint_stats) by the identity with the
  *     log-likelihood evaluated directly from the sum tree and by central
  *     finite differences of it (tests/test_adjoint.py).
+ *   - InitSt 'ST' interpolation (orc_set_st_tables) by known answers: a
+ *     temperature at a reference takes that table, and between two it
+ *     takes the weighted blend the integer weights give, checked against
+ *     decks whose tables are those blends (tests/test_oracle_st.py).
  *   - The laser-annealing coupling maps (orc_set_temperature_field,
  *     orc_phases) by known answers built from SetT's and main.f's rules:
  *     a map whose value names its own grid point (a transposed or shifted
@@ -136,6 +140,13 @@ int orc_set_la(orc_sim *s, double tm, double p0, double sigma, double tflex);
 int orc_set_la_variant(orc_sim *s, const double p0[3], const double tm[3], const double sigma[3],
                        const double tflex[3], const int32_t init_count[3], const int32_t count[3], int swap);
 void orc_la_counts(const orc_sim *s, int32_t count[3], int32_t liquid[3]);
+/* InitSt 'ST' (mcevent/SetProbabilityFromField.f, InterpProbFromField.f):
+ * ntemp increasing reference temperatures and their tables, laid out as the
+ * oracle's own (pe 12288, pd 9, pabs 12, pdes 12 per temperature, one after
+ * another); every Get_Prob interpolates the working tables at the site's
+ * temperature.  ntemp = 0 switches it off. */
+int orc_set_st_tables(orc_sim *s, int ntemp, const int32_t *temps, const double *pe, const double *pd,
+                      const double *pabs, const double *pdes);
 int orc_refresh_all(orc_sim *s);
 
 /* Trajectory-likelihood statistics by brute force (the check for the

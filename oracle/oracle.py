@@ -49,6 +49,7 @@ def _load():
         L.orc_set_la.argtypes = [P, D, D, D, D]
         L.orc_set_la_variant.argtypes = [P, P, P, P, P, P, P, I]
         L.orc_la_counts.argtypes = [P, P, P]
+        L.orc_set_st_tables.argtypes = [P, I, P, P, P, P, P]
         L.orc_run.argtypes = [P, I64, D, P]
         L.orc_adjoint_stats.argtypes = [P, P, I64, P, P, P, P]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
@@ -168,6 +169,14 @@ class Oracle:
         self._keep_la = d3 + i3
         self._check(self.L.orc_set_la_variant(self.h, *[_p(a) for a in d3 + i3], 1 if swap else 0),
                     "orc_set_la_variant")
+
+    def set_st_tables(self, temps, tables):
+        """InitSt 'ST': reference temperatures and one table dict each."""
+        n = len(temps)
+        self._keep_st = [np.ascontiguousarray(temps, np.int32)] + [
+            np.ascontiguousarray(np.concatenate([np.asarray(tb[k], np.float64).reshape(-1) for tb in tables]))
+            for k in ("ptrans_e", "ptrans_d", "ptrans_abs", "ptrans_des")]
+        self._check(self.L.orc_set_st_tables(self.h, n, *[_p(a) for a in self._keep_st]), "orc_set_st_tables")
 
     def la_counts(self):
         c = np.zeros(3, np.int32)

@@ -234,6 +234,46 @@ def test_la_variant_needs_la_mode(tmp_path):
     eng.close()
 
 
+def test_st_tables_replay_bit_exact(tmp_path):
+    """InitSt 'ST' (InterpProbFromField.f:22-57 at each site's temperature):
+    reference tables at 1400 / 1500 / 1600 K, a temperature field spread
+    over 1410-1590 K; 6000 oracle events replayed by the engine bit for bit,
+    every rate within 1e-6 of the oracle's, then a parallel run in the same
+    mode."""
+    from tests.test_oracle_st import TEMPS, scaled
+    base, box, lens = SMALL["Si-SL"]
+    path, d = small_deck(tmp_path, base, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    eng = Engine(box, d.tables())
+    eng.load_state(*oracle_state(o))
+    lx, ly, lz = box
+    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+    f = (1410 + (z * 7 + x * 3 + y) % 181).astype(np.int32)
+    tab = d.tables()
+    refs = [scaled(tab, 0.5), tab, scaled(tab, 2.0)]
+    o.set_temperature_field(f, "planes")
+    eng.set_temperature_field(f, "planes")
+    o.set_st_tables(TEMPS, refs)
+    eng.set_st_tables(TEMPS, refs)
+    o.refresh_all()
+    eng.refresh_rates()
+    log = o.step(6000)
+    print(f"[ST] replaying {len(log)} events", flush=True)
+    eng.replay(log)
+    same_lattcoo(o, eng, box)
+    mc, _, _ = o.lists()
+    cls, rate = eng.site_rates(mc["xyz"])
+    keep = cls != 6
+    rel = np.abs(rate[keep].astype(np.float64) - mc["prob"][keep]) / mc["prob"][keep]
+    assert rel.max() <= 1e-6, rel.max()
+    n, t, stop = eng.run(20000)
+    print(f"[ST run] {n} events, stop {stop}", flush=True)
+    # the deck evaporates at these temperatures: the reference's few-MC stop may come first
+    assert n > 5000 and stop in (0, 1)
+    eng.close()
+
+
 def test_strict_temperature_rejects_and_keeps(tmp_path):
     o, eng, box = grown(tmp_path, "surface", n=500)
     lx, ly, lz = box
