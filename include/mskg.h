@@ -237,6 +237,13 @@ int mskg_site_rates(mskg_engine *e, int64_t n, const int32_t *xyz, int32_t *cls,
 int mskg_counters(mskg_engine *e, int64_t counts[8], double *kmc_time,
                   int64_t *n_adatom, int64_t *n_atoms, int64_t *n_voids);
 
+/* Evaporations by species (1..3) and desorptions by coverage species
+ * (ListCov 1..3) since the state was loaded: with the depositions and
+ * absorptions of mskg_counters, the host keeps CountCrystal and CountCov
+ * (main.f:341,356, Evaporation.f:41, desorption.f:43-47), which
+ * WriteMolMolSource.f:43-54 prints. */
+int mskg_species_counts(mskg_engine *e, int64_t evaporated[3], int64_t desorbed[3]);
+
 /* Timing of the last mskg_run / mskg_refresh_rates (device milliseconds,
  * measured with HIP events on the engine's stream) and the number of lattice
  * points the last rate sweep read. */

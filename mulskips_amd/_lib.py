@@ -20,7 +20,7 @@ EXPORTS = [
     "mskg_counters", "mskg_last_timing", "mskg_geometry_find_nn", "mskg_geometry_find_sn",
     "mskg_build_slab", "mskg_set_event_log", "mskg_get_event_log", "mskg_run_serial", "mskg_adjoint", "mskg_build_alloy_slab",
     "mskg_track_classes", "mskg_copy_state", "mskg_frame", "mskg_set_temperature_field", "mskg_phases",
-    "mskg_set_la_variant", "mskg_la_counts", "mskg_set_st_tables",
+    "mskg_set_la_variant", "mskg_la_counts", "mskg_set_st_tables", "mskg_species_counts",
 ]
 MSKG_FIELD_SOCKET, MSKG_FIELD_PLANES = 0, 1  # layouts of the laser-annealing maps
 
@@ -87,6 +87,7 @@ def lib():
     L.mskg_set_la_variant.argtypes = [P, P, P, P, P, P, P]
     L.mskg_la_counts.argtypes = [P, P, P]
     L.mskg_set_st_tables.argtypes = [P, I32, P, P, P, P, P]
+    L.mskg_species_counts.argtypes = [P, P, P]
     L.mskg_adjoint.argtypes = [P, I32, P, P, P, P, P, P]
     L.mskg_geometry_find_nn.argtypes = [P, P, P, I32, I32, P]
     L.mskg_geometry_find_sn.argtypes = [P, P, P, I32, I32, P, P]

@@ -46,9 +46,13 @@ constexpr int WAVE = 64;
 constexpr int NSLOT = 128;
 // Device-side stop at the run's event or time limit (reported as MSKG_STOP_LIMIT).
 constexpr int STOP_AT_LIMIT = 100;
+// Event counters: 0-2 depositions by species, 3 evaporations, 4 desorptions,
+// 5-7 absorptions by coverage species, 8-10 evaporations by species, 11-13
+// desorptions by coverage species (CountCrystal and CountCov follow from them).
+constexpr int NCOUNT = 14;
 struct Slot {
   unsigned long long rmax;  // bit pattern of the largest (non-negative) tile rate
-  unsigned long long n, tiles, events, counts[8];
+  unsigned long long n, tiles, events, counts[NCOUNT];
 };
 // The FP64 total rate is summed without atomics, in a fixed order: every
 // tile keeps its total (Lattice::tile_r), NPART blocks each sum a fixed
@@ -65,7 +69,7 @@ struct RunState {
   double events_per_pass;
   int tau_policy;        // 0: tau = k / max tile rate; 1: tau = k * active tiles / total rate
   unsigned long long events;
-  unsigned long long counts[8];
+  unsigned long long counts[NCOUNT];
   unsigned long long sweeps;      // sweep counter (RNG stream position)
   unsigned long long sweeps_run;  // sweeps that advanced the clock
   long long n_mc;                 // MC particles at the end of the last sweep
@@ -122,7 +126,7 @@ __device__ inline void slot_add(RunState *st, int b, double r, unsigned long lon
   if (nev) {
     atomicAdd(&sl.events, nev);
     if (cnt)
-      for (int q = 0; q < 8; q++)
+      for (int q = 0; q < NCOUNT; q++)
         if (cnt[q]) atomicAdd(&sl.counts[q], cnt[q]);
   }
 }
@@ -281,7 +285,8 @@ __global__ void __launch_bounds__(WAVE) k_replay(Lattice L, const mskg_event *lo
     if (lane == 0) {
       int slot = ev.itrans <= 3 ? ev.itrans - 1 : (ev.itrans == 4 ? 3 : (ev.itrans == 5 ? 4 : ev.itrans - 1));
       if (slot >= 0 && slot < 8) st->counts[slot]++;
-      if (LA && L.cl && res.species) {  // CountCrystal, then the next iteration's CountLiquid
+      if (res.species && (ev.itrans == 4 || ev.itrans == 5)) st->counts[(ev.itrans == 4 ? 7 : 10) + res.species]++;
+      if (LA && L.cl && res.species && ev.itrans <= 4) {  // CountCrystal, then the next iteration's CountLiquid
         st->cc[res.species - 1] += ev.itrans <= 3 ? 1 : -1;
         for (int q = 0; q < 3; q++) st->cl[q] = st->cinit[q] - st->cc[q];
       }
@@ -428,7 +433,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   __shared__ float s_tr[MAX_TOUCHED];
   __shared__ uint8_t s_tc[MAX_TOUCHED];
   __shared__ int s_found[MAX_TOUCHED];
-  __shared__ unsigned long long s_cnt[8];  // events by type (lane 0)
+  __shared__ unsigned long long s_cnt[NCOUNT];  // events by type (lane 0)
   __shared__ long long s_dcc[3];           // crystal count changes (laser-annealing build's form)
   __shared__ EvScratch sc;
 
@@ -494,7 +499,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   const double rtot = st->rtot;
   double tloc = 0.0;
   unsigned long long nev = 0;
-  if (lane < 8) s_cnt[lane] = 0;
+  if (lane < NCOUNT) s_cnt[lane] = 0;
   if (lane < 3) s_dcc[lane] = 0;
   U4 ctr;
   ctr.x = (uint32_t)t;
@@ -575,7 +580,8 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
         }
         int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
         if (slot >= 0 && slot < 8) s_cnt[slot]++;
-        if (LA && L.cl && res.species) s_dcc[res.species - 1] += itrans <= 3 ? 1 : -1;
+        if (res.species && (itrans == 4 || itrans == 5)) s_cnt[(itrans == 4 ? 7 : 10) + res.species]++;
+        if (LA && L.cl && res.species && itrans <= 4) s_dcc[res.species - 1] += itrans <= 3 ? 1 : -1;
         for (int q = 0; q < sc.ntouched; q++) s_tp[q] = sc.touched[q];
         s_nt = sc.ntouched;
         s_stop = err;
@@ -710,14 +716,14 @@ __global__ void __launch_bounds__(256) k_rsum(const double *tile_r, int ntiles, 
 // early stop applies when at most 20 MC particles remain (main.f:307-310).
 __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, const int *list_err) {
   __shared__ double srsum[NSLOT];
-  __shared__ unsigned long long srmax[NSLOT], sn[NSLOT], stl[NSLOT], sev[NSLOT], scnt[8][NSLOT];
+  __shared__ unsigned long long srmax[NSLOT], sn[NSLOT], stl[NSLOT], sev[NSLOT], scnt[NCOUNT][NSLOT];
   const int i = threadIdx.x;
   Slot &sl = st->slot[i];
   srsum[i] = st->rpart[i] + st->rpart[i + NSLOT];
   srmax[i] = sl.rmax; sn[i] = sl.n; stl[i] = sl.tiles; sev[i] = sl.events;
-  for (int q = 0; q < 8; q++) scnt[q][i] = sl.counts[q];
+  for (int q = 0; q < NCOUNT; q++) scnt[q][i] = sl.counts[q];
   sl.rmax = 0ull; sl.n = 0ull; sl.tiles = 0ull; sl.events = 0ull;
-  for (int q = 0; q < 8; q++) sl.counts[q] = 0ull;
+  for (int q = 0; q < NCOUNT; q++) sl.counts[q] = 0ull;
   __syncthreads();
   for (int off = NSLOT / 2; off > 0; off >>= 1) {
     if (i < off) {
@@ -726,7 +732,7 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
       sn[i] += sn[i + off];
       stl[i] += stl[i + off];
       sev[i] += sev[i + off];
-      for (int q = 0; q < 8; q++) scnt[q][i] += scnt[q][i + off];
+      for (int q = 0; q < NCOUNT; q++) scnt[q][i] += scnt[q][i + off];
     }
     __syncthreads();
   }
@@ -735,7 +741,7 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
   const double rsum = srsum[0];
   const long long nmc = (long long)sn[0];
   st->events += sev[0];
-  for (int q = 0; q < 8; q++) st->counts[q] += scnt[q][0];
+  for (int q = 0; q < NCOUNT; q++) st->counts[q] += scnt[q][0];
   if (st->la_variant)  // CountLiquid for the next sweep (main.f:637 of the laser-annealing build)
     for (int k = 0; k < 3; k++) {
       st->cc[k] += (long long)st->dcc[k];
@@ -932,7 +938,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
   __shared__ EvScratch sc;
   __shared__ int s_tiles[MAX_TOUCHED], s_ntl, s_err;
   __shared__ double s_tsum[SERIAL_LDS_TILES], s_bsum[SERIAL_LDS_TILES / WAVE];
-  __shared__ unsigned long long s_cnt[8];
+  __shared__ unsigned long long s_cnt[NCOUNT];
   const int lane = threadIdx.x;
   const SerialDesc D = descs[blockIdx.x];
   Lattice L = D.L;
@@ -942,7 +948,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
   const bool on_chip = D.ntiles <= SERIAL_LDS_TILES;
   double *tsum = on_chip ? s_tsum : D.tsum;
   double *bsum = on_chip ? s_bsum : D.bsum;
-  if (lane < 8) s_cnt[lane] = 0;
+  if (lane < NCOUNT) s_cnt[lane] = 0;
   // initial sums
   for (int t = 0; t < D.ntiles; t++) {
     const double ts = tile_sum_compact(L, t, lane);
@@ -1032,6 +1038,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
       }
       const int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
       if (slot >= 0 && slot < 8) s_cnt[slot]++;
+      if (res.species && (itrans == 4 || itrans == 5)) s_cnt[(itrans == 4 ? 7 : 10) + res.species]++;
       int nt = 0;
       for (int q = 0; q < sc.ntouched; q++) {
         if (sc.touched[q].i < 0) continue;
@@ -1068,7 +1075,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
     st->events = events;
     st->log_n += nev;
     st->stop = stop;
-    for (int q = 0; q < 8; q++) st->counts[q] += s_cnt[q];
+    for (int q = 0; q < NCOUNT; q++) st->counts[q] += s_cnt[q];
     if (err) st->err = err;
   }
 }
@@ -2255,6 +2262,18 @@ int mskg_site_rates(mskg_engine *e, int64_t n, const int32_t *xyz, int32_t *cls_
     if (p < 0) return fail_(e, MSKG_EINVAL, "site %lld (%d,%d,%d) is not a lattice point", (long long)i, s[0], s[1], s[2]);
     cls_out[i] = cls[(size_t)p];
     rate_out[i] = rate[(size_t)p];
+  }
+  return MSKG_OK;
+}
+
+int mskg_species_counts(mskg_engine *e, int64_t evaporated[3], int64_t desorbed[3]) {
+  if (!e || !e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  RunState st;
+  int rc = read_state(e, st);
+  if (rc) return rc;
+  for (int k = 0; k < 3; k++) {
+    if (evaporated) evaporated[k] = (int64_t)st.counts[8 + k];
+    if (desorbed) desorbed[k] = (int64_t)st.counts[11 + k];
   }
   return MSKG_OK;
 }

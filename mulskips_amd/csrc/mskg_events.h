@@ -475,7 +475,7 @@ constexpr int MAX_TOUCHED = 17;
 struct EvResult {
   int arm;        // stacking choice made (-1 none)
   int err;
-  int species;    // deposited or evaporated species (1..3), 0 otherwise
+  int species;    // deposited or evaporated species (1..3), desorbed coverage slot (1..3), 0 otherwise
 };
 
 MSKG_HD bool put_list(const Lattice &L, Site &s, const P3 &p, const NL &l, EvResult &res) {
@@ -834,6 +834,7 @@ __device__ inline void evap_site(const Lattice &L, const P3 &s, EvScratch &sc) {
 __device__ inline void cov_site(const Lattice &L, const P3 &s, int slot, EvScratch &sc) {
   Site ss = rd(L, s.i);
   sc.coor_old = s_coor(ss);
+  sc.old_kind = s_kind(ss);
   s_set_kind(ss, slot ? KIND_COV0 + slot - 1 : KIND_EMPTY);
   wr(L, s.i, ss);
   sc.nn = list_of(L, s, ss);
@@ -933,7 +934,10 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
     if (dep) dep_site(L, s, itrans, ch, res, sc);
     else if (evap) evap_site(L, s, sc);
     else if (COV) cov_site(L, s, absorb ? itrans - 5 : 0, sc);
-    res.species = dep ? itrans : (evap && kind_species(sc.old_kind) ? sc.old_kind : 0);
+    // the species deposited or evaporated, or the coverage slot desorbed
+    res.species = dep ? itrans
+                      : (evap ? (kind_species(sc.old_kind) ? sc.old_kind : 0)
+                              : (itrans == 5 ? kind_cov_slot(sc.old_kind) : 0));
   }
   MSKG_SYNC();
   if (sc.abort) {

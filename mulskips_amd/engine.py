@@ -245,6 +245,14 @@ class Engine:
         arrs = [stack("ptrans_e", 12288), stack("ptrans_d", 9), stack("ptrans_abs", 12), stack("ptrans_des", 12)]
         self._check(self.L.mskg_set_st_tables(self.h, n, ptr(t), *[ptr(a) for a in arrs]))
 
+    def species_counts(self):
+        """(evaporations by species, desorptions by coverage species) since
+        the state was loaded."""
+        ev = np.zeros(3, np.int64)
+        de = np.zeros(3, np.int64)
+        self._check(self.L.mskg_species_counts(self.h, ptr(ev), ptr(de)))
+        return ev, de
+
     def la_counts(self):
         """(CountCrystal, CountLiquid) of the laser-annealing build's form."""
         c = np.zeros(3, np.int64)

@@ -121,6 +121,12 @@ module mskg_bindings
       integer(c_int32_t), value :: ntemp
     end function
 
+    integer(c_int) function mskg_species_counts(eng, evaporated, desorbed) bind(c, name="mskg_species_counts")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: eng
+      integer(c_int64_t), intent(out) :: evaporated(3), desorbed(3)
+    end function
+
     integer(c_int) function mskg_la_counts(eng, count, liquid) bind(c, name="mskg_la_counts")
       import :: c_int, c_int64_t, c_ptr
       type(c_ptr), value :: eng

@@ -3,7 +3,8 @@ reference's regression fixtures byte for byte).
 
 * replay: an oracle event log applied by the engine reproduces the oracle's
   lattice bit for bit (occupancy, species, coverage, coordination, every
-  neighbour list), its event-class counts and its FP64 clock, and every MC
+  neighbour list), its event-class counts, its CountCrystal / CountCov
+  changes and its FP64 clock, and every MC
   particle's FP32 rate is within 1e-6 relative of the oracle's FP64
   ProbTrans (the isolated-atom rate 1e5*R is compared by class only);
 * rate sweep: mskg_refresh_rates reproduces the oracle's Get_Prob on the
@@ -74,6 +75,7 @@ def test_replay_bit_exact(tmp_path, key):
     lat0, mc0, bulk0, void0 = oracle_state(o)
     eng = engine_for(o, d, box)
     eng.load_state(lat0, mc0, bulk0, void0)
+    cc0, cv0 = o.count_crystal()
     n = 20000
     log = o.step(n)
     print(f"[{key}] replaying {len(log)} events", flush=True)
@@ -86,6 +88,11 @@ def test_replay_bit_exact(tmp_path, key):
               np.sum(it == 6), np.sum(it == 7), np.sum(it == 8)]
     assert list(counts) == expect
     assert clock == o.time(), (clock, o.time())
+    # CountCrystal / CountCov (main.f:341,356, Evaporation.f:41, desorption.f:43-47)
+    cc1, cv1 = o.count_crystal()
+    ev_sp, de_sp = eng.species_counts()
+    np.testing.assert_array_equal(np.asarray(counts[0:3]) - ev_sp, cc1 - cc0)
+    np.testing.assert_array_equal(np.asarray(counts[5:8]) - de_sp, cv1 - cv0)
     print(f"[{key}] ok, max rel rate err {rel:.2e}", flush=True)
     eng.close()
 
