@@ -241,7 +241,8 @@ int mskg_counters(mskg_engine *e, int64_t counts[8], double *kmc_time,
  * (ListCov 1..3) since the state was loaded: with the depositions and
  * absorptions of mskg_counters, the host keeps CountCrystal and CountCov
  * (main.f:341,356, Evaporation.f:41, desorption.f:43-47), which
- * WriteMolMolSource.f:43-54 prints. */
+ * WriteMolMolSource.f:43-54 prints.  They follow from a census of the
+ * lattice (one pass over the stored points) against the one taken at load. */
 int mskg_species_counts(mskg_engine *e, int64_t evaporated[3], int64_t desorbed[3]);
 
 /* Timing of the last mskg_run / mskg_refresh_rates (device milliseconds,

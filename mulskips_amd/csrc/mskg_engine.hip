@@ -46,13 +46,9 @@ constexpr int WAVE = 64;
 constexpr int NSLOT = 128;
 // Device-side stop at the run's event or time limit (reported as MSKG_STOP_LIMIT).
 constexpr int STOP_AT_LIMIT = 100;
-// Event counters: 0-2 depositions by species, 3 evaporations, 4 desorptions,
-// 5-7 absorptions by coverage species, 8-10 evaporations by species, 11-13
-// desorptions by coverage species (CountCrystal and CountCov follow from them).
-constexpr int NCOUNT = 14;
 struct Slot {
   unsigned long long rmax;  // bit pattern of the largest (non-negative) tile rate
-  unsigned long long n, tiles, events, counts[NCOUNT];
+  unsigned long long n, tiles, events, counts[8];
 };
 // The FP64 total rate is summed without atomics, in a fixed order: every
 // tile keeps its total (Lattice::tile_r), NPART blocks each sum a fixed
@@ -69,7 +65,7 @@ struct RunState {
   double events_per_pass;
   int tau_policy;        // 0: tau = k / max tile rate; 1: tau = k * active tiles / total rate
   unsigned long long events;
-  unsigned long long counts[NCOUNT];
+  unsigned long long counts[8];
   unsigned long long sweeps;      // sweep counter (RNG stream position)
   unsigned long long sweeps_run;  // sweeps that advanced the clock
   long long n_mc;                 // MC particles at the end of the last sweep
@@ -87,6 +83,7 @@ struct RunState {
   int la_variant;
   long long cc[3], cinit[3], cl[3];
   unsigned long long dcc[3];
+  unsigned long long desorbed[3];  // desorptions by coverage slot (CountCov)
 };
 
 struct PassParams {
@@ -126,7 +123,7 @@ __device__ inline void slot_add(RunState *st, int b, double r, unsigned long lon
   if (nev) {
     atomicAdd(&sl.events, nev);
     if (cnt)
-      for (int q = 0; q < NCOUNT; q++)
+      for (int q = 0; q < 8; q++)
         if (cnt[q]) atomicAdd(&sl.counts[q], cnt[q]);
   }
 }
@@ -285,7 +282,7 @@ __global__ void __launch_bounds__(WAVE) k_replay(Lattice L, const mskg_event *lo
     if (lane == 0) {
       int slot = ev.itrans <= 3 ? ev.itrans - 1 : (ev.itrans == 4 ? 3 : (ev.itrans == 5 ? 4 : ev.itrans - 1));
       if (slot >= 0 && slot < 8) st->counts[slot]++;
-      if (res.species && (ev.itrans == 4 || ev.itrans == 5)) st->counts[(ev.itrans == 4 ? 7 : 10) + res.species]++;
+      if (COV && ev.itrans == 5 && res.species) st->desorbed[res.species - 1]++;
       if (LA && L.cl && res.species && ev.itrans <= 4) {  // CountCrystal, then the next iteration's CountLiquid
         st->cc[res.species - 1] += ev.itrans <= 3 ? 1 : -1;
         for (int q = 0; q < 3; q++) st->cl[q] = st->cinit[q] - st->cc[q];
@@ -433,8 +430,9 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   __shared__ float s_tr[MAX_TOUCHED];
   __shared__ uint8_t s_tc[MAX_TOUCHED];
   __shared__ int s_found[MAX_TOUCHED];
-  __shared__ unsigned long long s_cnt[NCOUNT];  // events by type (lane 0)
+  __shared__ unsigned long long s_cnt[8];  // events by type (lane 0)
   __shared__ long long s_dcc[3];           // crystal count changes (laser-annealing build's form)
+  __shared__ unsigned long long s_des[3];  // desorptions by coverage slot
   __shared__ EvScratch sc;
 
   RunState *st = P.st;
@@ -499,8 +497,9 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   const double rtot = st->rtot;
   double tloc = 0.0;
   unsigned long long nev = 0;
-  if (lane < NCOUNT) s_cnt[lane] = 0;
+  if (lane < 8) s_cnt[lane] = 0;
   if (lane < 3) s_dcc[lane] = 0;
+  if (COV && lane < 3) s_des[lane] = 0;
   U4 ctr;
   ctr.x = (uint32_t)t;
   ctr.y = (uint32_t)st->sweeps;
@@ -580,8 +579,8 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
         }
         int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
         if (slot >= 0 && slot < 8) s_cnt[slot]++;
-        if (res.species && (itrans == 4 || itrans == 5)) s_cnt[(itrans == 4 ? 7 : 10) + res.species]++;
         if (LA && L.cl && res.species && itrans <= 4) s_dcc[res.species - 1] += itrans <= 3 ? 1 : -1;
+        if (COV && itrans == 5 && res.species) s_des[res.species - 1]++;
         for (int q = 0; q < sc.ntouched; q++) s_tp[q] = sc.touched[q];
         s_nt = sc.ntouched;
         s_stop = err;
@@ -678,6 +677,9 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
     if (LA && L.cl)
       for (int k = 0; k < 3; k++)
         if (s_dcc[k]) atomicAdd(&st->dcc[k], (unsigned long long)s_dcc[k]);
+    if (COV)
+      for (int k = 0; k < 3; k++)
+        if (s_des[k]) atomicAdd(&st->desorbed[k], s_des[k]);
     if (nout) {
       if (g.tz < L.win[4 * P.colour + 2]) atomic_min_i32(&L.win[4 * P.colour + 2], g.tz);
       if (g.tz > L.win[4 * P.colour + 3]) atomic_max_i32(&L.win[4 * P.colour + 3], g.tz);
@@ -716,14 +718,14 @@ __global__ void __launch_bounds__(256) k_rsum(const double *tile_r, int ntiles, 
 // early stop applies when at most 20 MC particles remain (main.f:307-310).
 __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, const int *list_err) {
   __shared__ double srsum[NSLOT];
-  __shared__ unsigned long long srmax[NSLOT], sn[NSLOT], stl[NSLOT], sev[NSLOT], scnt[NCOUNT][NSLOT];
+  __shared__ unsigned long long srmax[NSLOT], sn[NSLOT], stl[NSLOT], sev[NSLOT], scnt[8][NSLOT];
   const int i = threadIdx.x;
   Slot &sl = st->slot[i];
   srsum[i] = st->rpart[i] + st->rpart[i + NSLOT];
   srmax[i] = sl.rmax; sn[i] = sl.n; stl[i] = sl.tiles; sev[i] = sl.events;
-  for (int q = 0; q < NCOUNT; q++) scnt[q][i] = sl.counts[q];
+  for (int q = 0; q < 8; q++) scnt[q][i] = sl.counts[q];
   sl.rmax = 0ull; sl.n = 0ull; sl.tiles = 0ull; sl.events = 0ull;
-  for (int q = 0; q < NCOUNT; q++) sl.counts[q] = 0ull;
+  for (int q = 0; q < 8; q++) sl.counts[q] = 0ull;
   __syncthreads();
   for (int off = NSLOT / 2; off > 0; off >>= 1) {
     if (i < off) {
@@ -732,7 +734,7 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
       sn[i] += sn[i + off];
       stl[i] += stl[i + off];
       sev[i] += sev[i + off];
-      for (int q = 0; q < NCOUNT; q++) scnt[q][i] += scnt[q][i + off];
+      for (int q = 0; q < 8; q++) scnt[q][i] += scnt[q][i + off];
     }
     __syncthreads();
   }
@@ -741,7 +743,7 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
   const double rsum = srsum[0];
   const long long nmc = (long long)sn[0];
   st->events += sev[0];
-  for (int q = 0; q < NCOUNT; q++) st->counts[q] += scnt[q][0];
+  for (int q = 0; q < 8; q++) st->counts[q] += scnt[q][0];
   if (st->la_variant)  // CountLiquid for the next sweep (main.f:637 of the laser-annealing build)
     for (int k = 0; k < 3; k++) {
       st->cc[k] += (long long)st->dcc[k];
@@ -938,7 +940,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
   __shared__ EvScratch sc;
   __shared__ int s_tiles[MAX_TOUCHED], s_ntl, s_err;
   __shared__ double s_tsum[SERIAL_LDS_TILES], s_bsum[SERIAL_LDS_TILES / WAVE];
-  __shared__ unsigned long long s_cnt[NCOUNT];
+  __shared__ unsigned long long s_cnt[8];
   const int lane = threadIdx.x;
   const SerialDesc D = descs[blockIdx.x];
   Lattice L = D.L;
@@ -948,7 +950,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
   const bool on_chip = D.ntiles <= SERIAL_LDS_TILES;
   double *tsum = on_chip ? s_tsum : D.tsum;
   double *bsum = on_chip ? s_bsum : D.bsum;
-  if (lane < NCOUNT) s_cnt[lane] = 0;
+  if (lane < 8) s_cnt[lane] = 0;
   // initial sums
   for (int t = 0; t < D.ntiles; t++) {
     const double ts = tile_sum_compact(L, t, lane);
@@ -1038,7 +1040,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
       }
       const int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
       if (slot >= 0 && slot < 8) s_cnt[slot]++;
-      if (res.species && (itrans == 4 || itrans == 5)) s_cnt[(itrans == 4 ? 7 : 10) + res.species]++;
+      if (COV && itrans == 5 && res.species) st->desorbed[res.species - 1]++;
       int nt = 0;
       for (int q = 0; q < sc.ntouched; q++) {
         if (sc.touched[q].i < 0) continue;
@@ -1075,7 +1077,7 @@ __global__ void __launch_bounds__(WAVE) k_serial(const SerialDesc *descs) {
     st->events = events;
     st->log_n += nev;
     st->stop = stop;
-    for (int q = 0; q < NCOUNT; q++) st->counts[q] += s_cnt[q];
+    for (int q = 0; q < 8; q++) st->counts[q] += s_cnt[q];
     if (err) st->err = err;
   }
 }
@@ -1355,6 +1357,32 @@ __global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_
   }
 }
 
+// ---- census: atoms by species and covered points by coverage slot ---------
+// Every deposition adds one atom of its species and every evaporation
+// removes one (absorption / desorption likewise for coverage), so the
+// evaporations and desorptions by species since a state was loaded follow
+// from the depositions and absorptions counted by the events and the
+// census then and now, with no per-event work.
+__global__ void __launch_bounds__(256) k_census(Lattice L, unsigned long long *out) {
+  __shared__ unsigned long long s[6];
+  if (threadIdx.x < 6) s[threadIdx.x] = 0ull;
+  __syncthreads();
+  unsigned long long c[6] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < L.box.npts;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int k = s_kind(L.site[p]);
+    if (kind_species(k)) c[k - 1]++;
+    else if (kind_cov_slot(k)) c[2 + kind_cov_slot(k)]++;
+  }
+  for (int q = 0; q < 6; q++) {
+    unsigned long long v = c[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && v) atomicAdd(&s[q], v);
+  }
+  __syncthreads();
+  if (threadIdx.x < 6 && s[threadIdx.x]) atomicAdd(&out[threadIdx.x], s[threadIdx.x]);
+}
+
 // ---- laser-annealing coupling: temperature map in, phase maps out ----------
 // The laser-annealing build exchanges whole-grid int32 maps with its FEM
 // driver: the temperature map b and the phase maps c (coordination) and d
@@ -1492,6 +1520,8 @@ struct mskg_engine {
   bool offwall_planes = false;
   // InitSt 'ST' reference tables (one allocation: temperatures, then the tables)
   void *d_st_tables = nullptr;
+  // census (atoms by species, covered points by slot) of the state as loaded
+  int64_t census0[6] = {0, 0, 0, 0, 0, 0};
   int *d_list_err = nullptr;
   int *d_win = nullptr;
   Geometry *d_geo = nullptr;
@@ -1627,6 +1657,7 @@ static int word_kind(const mskg_engine *e, int32_t v, int *kind) {
 }
 
 static int la_variant_off(mskg_engine *e);
+static int census(mskg_engine *e, int64_t out[6]);
 static int read_state(mskg_engine *e, RunState &st) {
   HIPCHK(e, hipMemcpyAsync(&st, e->d_st, sizeof st, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
@@ -1838,6 +1869,7 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt, int32_t n_ad, const int
   HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
   if (int rc_la = la_variant_off(e)) return rc_la;  // a new state: the counts are the caller's to give again
+  if (int rc_c = census(e, e->census0)) return rc_c;
   e->loaded = true;
   e->lists_dirty = true;
   e->keys_valid = false;
@@ -2266,14 +2298,35 @@ int mskg_site_rates(mskg_engine *e, int64_t n, const int32_t *xyz, int32_t *cls_
   return MSKG_OK;
 }
 
+static int census(mskg_engine *e, int64_t out[6]) {
+  unsigned long long *d = nullptr;
+  if (hipMalloc(&d, 6 * sizeof(unsigned long long)) != hipSuccess)
+    return fail_(e, MSKG_ENOMEM, "census: no device memory");
+  unsigned long long h[6];
+  hipError_t ce = hipMemsetAsync(d, 0, 6 * sizeof(unsigned long long), e->stream);
+  if (ce == hipSuccess) {
+    const int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_census, dim3(blocks), dim3(256), 0, e->stream, make_lattice(e), d);
+    ce = hipGetLastError();
+  }
+  if (ce == hipSuccess) ce = hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, e->stream);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
+  hipFree(d);
+  if (ce != hipSuccess) return fail_(e, MSKG_EDEVICE, "census: %s", hipGetErrorString(ce));
+  for (int q = 0; q < 6; q++) out[q] = (int64_t)h[q];
+  return MSKG_OK;
+}
+
 int mskg_species_counts(mskg_engine *e, int64_t evaporated[3], int64_t desorbed[3]) {
   if (!e || !e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
   RunState st;
   int rc = read_state(e, st);
   if (rc) return rc;
+  int64_t now[6];
+  if ((rc = census(e, now))) return rc;
   for (int k = 0; k < 3; k++) {
-    if (evaporated) evaporated[k] = (int64_t)st.counts[8 + k];
-    if (desorbed) desorbed[k] = (int64_t)st.counts[11 + k];
+    if (evaporated) evaporated[k] = (int64_t)st.counts[k] - (now[k] - e->census0[k]);
+    if (desorbed) desorbed[k] = (int64_t)st.desorbed[k];
   }
   return MSKG_OK;
 }
@@ -2351,6 +2404,7 @@ static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction
   HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
   if (int rc_la = la_variant_off(e)) return rc_la;  // a new state: the counts are the caller's to give again
+  if (int rc_c = census(e, e->census0)) return rc_c;
   e->loaded = true;
   e->lists_dirty = true;
   e->keys_valid = false;
@@ -2419,6 +2473,7 @@ int mskg_copy_state(mskg_engine *dst, const mskg_engine *src) {
   }
   HIPCHK(dst, hipMemcpyAsync(dst->d_st, &s0, sizeof s0, hipMemcpyHostToDevice, dst->stream));
   HIPCHK(dst, hipStreamSynchronize(dst->stream));
+  if (int rc = census(dst, dst->census0)) return rc;  // the event counters restart here
   dst->loaded = true;
   dst->lists_dirty = true;
   return MSKG_OK;

@@ -934,10 +934,8 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
     if (dep) dep_site(L, s, itrans, ch, res, sc);
     else if (evap) evap_site(L, s, sc);
     else if (COV) cov_site(L, s, absorb ? itrans - 5 : 0, sc);
-    // the species deposited or evaporated, or the coverage slot desorbed
-    res.species = dep ? itrans
-                      : (evap ? (kind_species(sc.old_kind) ? sc.old_kind : 0)
-                              : (itrans == 5 ? kind_cov_slot(sc.old_kind) : 0));
+    res.species = dep ? itrans : (evap && kind_species(sc.old_kind) ? sc.old_kind : 0);
+    if (COV && itrans == 5) res.species = kind_cov_slot(sc.old_kind);  // the coverage desorbed
   }
   MSKG_SYNC();
   if (sc.abort) {
