@@ -24,7 +24,7 @@ def test_header_and_exports_agree():
 
 
 def test_create_without_gpu_fails_cleanly():
-    """mskg_create reports MSKG_EDEVICE (or succeeds on a GPU box) -- never crashes."""
+    """mskg_create reports MSKG_EDEVICE (or succeeds where a GPU is present) -- never crashes."""
     import numpy as np
     from mulskips_amd.engine import Engine, EngineError
     t = dict(ptrans_e=np.zeros(12288), ptrans_d=np.ones(9), ptrans_abs=np.zeros(12), ptrans_des=np.zeros(12),
