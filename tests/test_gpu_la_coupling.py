@@ -274,6 +274,43 @@ def test_st_tables_replay_bit_exact(tmp_path):
     eng.close()
 
 
+@pytest.mark.parametrize("mode", ["parallel", "serial"])
+def test_st_runs_are_serial_histories(tmp_path, mode):
+    """InitSt 'ST' in the synchronous-sublattice engine (mskg_run) and in the
+    serial kernel (mskg_run_serial): the engine's own event log, applied by
+    the oracle with the same tables and temperature field, reproduces the
+    engine's lattice bit for bit."""
+    from tests.test_oracle_st import TEMPS, scaled
+    base, box, lens = SMALL["Si-SL"]
+    path, d = small_deck(tmp_path, base, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    eng = Engine(box, d.tables(), seed=7, tile=24, events_per_pass=1.0)
+    eng.load_state(*oracle_state(o))
+    lx, ly, lz = box
+    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+    f = (1460 + (z * 5 + x + 2 * y) % 121).astype(np.int32)
+    tab = d.tables()
+    refs = [scaled(tab, 0.5), tab, scaled(tab, 2.0)]
+    for side in (o, eng):
+        side.set_temperature_field(f, "planes")
+        side.set_st_tables(TEMPS, refs)
+    o.refresh_all()
+    eng.refresh_rates()
+    eng.set_event_log(200000)
+    n, t, stop = eng.run(15000) if mode == "parallel" else eng.run_serial(15000)
+    log = eng.event_log()
+    print(f"[ST {mode}] {n} events, {len(log)} logged, stop {stop}", flush=True)
+    assert len(log) == n and n > 3000 and stop in (0, 1)
+    o.apply_log(log)
+    same_lattcoo(o, eng, box)
+    mc, _, _ = o.lists()
+    cls, rate = eng.site_rates(mc["xyz"])
+    keep = cls != 6
+    np.testing.assert_allclose(rate[keep], mc["prob"][keep], rtol=1e-6)
+    eng.close()
+
+
 def test_strict_temperature_rejects_and_keeps(tmp_path):
     o, eng, box = grown(tmp_path, "surface", n=500)
     lx, ly, lz = box
