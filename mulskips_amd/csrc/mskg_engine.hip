@@ -1599,6 +1599,14 @@ static Lattice make_lattice(mskg_engine *e) {
   return L;
 }
 
+// The captured sweep holds the k_pass variant (InitSt 'ST' tables on or off)
+// and what make_lattice gave at capture (CountLiquid, class keys) and the
+// event log: a change to any of these needs a new capture at the next run.
+static void drop_graph(mskg_engine *e) {
+  if (e->graph_exec) { hipGraphExecDestroy(e->graph_exec); e->graph_exec = nullptr; }
+  if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
+}
+
 // Same-colour tiles are one tile apart, so two concurrent events are at
 // least tile+1 grid units apart along some axis.  An event writes records
 // and rates at most 3 units from its site per axis (6 with the
@@ -1959,6 +1967,7 @@ int mskg_phases(mskg_engine *e, int32_t *coor, int32_t *species, int32_t layout,
 static int la_variant_off(mskg_engine *e) {
   if (!e->tab.la_variant) return MSKG_OK;
   e->tab.la_variant = 0;
+  drop_graph(e);
   HIPCHK(e, hipMemcpyAsync(e->d_tab, &e->tab, sizeof(Tables), hipMemcpyHostToDevice, e->stream));
   return MSKG_OK;
 }
@@ -1972,6 +1981,7 @@ int mskg_set_la_variant(mskg_engine *e, const double p0[3], const double tm[3], 
     if (k < e->ncrystal && !(sigma[k] != 0.0))
       return fail_(e, MSKG_EINVAL, "sigma of species %d is zero", k + 1);
   e->tab.la_variant = 1;
+  drop_graph(e);
   e->tab.la_swap = e->ncrystal == 2 && e->cfg.list_crystal[0] == 14 && e->cfg.list_crystal[1] == 32;
   for (int k = 0; k < 3; k++) {
     e->tab.la_p0[k] = p0[k];
@@ -2007,6 +2017,7 @@ int mskg_set_st_tables(mskg_engine *e, int32_t ntemp, const int32_t *temps, cons
       return fail_(e, MSKG_EINVAL, "reference temperatures must increase (InterpProbFromField.f:39 assumes it)");
   }
   HIPCHK(e, hipStreamSynchronize(e->stream));
+  drop_graph(e);
   hipFree(e->d_st_tables);
   e->d_st_tables = nullptr;
   e->tab.st_n = 0;
@@ -2414,6 +2425,7 @@ static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction
 int mskg_track_classes(mskg_engine *e, int32_t on) {
   if (!e || !e->d_site) return MSKG_EINVAL;
   if (!on) {
+    if (e->track) drop_graph(e);
     e->track = false;
     e->keys_valid = false;
     return MSKG_OK;
@@ -2426,6 +2438,7 @@ int mskg_track_classes(mskg_engine *e, int32_t on) {
     HIPCHK(e, hipMalloc(&e->d_key_seen, np * sizeof(int32_t)));
     HIPCHK(e, hipMalloc(&e->d_rate_seen, np * sizeof(float)));
   }
+  if (!e->track) drop_graph(e);
   e->track = true;
   e->keys_valid = false;
   if (!e->loaded) return MSKG_OK;
@@ -2487,8 +2500,7 @@ int mskg_set_event_log(mskg_engine *e, int64_t capacity) {
     HIPCHK(e, hipMalloc(&e->d_log, capacity * sizeof(mskg_event)));
     e->log_cap = capacity;
   }
-  if (e->graph_exec) { hipGraphExecDestroy(e->graph_exec); e->graph_exec = nullptr; }
-  if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
+  drop_graph(e);
   if (e->loaded) {
     RunState st;
     int rc = read_state(e, st);

@@ -311,6 +311,78 @@ def test_st_runs_are_serial_histories(tmp_path, mode):
     eng.close()
 
 
+def test_st_tables_set_after_a_run(tmp_path):
+    """A run captures the sweep; InitSt 'ST' tables set after it switch the
+    next run to the interpolated rates: the oracle applies the first run's
+    events with the plain tables and the second's with the same ST tables,
+    and ends with the engine's lattice and rates."""
+    from tests.test_oracle_st import TEMPS, scaled
+    base, box, lens = SMALL["Si-SL"]
+    path, d = small_deck(tmp_path, base, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    eng = Engine(box, d.tables(), seed=5)
+    eng.load_state(*oracle_state(o))
+    eng.set_event_log(200000)
+    n1, _, stop1 = eng.run(3000)
+    log1 = eng.event_log()
+    assert len(log1) == n1 and n1 >= 3000 and stop1 in (0, 1)
+    o.apply_log(log1)
+    lx, ly, lz = box
+    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+    f = (1460 + (z * 5 + x + 2 * y) % 121).astype(np.int32)
+    tab = d.tables()
+    refs = [scaled(tab, 0.5), tab, scaled(tab, 2.0)]
+    for side in (o, eng):
+        side.set_temperature_field(f, "planes")
+        side.set_st_tables(TEMPS, refs)
+    o.refresh_all()
+    eng.refresh_rates()
+    n2, _, stop2 = eng.run(10000)
+    log = eng.event_log()
+    print(f"[ST after a run] {n1} + {n2} events, stop {stop2}", flush=True)
+    assert len(log) == n1 + n2 and n2 > 3000 and stop2 in (0, 1)
+    np.testing.assert_array_equal(log[:n1], log1)
+    o.apply_log(log[n1:])
+    same_lattcoo(o, eng, box)
+    mc, _, _ = o.lists()
+    cls, rate = eng.site_rates(mc["xyz"])
+    keep = cls != 6
+    np.testing.assert_allclose(rate[keep], mc["prob"][keep], rtol=1e-6)
+    eng.close()
+
+
+def test_la_variant_set_after_a_run(tmp_path):
+    """A run in the main build's laser-annealing form captures the sweep; the
+    laser-annealing build's form set after it, with no liquid of species 2
+    (which hardly melts: 2.5 eV bonds), keeps every deposition of the next
+    run to species 1 (the main form would deposit both)."""
+    from types import SimpleNamespace
+    from tests.test_oracle_la_variant import field as vfield, la_energy_deck, species_counts
+    path, d, box = la_energy_deck(tmp_path, "surface", e_pair=((0.6, 0.8), (2.5, 2.5)))
+    o = Oracle(*box)
+    o.load_deck(path)
+    eng = Engine(box, d.tables(), la=LA)
+    eng.load_state(*oracle_state(o))
+    eng.set_temperature_field(vfield(box), "socket")
+    eng.refresh_rates()
+    n1, _, _ = eng.run(3000)
+    dep1 = eng.counters(lists=False)[0][:3].copy()
+    assert dep1[0] > 0 and dep1[1] > 0, dep1
+    cc = species_counts(SimpleNamespace(lattcoo=lambda: eng.store_state()[0]))
+    eng.set_la_variant(VARIANT["p0"], VARIANT["tm"], VARIANT["sigma"], VARIANT["tflex"],
+                       [cc[0] + 20000, cc[1]], cc)
+    eng.refresh_rates()
+    n2, _, stop = eng.run(10000)
+    dep2 = eng.counters(lists=False)[0][:3] - dep1
+    evap, _ = eng.species_counts()
+    print(f"[LA variant after a run] {n1} + {n2} events, stop {stop}, deposits {dep2}, "
+          f"evaporated {evap}", flush=True)
+    assert n2 >= 10000 and dep2[0] > 1000
+    assert dep2[1] <= evap[1]
+    eng.close()
+
+
 def test_strict_temperature_rejects_and_keeps(tmp_path):
     o, eng, box = grown(tmp_path, "surface", n=500)
     lx, ly, lz = box
