@@ -275,32 +275,45 @@ def test_st_tables_replay_bit_exact(tmp_path):
 
 
 @pytest.mark.parametrize("mode", ["parallel", "serial"])
-def test_st_runs_are_serial_histories(tmp_path, mode):
-    """InitSt 'ST' in the synchronous-sublattice engine (mskg_run) and in the
-    serial kernel (mskg_run_serial): the engine's own event log, applied by
-    the oracle with the same tables and temperature field, reproduces the
-    engine's lattice bit for bit."""
-    from tests.test_oracle_st import TEMPS, scaled
-    base, box, lens = SMALL["Si-SL"]
-    path, d = small_deck(tmp_path, base, lens)
+@pytest.mark.parametrize("model", ["st", "la"])
+def test_runs_are_serial_histories(tmp_path, model, mode):
+    """InitSt 'ST' tables, and the main build's laser-annealing rates from a
+    temperature map, in the synchronous-sublattice engine (mskg_run) and in
+    the serial kernel (mskg_run_serial): the engine's own event log, applied
+    by the oracle with the same tables and temperature field, reproduces the
+    engine's lattice bit for bit and its rates within 1e-6."""
+    if model == "st":
+        from tests.test_oracle_st import TEMPS, scaled
+        base, box, lens = SMALL["Si-SL"]
+        path, d = small_deck(tmp_path, base, lens)
+        la = None
+    else:
+        path, d, box = la_deck(tmp_path)
+        la = LA
     o = Oracle(*box)
     o.load_deck(path)
-    eng = Engine(box, d.tables(), seed=7, tile=24, events_per_pass=1.0)
+    eng = Engine(box, d.tables(), seed=7, tile=24, events_per_pass=1.0, la=la)
     eng.load_state(*oracle_state(o))
-    lx, ly, lz = box
-    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
-    f = (1460 + (z * 5 + x + 2 * y) % 121).astype(np.int32)
-    tab = d.tables()
-    refs = [scaled(tab, 0.5), tab, scaled(tab, 2.0)]
-    for side in (o, eng):
-        side.set_temperature_field(f, "planes")
-        side.set_st_tables(TEMPS, refs)
+    if model == "st":
+        lx, ly, lz = box
+        z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+        f = (1460 + (z * 5 + x + 2 * y) % 121).astype(np.int32)
+        tab = d.tables()
+        refs = [scaled(tab, 0.5), tab, scaled(tab, 2.0)]
+        for side in (o, eng):
+            side.set_temperature_field(f, "planes")
+            side.set_st_tables(TEMPS, refs)
+    else:
+        o.set_la(LA["tm"], LA["p0"], LA["sigma"], LA["tflex"])
+        f = to_socket(la_field(box, 0))
+        for side in (o, eng):
+            side.set_temperature_field(f, "socket")
     o.refresh_all()
     eng.refresh_rates()
     eng.set_event_log(200000)
     n, t, stop = eng.run(15000) if mode == "parallel" else eng.run_serial(15000)
     log = eng.event_log()
-    print(f"[ST {mode}] {n} events, {len(log)} logged, stop {stop}", flush=True)
+    print(f"[{model} {mode}] {n} events, {len(log)} logged, stop {stop}", flush=True)
     assert len(log) == n and n > 3000 and stop in (0, 1)
     o.apply_log(log)
     same_lattcoo(o, eng, box)
