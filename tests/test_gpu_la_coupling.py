@@ -275,16 +275,18 @@ def test_st_tables_replay_bit_exact(tmp_path):
 
 
 @pytest.mark.parametrize("mode", ["parallel", "serial"])
-@pytest.mark.parametrize("model", ["st", "la"])
+@pytest.mark.parametrize("model", ["st", "st-cvd", "la"])
 def test_runs_are_serial_histories(tmp_path, model, mode):
-    """InitSt 'ST' tables, and the main build's laser-annealing rates from a
-    temperature map, in the synchronous-sublattice engine (mskg_run) and in
+    """InitSt 'ST' tables (on the Si-SL deck, and on the CVD deck, whose
+    adsorption and desorption tables are interpolated too), and the main
+    build's laser-annealing rates from a temperature map, in the
+    synchronous-sublattice engine (mskg_run) and in
     the serial kernel (mskg_run_serial): the engine's own event log, applied
     by the oracle with the same tables and temperature field, reproduces the
     engine's lattice bit for bit and its rates within 1e-6."""
-    if model == "st":
+    if model.startswith("st"):
         from tests.test_oracle_st import TEMPS, scaled
-        base, box, lens = SMALL["Si-SL"]
+        base, box, lens = SMALL["CVD-Si-SiH2" if model == "st-cvd" else "Si-SL"]
         path, d = small_deck(tmp_path, base, lens)
         la = None
     else:
@@ -294,7 +296,7 @@ def test_runs_are_serial_histories(tmp_path, model, mode):
     o.load_deck(path)
     eng = Engine(box, d.tables(), seed=7, tile=24, events_per_pass=1.0, la=la)
     eng.load_state(*oracle_state(o))
-    if model == "st":
+    if model.startswith("st"):
         lx, ly, lz = box
         z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
         f = (1460 + (z * 5 + x + 2 * y) % 121).astype(np.int32)
