@@ -61,7 +61,8 @@ typedef struct {
   uint64_t seed;            /* Philox-4x32-10 key for this replica                */
   int32_t tile;             /* synchronous-sublattice tile edge (grid units, even,
                                >=16, len_x/tile and len_y/tile even); 0 = choose  */
-  int32_t tile_capacity;    /* max MC particles per tile held on chip; 0 = 1024   */
+  int32_t tile_capacity;    /* max MC particles per tile held on chip; 0 = 1024,
+                               at most 6144 (LDS)                                 */
   double events_per_pass;   /* k of the window policy below; 0 = 1.0                */
   int32_t tau_policy;       /* 0: tau = k / (largest tile rate), every tile does at most
                                about k events per colour pass; 1: tau = k * (tiles with MC

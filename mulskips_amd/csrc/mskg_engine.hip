@@ -1727,6 +1727,10 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
                  "no valid tile edge for box %d x %d (requested %d): need an even tile >= %d with an even "
                  "number of tiles along x and y",
                  cfg->len_x, cfg->len_y, cfg->tile, min_tile);
+  // the pass kernel holds a tile's list in LDS at 9 B per entry: at most
+  // 6144 entries (55 KB) keep one wave's LDS under 64 KB
+  if (cfg->tile_capacity < 0 || cfg->tile_capacity > 6144)
+    return fail_(e, MSKG_EINVAL, "tile_capacity %d: 0 (1024) or 1..6144", cfg->tile_capacity);
   e->cap = cfg->tile_capacity > 0 ? cfg->tile_capacity : 1024;
   e->capg = e->cap;
   e->ntx = cfg->len_x / e->tile;

@@ -11,6 +11,7 @@
 import numpy as np
 import pytest
 
+from mulskips_amd.autotune import tile_capacity
 from mulskips_amd.engine import Engine
 from oracle.oracle import Oracle
 from tests.helpers import list_map, oracle_state, parity_mask, small_deck
@@ -51,13 +52,13 @@ def test_device_slab_equals_reference_builder(tmp_path, case):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("tile,epp", [(24, 1.0), (16, 4.0)])
+@pytest.mark.parametrize("tile,epp", [(24, 1.0), (16, 4.0), (48, 4.0)])
 def test_parallel_run_is_a_serial_history(tmp_path, case, tile, epp):
     deck, mode, box, lens = CASES[case]
     path, d = small_deck(tmp_path, deck, lens)
     o = Oracle(*box)
     o.load_deck(path)
-    eng = Engine(box, d.tables(), seed=11, tile=tile, events_per_pass=epp)
+    eng = Engine(box, d.tables(), seed=11, tile=tile, tile_capacity=tile_capacity(tile), events_per_pass=epp)
     eng.build_slab(mode, lens[0] - lens[0] % 24)
     eng.set_event_log(400000)
     n, t, stop = eng.run(100000)
