@@ -74,3 +74,11 @@ def test_all_failed_and_no_tile():
              runner=Runner({}, fail=[(24, 1.0), (48, 1.0)]))
     with pytest.raises(ValueError, match="no valid tile"):
         tune((100, 100, 100), tab, setup=None, device="dev", runner=Runner({}))
+
+
+def test_capacity_beyond_lds_is_refused():
+    """mskg_create rejects a list capacity whose LDS would not fit one wave's
+    64 KB, before it looks for a device."""
+    from mulskips_amd.engine import Engine
+    with pytest.raises(EngineError, match="tile_capacity 7000"):
+        Engine((96, 96, 192), models.sic_c2_tables(), tile=24, tile_capacity=7000)
