@@ -83,6 +83,8 @@ def parse():
     ap.add_argument("--trajectory-events", type=int, default=1000000)
     ap.add_argument("--la-cycles", type=int, default=10, help="laser-annealing coupling leg (0 = skip)")
     ap.add_argument("--checkpoint-every", type=int, default=100000)
+    ap.add_argument("--tune", action="store_true",
+                    help="launch-shape leg: every tile edge x events per pass 2/4/8 timed on the benchmark model")
     return ap.parse_args()
 
 
@@ -352,6 +354,22 @@ def cpu_baseline(tables, substrate_len, seconds, cores=1):
     return out
 
 
+def tune_leg(box, tables, substrate):
+    """The launch-shape autotuner on the benchmark model: each valid tile edge
+    with k = 2, 4 and 8 events per tile and pass, 2e5 warm-up and 4e6 timed
+    events from a fresh slab each. k = 8 is timed for comparison only; the
+    choice is the fastest with k <= 4, the validated windows."""
+    from mulskips_amd.autotune import MAX_VALIDATED_EVENTS_PER_PASS, tune
+    res = tune(box, tables, lambda e: e.build_slab(1, substrate), events_per_pass=(2.0, 4.0, 8.0),
+               warmup=200000, events=4000000, allow_unvalidated=True, tau_policy=1,
+               frozen_below_z=substrate + 1, seed=20261202)
+    ok = [t for t in res["trials"] if "events_per_s" in t and t["events_per_pass"] <= MAX_VALIDATED_EVENTS_PER_PASS]
+    best = max(ok, key=lambda t: t["events_per_s"])
+    return {"tile": best["tile"], "events_per_pass": best["events_per_pass"], "events_per_s": best["events_per_s"],
+            "trials": res["trials"],
+            "note": "k = 8 is faster but its growth rate leaves the serial statistics; the bench runs k = 4"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -449,6 +467,10 @@ def main():
         print(f"# laser-annealing leg: {out['laser_annealing']['value']:.4g} events/s, coupling "
               f"{out['laser_annealing']['set_field_ms']:.1f} + {out['laser_annealing']['phases_ms']:.1f} ms per cycle",
               file=sys.stderr, flush=True)
+    if a.tune:
+        out["tuning"] = tune_leg(box, tables, substrate)
+        print(f"# tuning leg: fastest validated tile {out['tuning']['tile']}, "
+              f"k = {out['tuning']['events_per_pass']}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tables, substrate, a.cpu_seconds, max(1, min(a.cpu_cores, os.cpu_count())))
     print(json.dumps(out), flush=True)
