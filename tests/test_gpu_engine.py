@@ -52,9 +52,14 @@ def test_device_slab_equals_reference_builder(tmp_path, case):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("tile,epp", [(24, 1.0), (16, 4.0), (48, 4.0)])
-def test_parallel_run_is_a_serial_history(tmp_path, case, tile, epp):
+@pytest.mark.parametrize("tile,epp,wide", [(24, 1.0, False), (16, 4.0, False), (48, 4.0, False), (32, 4.0, True)])
+def test_parallel_run_is_a_serial_history(tmp_path, case, tile, epp, wide):
+    """Tile edges 16, 24 and 48 on the 96-wide box; 32 (an even number of
+    tiles needs a width that is a multiple of 64, and the 3C cell one of
+    12) on a 192 x 192 x 144 box."""
     deck, mode, box, lens = CASES[case]
+    if wide:
+        box = (192, 192, box[2])
     path, d = small_deck(tmp_path, deck, lens)
     o = Oracle(*box)
     o.load_deck(path)
