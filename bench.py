@@ -114,38 +114,15 @@ def stream_probe(n_floats=1 << 30, reps=10):
     return {"copy_GBps": copy, "triad_GBps": triad}
 
 
-def ensemble(n, seed=2026):
-    """The batched-ensemble replicas: Si(1-x)Ge(x) (001) slabs of uneven
-    lateral size (4..32 conventional cells) and height, T uniform in
-    [900, 1500] K, flux log-uniform in [0.1, 10] ML/s, Ge fraction uniform
-    in [0, 0.5] (random substitutional alloy, also the Ge share of the
-    flux), pair energies Ge-Ge 0.45 / Si-Ge 0.5 / Si-Si 0.55 eV, a frozen
-    substrate; each replica runs for the KMC time in which 20 ML are
-    deposited (20 / flux)."""
-    import numpy as np
-
-    from mulskips_amd import models
-    rng = np.random.default_rng(seed)
-    reps = []
-    for i in range(n):
-        lc = int(rng.choice([4, 8, 16, 32]))
-        hc = int(rng.choice([8, 12, 16]))
-        temp = float(rng.uniform(900.0, 1500.0))
-        flux = float(10 ** rng.uniform(-1.0, 1.0))
-        x = float(rng.uniform(0.0, 0.5))
-        tab = models.pair_energy_tables([[0.55, 0.5], [0.5, 0.45]], [flux * (1 - x), flux * x], temp, 1e13, (14, 32))
-        reps.append(dict(box=(12 * lc, 12 * lc, 12 * hc), tables=tab, x=x, t_end=20.0 / flux, seed=10_000 + i))
-    return reps
-
-
 def batched_leg(n):
     """Aggregate events/s of the ensemble in one launch (serial BKL, one
     workgroup per replica) against the same replicas run one after another
     through the single-lattice synchronous-sublattice engine."""
     import numpy as np
 
+    from mulskips_amd import models
     from mulskips_amd.engine import Engine, run_serial
-    reps = ensemble(n)
+    reps = models.ensemble_replicas(n, laterals=(4, 8, 16, 32), heights=(8, 12, 16))
 
     def make(r):
         e = Engine(r["box"], r["tables"], seed=r["seed"], tile=24, frozen_below_z=36)
@@ -197,32 +174,14 @@ def la_leg(cycles, events_per_cycle=100_000, seed=2026):
 
     from mulskips_amd import models
     from mulskips_amd.engine import Engine
-    rng = np.random.default_rng(seed)
-    e_ss, e_sg, e_gg = (float(v) for v in rng.uniform(0.4, 1.2, 3))
-    pair = {(1, 1): e_ss, (1, 2): e_sg, (2, 1): e_sg, (2, 2): e_gg}
-    tab = models.pair_energy_tables([[0.55, 0.5], [0.5, 0.45]], [1.0, 1.0], 1000.0, 1e13, (14, 32))
-    pe = np.zeros_like(np.asarray(tab["ptrans_e"], np.float64))
-    for sp in (1, 2):
-        for a in range(4):
-            for b in range(4 - a):
-                if a + b:
-                    pe[models.pe_index(sp, a, b, 0)] = a * pair[(sp, 1)] + b * pair[(sp, 2)]
-    pd = np.zeros(9)
-    for sp in (1, 2):
-        for c in (1, 2, 3):
-            pd[(sp - 1) + 3 * (c - 1)] = c * pair[(sp, sp)]
-    tab["ptrans_e"], tab["ptrans_d"] = pe, pd
+    tab, barriers = models.la_sige_tables(seed)
     box = (264, 264, 384)
     lx, ly, lz = box
-    eng = Engine(box, tab, seed=seed, tau_policy=1, events_per_pass=4.0,
-                 la=dict(tm=1687.0, p0=1e13, sigma=50.0, tflex=1650.0))
+    eng = Engine(box, tab, seed=seed, tau_policy=1, events_per_pass=4.0, la=models.LA_CONSTANTS)
     eng.build_alloy_slab(240, 0.3)
-    zs = np.arange(lz, dtype=np.float64)
-    grid_nm = 0.5431 / 12  # the super-lattice step: a(Si) / 12
 
     def field(zm):
-        t = np.rint(1700.0 + 300.0 * np.tanh((zs - zm) * grid_nm / 5.0)).astype(np.int32)
-        return np.ascontiguousarray(np.broadcast_to(t, (lx * ly, lz)))  # b(1+z, 1+x+LenX*y)
+        return models.la_front_field(box, zm)  # b(1+z, 1+x+LenX*y)
 
     solid0 = int(((eng.phases("socket")[1] == 1) | (eng.phases("socket")[1] == 2)).sum())
     zm = 240.0
@@ -257,7 +216,7 @@ def la_leg(cycles, events_per_cycle=100_000, seed=2026):
             "pcie_GBps_phases": 2 * map_bytes / (t_ph / cycles) / 1e9,
             "solid_points_at_start": solid0, "solid_points_at_end": solid, "front_z_at_end": zm,
             "kmc_time_s": t_kmc, "stop_reasons": stops,
-            "barriers_eV": {"Si-Si": e_ss, "Si-Ge": e_sg, "Ge-Ge": e_gg},
+            "barriers_eV": barriers,
             "workload": "laser-annealing mode, SiGe x=0.3, 264x264x384 grid, FEM temperature map in and phase "
                         "maps out every 1e5 events (set_field_ms includes the rate re-evaluation); window policy 1, "
                         "k = 4 as in the main leg"}

@@ -252,6 +252,17 @@ int mskg_species_counts(mskg_engine *e, int64_t evaporated[3], int64_t desorbed[
 int mskg_last_timing(mskg_engine *e, double *run_ms, double *sweep_ms, int64_t *sweep_points,
                      int64_t *sweeps_done);
 
+/* Diagnostics of the synchronous-sublattice runs since the state was loaded
+ * (no reference counterpart; the reference's serial loop has no windows):
+ * guard_exits counts tile windows that ended at the cap of 65536 events per
+ * tile and window instead of at the window's end -- a nonzero count means the
+ * window policy lets one tile run far ahead of the others and the run is not
+ * a faithful splitting of the master equation (lower events_per_pass or use
+ * tau_policy 0); sweeps counts the sweeps that advanced the clock, tau is the
+ * window the next sweep will use, n_mc the MC particles at the last sweep.
+ * Any output may be null. */
+int mskg_run_diagnostics(mskg_engine *e, int64_t *guard_exits, int64_t *sweeps, double *tau, int64_t *n_mc);
+
 /* Serial BKL on a batch of independent lattices in one launch (one
  * workgroup per engine): the reference's algorithm (mcevent/PickTreeEvent.f,
  * main.f:304-371) with the selection over all MC particles of the lattice
@@ -325,6 +336,30 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
  * reproduced. */
 int mskg_frame(mskg_engine *e, int32_t *main_rec, int64_t cap_main, int32_t *void_rec, int64_t cap_void,
                int32_t *noepi_rec, int64_t cap_noepi, int64_t counts[5]);
+
+/* The SiC defect frames of the xyz output (inpout/WriteMolMolXYZFile.f),
+ * gathered on the device as (x, y, z, code) int32 quadruples in the
+ * reference's grid order (points with z <= LenZ-1):
+ *   vac_rec    every vacancy (an empty or covered point of coordination 4),
+ *              code = the sum of the species indices (1 Si, 2 C/Ge, 3 B; 0
+ *              for walls and empty points) of its four NextNXYZ points -- the
+ *              _v frame's classification (:222-260: 4 CV, 8 SV, 7 SAV, 5 CAV,
+ *              6 XV for SiC PVD);
+ *   wrong_rec  the _w frame (:264-398), for two crystal species one of which
+ *              is C: bulk atoms with 1 <= z <= LenZ-1 and three neighbours of
+ *              the other species, read while the isolated wrong atoms (bulk
+ *              atoms with no such neighbour, taken in grid order as the
+ *              reference flips them) are flipped; code = the species (1 Si,
+ *              2 C) as flipped.
+ * counts = (vacancies, wrong-occupation records).  Zero capacities size the
+ * buffers.  The reference lists vacancies in ListVoid order, which depends on
+ * the run's history; here they come in grid order.  The reference's writer
+ * leaves some flipped species behind (its restore loop reads one buffer
+ * entry for all); the engine restores the lattice exactly.  Exact when the
+ * neighbour lists of neighbouring bulk atoms are reciprocal, as the event
+ * rules keep them. */
+int mskg_frame_defects(mskg_engine *e, int32_t *vac_rec, int64_t cap_vac, int32_t *wrong_rec, int64_t cap_wrong,
+                       int64_t counts[2]);
 
 /* Host evaluation of the engine's FP64 geometry -- the same source the
  * kernels run -- for checking it against the reference formulas without a

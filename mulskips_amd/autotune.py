@@ -5,8 +5,9 @@ in each colour pass) and the number of events per tile and pass k, which
 sets the window tau. tune() times short runs of each candidate (T, k) from
 the same starting state and keeps the fastest, in events per device second.
 The choice is cached in a JSON file, keyed by the box, a digest of the rate
-tables and engine options, and the device name. A later call with the same
-key runs nothing.
+tables and engine options, the device name and the request (candidate
+tiles and windows, whether unvalidated windows were allowed, a tag of the
+starting state). A later call with the same key runs nothing.
 
 No reference counterpart: the reference runs one serial tree per binary.
 """
@@ -39,6 +40,9 @@ def candidate_tiles(box, second_neighbour=False, max_tile=48):
 MAX_VALIDATED_EVENTS_PER_PASS = 4.0
 
 MAX_CAPACITY = 4096  # 9 B of LDS per entry in the pass kernel: 37 KB per wave
+LDS_PER_CU = 160 * 1024
+PASS_STATIC_LDS = 2048   # the pass kernel's fixed LDS (touched points, counters, event scratch)
+PASS_WAVES_PER_CU = 12   # its launch bounds: 3 waves per SIMD, 4 SIMDs per CU
 
 
 def tile_capacity(tile):
@@ -49,7 +53,19 @@ def tile_capacity(tile):
     return min(MAX_CAPACITY, max(1024, -(-1024 * tile * tile // (24 * 24) // 64) * 64))
 
 
-def tune_key(box, tables, engine_kw, device):
+def lds_occupancy(capacity):
+    """One-wave pass workgroups per CU that the LDS allows at this list
+    capacity (at most the launch bounds' 12): a larger capacity reserves
+    more LDS per wave and can hold fewer tiles in flight."""
+    return min(PASS_WAVES_PER_CU, LDS_PER_CU // (9 * int(capacity) + PASS_STATIC_LDS))
+
+
+def tune_key(box, tables, engine_kw, device, tiles=None, events_per_pass=None, allow_unvalidated=False,
+             setup_tag=None):
+    """Cache key: the device, the box, a digest of the rate tables and engine
+    options, and the request itself -- the candidate tiles and windows, whether
+    unvalidated windows were allowed and the starting state's tag -- so that a
+    choice made under one request is never handed to another."""
     h = hashlib.sha256()
     for k in TABLE_KEYS:
         h.update(np.ascontiguousarray(tables[k], np.float64).tobytes())
@@ -58,7 +74,10 @@ def tune_key(box, tables, engine_kw, device):
     scal = {k: (list(map(int, tables[k])) if k.startswith("list_") else float(tables[k]))
             for k in ("n_crystal", "n_cov", "list_crystal", "list_cov", "ptrans_zig")}
     opts = {k: v for k, v in sorted(engine_kw.items()) if k != "seed"}
-    h.update(json.dumps([scal, opts], sort_keys=True, default=str).encode())
+    req = {"tiles": sorted(int(t) for t in tiles) if tiles is not None else None,
+           "events_per_pass": sorted(float(k) for k in events_per_pass) if events_per_pass is not None else None,
+           "allow_unvalidated": bool(allow_unvalidated), "setup": setup_tag}
+    h.update(json.dumps([scal, opts, req], sort_keys=True, default=str).encode())
     return f"{device}|{box[0]}x{box[1]}x{box[2]}|{h.hexdigest()[:16]}"
 
 
@@ -87,7 +106,7 @@ def time_candidate(box, tables, setup, tile, events_per_pass, warmup, events, en
 
 
 def tune(box, tables, setup, tiles=None, events_per_pass=(1.0, 2.0, 4.0), warmup=20000, events=200000,
-         cache=None, device=None, runner=time_candidate, allow_unvalidated=False, **engine_kw):
+         cache=None, device=None, runner=time_candidate, allow_unvalidated=False, setup_tag=None, **engine_kw):
     """Pick the fastest (tile, events_per_pass) for this box and model.
 
     setup(engine) brings a new engine to the starting state (for example
@@ -98,6 +117,11 @@ def tune(box, tables, setup, tiles=None, events_per_pass=(1.0, 2.0, 4.0), warmup
     trials and the cache key; cached=True when it came from the cache file.
     Candidates above MAX_VALIDATED_EVENTS_PER_PASS are refused unless
     allow_unvalidated is set: the tuner measures speed, not accuracy.
+    The result's tile_capacity is the list capacity the chosen trial ran with
+    (tile_capacity(tile)); build the deployed engine with it, so that it is
+    the engine that was timed.  setup_tag names the starting state setup()
+    builds, for the cache key.  Each trial records its capacity and the
+    pass workgroups per CU its LDS reservation allows.
     """
     bad = [k for k in events_per_pass if k > MAX_VALIDATED_EVENTS_PER_PASS]
     if bad and not allow_unvalidated:
@@ -105,31 +129,39 @@ def tune(box, tables, setup, tiles=None, events_per_pass=(1.0, 2.0, 4.0), warmup
                          "are validated up to there (allow_unvalidated=True to time them anyway)")
     box = tuple(int(v) for v in box)
     device = device or device_name()
-    key = tune_key(box, tables, engine_kw, device)
-    store = {}
-    if cache and os.path.exists(cache):
-        with open(cache) as f:
-            store = json.load(f)
-        if key in store:
-            return dict(store[key], cached=True)
     if tiles is None:
         tiles = candidate_tiles(box, second_neighbour=tables.get("second_factor") is not None)
     if not tiles:
         raise ValueError(f"no valid tile edge for box {box}")
+    key = tune_key(box, tables, engine_kw, device, tiles, events_per_pass, allow_unvalidated, setup_tag)
+    store = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            store = json.load(f)
+        hit = store.get(key)
+        # a cached entry must answer this request: a window within the
+        # validated ones unless allowed, a tile among the candidates
+        if hit and (allow_unvalidated or hit["events_per_pass"] <= MAX_VALIDATED_EVENTS_PER_PASS) \
+                and int(hit["tile"]) in [int(t) for t in tiles]:
+            return dict(hit, cached=True)
     trials = []
     for t in tiles:
         for k in events_per_pass:
+            cap = tile_capacity(int(t))
+            tr = {"tile": int(t), "events_per_pass": float(k), "tile_capacity": cap,
+                  "lds_workgroups_per_cu": lds_occupancy(cap)}
             try:
-                r = runner(box, tables, setup, int(t), float(k), int(warmup), int(events), engine_kw)
-                trials.append({"tile": int(t), "events_per_pass": float(k), "events_per_s": float(r)})
+                tr["events_per_s"] = float(runner(box, tables, setup, int(t), float(k), int(warmup), int(events),
+                                                  engine_kw))
             except EngineError as exc:
-                trials.append({"tile": int(t), "events_per_pass": float(k), "error": str(exc)})
+                tr["error"] = str(exc)
+            trials.append(tr)
     ok = [tr for tr in trials if "events_per_s" in tr]
     if not ok:
         raise RuntimeError(f"every candidate failed: {trials}")
     best = max(ok, key=lambda tr: tr["events_per_s"])
     out = {"tile": best["tile"], "events_per_pass": best["events_per_pass"], "events_per_s": best["events_per_s"],
-           "trials": trials, "key": key}
+           "tile_capacity": best["tile_capacity"], "trials": trials, "key": key}
     if cache:
         store[key] = out
         tmp = cache + ".tmp"

@@ -46,6 +46,8 @@ constexpr int WAVE = 64;
 constexpr int NSLOT = 128;
 // Device-side stop at the run's event or time limit (reported as MSKG_STOP_LIMIT).
 constexpr int STOP_AT_LIMIT = 100;
+// Events one tile may execute in one window (a bound on the wave's work).
+constexpr int PASS_EVENT_CAP = 1 << 16;
 struct Slot {
   unsigned long long rmax;  // bit pattern of the largest (non-negative) tile rate
   unsigned long long n, tiles, events, counts[8];
@@ -84,6 +86,8 @@ struct RunState {
   long long cc[3], cinit[3], cl[3];
   unsigned long long dcc[3];
   unsigned long long desorbed[3];  // desorptions by coverage slot (CountCov)
+  int final_sweep;                 // the window was shortened to end at max_time
+  unsigned long long guard_exits;  // tile windows cut short by the per-window event cap
 };
 
 struct PassParams {
@@ -507,8 +511,10 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   ctr.w = 0;
   int err = 0;
   // every wave ends its pass: the window tau bounds the expected number of
-  // events, and this cap bounds the worst case
-  for (int guard = 0; guard < (1 << 16); guard++) {
+  // events, and this cap bounds the worst case (a window cut short by it is
+  // counted in RunState::guard_exits and reported by the run)
+  int guard = 0;
+  for (; guard < PASS_EVENT_CAP; guard++) {
     // lane-major order: lane l owns entries l, l + 64, ...; one wave scan of
     // the lanes' partial sums gives the total and the lane that holds the pick
     double part = 0.0;
@@ -520,7 +526,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
     // stacking choice, 32 random bits each (the reference's ran3 has fewer)
     const U4 d = philox4x32_10(ctr, P.key0, P.key1);
     ctr.w++;
-    const double dt = -(double)__logf(u01f(d.x)) / R;
+    const double dt = -log(u01_32(d.x)) / R;  // FP64 waiting time
     if (tloc + dt > tau) break;
     tloc += dt;
     const double target = u01_32(d.y) * R;
@@ -655,6 +661,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
     n += __popcll(am);
     __syncthreads();
   }
+  if (guard >= PASS_EVENT_CAP && lane == 0) atomicAdd(&st->guard_exits, 1ull);
   // 3. write the list back (active entries only), tile totals, counters
   double part = 0.0;
   int nout = 0;
@@ -759,9 +766,11 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
   }
   if (*list_err && !st->err) st->err = *list_err;
   if (st->tau > 0.0 && !st->err && !st->stop) {
-    st->clock += st->tau;
+    // a window shortened to end at the time limit lands on it exactly
+    st->clock = st->final_sweep ? st->max_time : st->clock + st->tau;
     st->sweeps_run++;
   }
+  st->final_sweep = 0;
   st->sweeps++;
   st->rmax = rmax;
   st->n_mc = nmc;
@@ -774,6 +783,12 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
     if (nmc <= 20) st->stop = MSKG_STOP_FEW_MC;
     else if (!(rmax > 0.0)) st->stop = MSKG_STOP_NO_EVENTS;
     else if (st->clock >= st->max_time || st->events >= st->max_events) st->stop = STOP_AT_LIMIT;
+  }
+  // the last window of a run to a time limit ends at that limit: every tile
+  // stops at max_time, as the serial loop stops at the first event past it
+  if (!st->stop && st->tau > 0.0 && st->clock + st->tau >= st->max_time) {
+    st->tau = st->max_time - st->clock;
+    st->final_sweep = 1;
   }
 }
 
@@ -1255,18 +1270,35 @@ __global__ void __launch_bounds__(256) k_key_hist(Lattice L, int *cnt, double *i
 
 
 // ---- frame compaction (the xyz output of inpout/WriteMolMolXYZFile.f) --------
-// Three ordered lists in the reference's grid order (z, then y, then x; the
-// stored-point order restricted to one z plane is the same), z <= LenZ-1:
-//   0 the points the main frame lists: coverage species and atoms of
+// Ordered record lists in the reference's grid order (z, then y, then x; the
+// stored-point order restricted to one z plane is the same), z <= LenZ-1.
+// FRAME_MAIN, the main and _d frames:
+//   list 0 the points the main frame lists: coverage species and atoms of
 //     coordination <= 3 (WriteMolMolXYZFile.f:131-166);
-//   1 vacancies (empty, coordination 4);
-//   2 non-epitaxial bulk atoms: coordination 4 off the cubic diamond sites
-//     (the _d frame, :77-121).
-// and two header counts over z = 1..LenZ-1 (:93-110): 3 atoms of
-// coordination <= 3, 4 covered points.  Each block owns a contiguous chunk
-// of points; k_frame_count counts, the host scans, k_frame_write writes in
-// order with a block-wide prefix scan.
+//   list 1 vacancies (empty, coordination 4);
+//   list 2 non-epitaxial bulk atoms: coordination 4 off the cubic diamond
+//     sites (the _d frame, :77-121);
+//   and two header counts over z = 1..LenZ-1 (:93-110): atoms of
+//   coordination <= 3, covered points.
+// FRAME_DEFECTS, the SiC defect frames:
+//   list 0 vacancies with the species sum of their four listed neighbours
+//     (the _v frame's classification, :222-260);
+//   list 1 the _w frame (:264-398): bulk atoms with z = 1..LenZ-1 and three
+//     first neighbours of the other species (Si around C, C around Si), read
+//     while the isolated wrong atoms -- bulk atoms with no such neighbour --
+//     are flipped to the other species; code = the species as flipped.
+//     k_wrong_mark marks those flips (bit 7 of the record's `valid` byte,
+//     which the neighbour lists leave unused) and clears the marks after.
+// Each block owns a contiguous chunk of points; k_frame_count counts,
+// the host scans, k_frame_write writes in order with a block-wide prefix
+// scan.
 constexpr int FRAME_THREADS = 256;
+enum : int { FRAME_MAIN = 0, FRAME_DEFECTS = 1 };
+template <int MODE> struct FrameShape;
+template <> struct FrameShape<FRAME_MAIN> { static constexpr int NLIST = 3, NCNT = 5; };
+template <> struct FrameShape<FRAME_DEFECTS> { static constexpr int NLIST = 2, NCNT = 2; };
+constexpr uint8_t WRONG_FLIP = 0x80;
+
 __device__ inline int frame_cat(const Lattice &L, int64_t p, const Site &sp, int *code, bool hdr[2]) {
   int x, y, z;
   pxyz(L.box, p, x, y, z);
@@ -1294,63 +1326,170 @@ __device__ inline int frame_cat(const Lattice &L, int64_t p, const Site &sp, int
   return coor == 4 ? 1 : -1;  // an empty point with coordination 4 is a vacancy
 }
 
-__global__ void __launch_bounds__(FRAME_THREADS) k_frame_count(Lattice L, int64_t chunk, unsigned long long *counts) {
-  __shared__ unsigned long long sc[5][FRAME_THREADS / WAVE];
-  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(lo + chunk, L.box.npts);
-  unsigned long long c[5] = {0, 0, 0, 0, 0};
-  for (int64_t p = lo + threadIdx.x; p < hi; p += FRAME_THREADS) {
-    int code;
+// Species of a record as the _w analysis sees it: the wrong-site flip swaps
+// Si (1) and C (2).
+__device__ inline int flipped_species(const Site &s) {
+  const int k = s_kind(s);
+  return (s.valid & WRONG_FLIP) ? 3 - k : k;
+}
+// NCoorCorr (WriteMolMolXYZFile.f:284-296) of a bulk atom: its listed
+// neighbours carrying the other species' bit (the C bit around Si, the Si
+// bit around C), with the flips marked or not.
+__device__ inline int wrong_corr(const Lattice &L, const P3 &pt, const Site &sp, bool flips) {
+  const int k = flips ? flipped_species(sp) : s_kind(sp);
+  const NL l = list_of(L, pt, sp);
+  int corr = 0;
+  for (int i = 0; i < 4; i++) {
+    const Site q = rd(L, entry(L, l, i).i);
+    int kq = s_kind(q);
+    if (!kind_species(kq)) continue;
+    if (flips) kq = flipped_species(q);
+    corr += (k == 1) ? ((kq >> 1) & 1) : (k == 2 ? (kq & 1) : 0);
+  }
+  return corr;
+}
+
+// Membership of point p in the lists of a frame, the code per list and the
+// header counts.
+template <int MODE>
+__device__ inline void frame_point(const Lattice &L, int64_t p, const Site &sp, int sic, bool in[],
+                                   int code[], unsigned long long cnt[]) {
+  if (MODE == FRAME_MAIN) {
+    int c = 0;
     bool hdr[2];
-    const int cat = frame_cat(L, p, L.site[p], &code, hdr);
-    if (cat >= 0) c[cat]++;
-    if (cat == 1 && code < 0) c[0]++;  // a covered vacancy is also listed in the main frame
-    c[3] += hdr[0];
-    c[4] += hdr[1];
+    const int cat = frame_cat(L, p, sp, &c, hdr);
+    in[0] = cat == 0 || (cat == 1 && c < 0);  // a covered vacancy is also listed in the main frame
+    in[1] = cat == 1;
+    in[2] = cat == 2;
+    code[0] = code[2] = c;
+    code[1] = 0;
+    for (int k = 0; k < 3; k++) cnt[k] += in[k];
+    cnt[3] += hdr[0];
+    cnt[4] += hdr[1];
+  } else {
+    in[0] = in[1] = false;
+    code[0] = code[1] = 0;
+    const int kind = s_kind(sp), coor = s_coor(sp);
+    if (kind == KIND_WALL || coor != 4) return;
+    P3 pt;
+    pt.i = p;
+    pxyz(L.box, p, pt.x, pt.y, pt.z);
+    if (pt.z > L.box.lz - 1) return;
+    if (kind == KIND_EMPTY || kind >= KIND_COV0) {  // a vacancy: the species sum of its neighbours
+      const NL l = list_of(L, pt, sp);
+      int sum = 0;
+      for (int i = 0; i < 4; i++) {
+        const int kq = s_kind(rd(L, entry(L, l, i).i));
+        sum += kind_species(kq) ? kq : 0;
+      }
+      in[0] = true;
+      code[0] = sum;
+    } else if (sic && kind_species(kind) && pt.z >= 1 && wrong_corr(L, pt, sp, true) == 3) {
+      in[1] = true;
+      code[1] = flipped_species(sp);
+    }
+    cnt[0] += in[0];
+    cnt[1] += in[1];
+  }
+}
+
+// The isolated wrong atoms of the SiC analysis (WriteMolMolXYZFile.f:269-299).
+// The reference visits the bulk atoms in grid order and flips each one with
+// no neighbour of the other species on the spot, so a flip is seen by the
+// atoms visited after it.  A flip only raises its neighbours' counts (its
+// neighbours share its species until it flips), so an atom flips iff it has
+// no neighbour of the other species to begin with (a candidate) and no
+// candidate neighbour before it in grid order flips: the candidates are few,
+// k_wrong_cand lists them with their neighbours, the host resolves that
+// order, and k_wrong_set marks (or unmarks) the flips.
+__global__ void __launch_bounds__(256) k_wrong_cand(Lattice L, int64_t cap, unsigned long long *n, int4 *out) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < L.box.npts;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const Site sp = L.site[p];
+    const int kind = s_kind(sp);
+    if (!(kind == 1 || kind == 2) || s_coor(sp) != 4) continue;
+    P3 pt;
+    pt.i = p;
+    pxyz(L.box, p, pt.x, pt.y, pt.z);
+    if (pt.z < 1 || pt.z > L.box.lz - 1 || wrong_corr(L, pt, sp, false) != 0) continue;
+    const unsigned long long i = atomicAdd(n, 1ull);
+    if ((int64_t)i >= cap) continue;  // counted; the host grows the list and asks again
+    const NL l = list_of(L, pt, sp);
+    out[5 * i] = make_int4(pt.x, pt.y, pt.z, 0);
+    for (int k = 0; k < 4; k++) out[5 * i + 1 + k] = make_int4(l.x[k], l.y[k], l.z[k], 0);
+  }
+}
+__global__ void __launch_bounds__(256) k_wrong_set(Lattice L, const int4 *pos, int64_t n, int set) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = pidx(L.box, pos[i].x, pos[i].y, pos[i].z);
+    const uint8_t v = L.site[p].valid;
+    L.site[p].valid = (uint8_t)(set ? (v | WRONG_FLIP) : (v & ~WRONG_FLIP));
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(FRAME_THREADS) k_frame_count(Lattice L, int64_t chunk, int sic,
+                                                               unsigned long long *counts) {
+  constexpr int NL_ = FrameShape<MODE>::NLIST, NC = FrameShape<MODE>::NCNT;
+  __shared__ unsigned long long sc[NC][FRAME_THREADS / WAVE];
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(lo + chunk, L.box.npts);
+  unsigned long long c[NC];
+  for (int k = 0; k < NC; k++) c[k] = 0;
+  for (int64_t p = lo + threadIdx.x; p < hi; p += FRAME_THREADS) {
+    bool in[NL_];
+    int code[NL_];
+    frame_point<MODE>(L, p, L.site[p], sic, in, code, c);
   }
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  for (int k = 0; k < 5; k++) {
+  for (int k = 0; k < NC; k++) {
     unsigned long long v = c[k];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
     if (lane == 0) sc[k][w] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 5) {
+  if (threadIdx.x < NC) {
     unsigned long long v = 0;
     for (int q = 0; q < FRAME_THREADS / WAVE; q++) v += sc[threadIdx.x][q];
-    counts[(int64_t)blockIdx.x * 5 + threadIdx.x] = v;
+    counts[(int64_t)blockIdx.x * NC + threadIdx.x] = v;
   }
 }
 
-__global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_t chunk, const unsigned long long *offs,
-                                                               int4 *out0, int4 *out1, int4 *out2) {
-  __shared__ unsigned int wsum[3][FRAME_THREADS / WAVE];
+template <int MODE>
+__global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_t chunk, int sic,
+                                                               const unsigned long long *offs, int4 *out0,
+                                                               int4 *out1, int4 *out2) {
+  constexpr int NL_ = FrameShape<MODE>::NLIST, NC = FrameShape<MODE>::NCNT;
+  __shared__ unsigned int wsum[NL_][FRAME_THREADS / WAVE];
   const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(lo + chunk, L.box.npts);
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  unsigned long long base[3] = {offs[(int64_t)blockIdx.x * 3], offs[(int64_t)blockIdx.x * 3 + 1],
-                                offs[(int64_t)blockIdx.x * 3 + 2]};
-  int4 *outs[3] = {out0, out1, out2};
+  unsigned long long base[NL_];
+  for (int k = 0; k < NL_; k++) base[k] = offs[(int64_t)blockIdx.x * NL_ + k];
+  int4 *const outs[3] = {out0, out1, out2};
   for (int64_t p0 = lo; p0 < hi; p0 += FRAME_THREADS) {
     const int64_t p = p0 + threadIdx.x;
-    int code = 0, cat = -1;
-    bool hdr[2];
-    if (p < hi) cat = frame_cat(L, p, L.site[p], &code, hdr);
+    bool in[NL_];
+    int code[NL_];
+    unsigned long long cdummy[NC];
+    for (int k = 0; k < NL_; k++) in[k] = false;
+    if (p < hi) frame_point<MODE>(L, p, L.site[p], sic, in, code, cdummy);
     int x = 0, y = 0, z = 0;
-    if (cat >= 0) pxyz(L.box, p, x, y, z);
-    bool in[3] = {cat == 0 || (cat == 1 && code < 0), cat == 1, cat == 2};
-    unsigned int rank[3];
-    for (int k = 0; k < 3; k++) {
+    bool any = false;
+    for (int k = 0; k < NL_; k++) any |= in[k];
+    if (any) pxyz(L.box, p, x, y, z);
+    unsigned int rank[NL_];
+    for (int k = 0; k < NL_; k++) {
       const unsigned long long m = __ballot(in[k]);
       rank[k] = __popcll(m & ((1ull << lane) - 1ull));
       if (lane == 0) wsum[k][w] = __popcll(m);
     }
     __syncthreads();
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < NL_; k++) {
       unsigned int before = 0, total = 0;
       for (int q = 0; q < FRAME_THREADS / WAVE; q++) {
         if (q < w) before += wsum[k][q];
         total += wsum[k][q];
       }
-      if (in[k]) outs[k][base[k] + before + rank[k]] = make_int4(x, y, z, k == 1 ? 0 : code);
+      if (in[k]) outs[k][base[k] + before + rank[k]] = make_int4(x, y, z, code[k]);
       base[k] += total;
     }
     __syncthreads();
@@ -1669,6 +1808,155 @@ static int census(mskg_engine *e, int64_t out[6]);
 static int read_state(mskg_engine *e, RunState &st) {
   HIPCHK(e, hipMemcpyAsync(&st, e->d_st, sizeof st, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
+  return MSKG_OK;
+}
+
+// The ordered record lists of one frame mode (k_frame_count / k_frame_write):
+// hosts[k] receives list k (caps[k] records), counts the NCNT totals.  A call
+// with too small capacities only sizes the lists (MSKG_ECAPACITY when it
+// brought buffers).
+template <int MODE>
+static int frame_lists(mskg_engine *e, int sic, int32_t *const hosts[], const int64_t caps[], int64_t counts[]) {
+  constexpr int NL_ = FrameShape<MODE>::NLIST, NC = FrameShape<MODE>::NCNT;
+  const int64_t n = e->box.npts;
+  const int nblk = (int)std::min<int64_t>(std::max<int64_t>(1, (n + 65535) / 65536), 65536);
+  const int64_t chunk = (n + nblk - 1) / nblk;
+  Lattice L = make_lattice(e);
+  unsigned long long *d_counts = nullptr, *d_offs = nullptr;
+  int4 *d_out[3] = {nullptr, nullptr, nullptr};
+  auto cleanup = [&]() {
+    hipFree(d_counts);
+    hipFree(d_offs);
+    for (int k = 0; k < 3; k++) hipFree(d_out[k]);
+  };
+  if (hipMalloc(&d_counts, (size_t)nblk * NC * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&d_offs, (size_t)nblk * NL_ * sizeof(unsigned long long)) != hipSuccess) {
+    cleanup();
+    return fail_(e, MSKG_ENOMEM, "frame: no device memory for the block counts");
+  }
+  hipLaunchKernelGGL(k_frame_count<MODE>, dim3(nblk), dim3(FRAME_THREADS), 0, e->stream, L, chunk, sic, d_counts);
+  std::vector<unsigned long long> cnt((size_t)nblk * NC), offs((size_t)nblk * NL_);
+  hipError_t ce = hipGetLastError();
+  if (ce == hipSuccess)
+    ce = hipMemcpyAsync(cnt.data(), d_counts, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        e->stream);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
+  if (ce != hipSuccess) {
+    cleanup();
+    return fail_(e, MSKG_EDEVICE, "frame count: %s", hipGetErrorString(ce));
+  }
+  unsigned long long tot[NC];
+  for (int k = 0; k < NC; k++) tot[k] = 0;
+  for (int b = 0; b < nblk; b++)
+    for (int k = 0; k < NC; k++) {
+      if (k < NL_) offs[(size_t)b * NL_ + k] = tot[k];
+      tot[k] += cnt[(size_t)b * NC + k];
+    }
+  for (int k = 0; k < NC; k++) counts[k] = (int64_t)tot[k];
+  bool fits = true, brought = false;
+  for (int k = 0; k < NL_; k++) {
+    fits &= (int64_t)tot[k] <= caps[k];
+    brought |= hosts[k] && caps[k];
+  }
+  if (!fits) {
+    cleanup();
+    if (brought) return fail_(e, MSKG_ECAPACITY, "frame: the records do not fit the buffers given");
+    return MSKG_OK;  // sizing call
+  }
+  for (int k = 0; k < NL_; k++)
+    if (hipMalloc(&d_out[k], std::max<unsigned long long>(1, tot[k]) * sizeof(int4)) != hipSuccess) {
+      cleanup();
+      return fail_(e, MSKG_ENOMEM, "frame: no device memory for the records");
+    }
+  ce = hipMemcpyAsync(d_offs, offs.data(), offs.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                      e->stream);
+  if (ce == hipSuccess) {
+    hipLaunchKernelGGL(k_frame_write<MODE>, dim3(nblk), dim3(FRAME_THREADS), 0, e->stream, L, chunk, sic, d_offs,
+                       d_out[0], d_out[1], d_out[2]);
+    ce = hipGetLastError();
+  }
+  for (int k = 0; k < NL_ && ce == hipSuccess; k++)
+    if (tot[k] && hosts[k])
+      ce = hipMemcpyAsync(hosts[k], d_out[k], tot[k] * sizeof(int4), hipMemcpyDeviceToHost, e->stream);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
+  cleanup();
+  if (ce != hipSuccess) return fail_(e, MSKG_EDEVICE, "frame write: %s", hipGetErrorString(ce));
+  return MSKG_OK;
+}
+
+// The flips of the SiC wrong-site analysis in the reference's grid order
+// (see k_wrong_cand): the positions of the atoms that flip.
+static int wrong_flips(mskg_engine *e, std::vector<int4> &flips) {
+  flips.clear();
+  Lattice L = make_lattice(e);
+  const int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
+  int64_t cap = 1 << 16;
+  for (;;) {
+    unsigned long long *d_n = nullptr;
+    int4 *d_c = nullptr;
+    if (hipMalloc(&d_n, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&d_c, (size_t)cap * 5 * sizeof(int4)) != hipSuccess) {
+      hipFree(d_n);
+      return fail_(e, MSKG_ENOMEM, "frame: no device memory for the wrong-site candidates");
+    }
+    unsigned long long n = 0;
+    std::vector<int4> c;
+    hipError_t ce = hipMemsetAsync(d_n, 0, sizeof n, e->stream);
+    if (ce == hipSuccess) {
+      hipLaunchKernelGGL(k_wrong_cand, dim3(blocks), dim3(256), 0, e->stream, L, cap, d_n, d_c);
+      ce = hipGetLastError();
+    }
+    if (ce == hipSuccess) ce = hipMemcpyAsync(&n, d_n, sizeof n, hipMemcpyDeviceToHost, e->stream);
+    if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
+    if (ce == hipSuccess && (int64_t)n <= cap) {
+      c.resize((size_t)n * 5);
+      if (n) ce = hipMemcpy(c.data(), d_c, c.size() * sizeof(int4), hipMemcpyDeviceToHost);
+    }
+    hipFree(d_n);
+    hipFree(d_c);
+    if (ce != hipSuccess) return fail_(e, MSKG_EDEVICE, "frame wrong-site candidates: %s", hipGetErrorString(ce));
+    if ((int64_t)n > cap) {
+      cap = (int64_t)n;
+      continue;
+    }
+    // grid order (z, then y, then x); a candidate flips unless a candidate
+    // neighbour before it did
+    const Box &b = e->box;
+    auto key = [&](const int4 &q) { return ((int64_t)q.z * b.ly + q.y) * b.lx + q.x; };
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t bb) { return key(c[5 * a]) < key(c[5 * bb]); });
+    std::vector<int64_t> flipped_keys;
+    for (size_t oi = 0; oi < n; oi++) {
+      const size_t i = order[oi];
+      bool flip = true;
+      for (int k = 0; k < 4 && flip; k++) {
+        const int64_t q = key(c[5 * i + 1 + k]);
+        flip = !std::binary_search(flipped_keys.begin(), flipped_keys.end(), q);
+      }
+      if (flip) {
+        flipped_keys.push_back(key(c[5 * i]));  // keys arrive in increasing order
+        flips.push_back(c[5 * i]);
+      }
+    }
+    return MSKG_OK;
+  }
+}
+
+static int set_wrong_marks(mskg_engine *e, const std::vector<int4> &flips, int set) {
+  if (flips.empty()) return MSKG_OK;
+  int4 *d = nullptr;
+  HIPCHK(e, hipMalloc(&d, flips.size() * sizeof(int4)));
+  hipError_t ce = hipMemcpyAsync(d, flips.data(), flips.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream);
+  if (ce == hipSuccess) {
+    const int blocks = (int)std::min<size_t>((flips.size() + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_wrong_set, dim3(blocks), dim3(256), 0, e->stream, make_lattice(e), d,
+                       (int64_t)flips.size(), set);
+    ce = hipGetLastError();
+  }
+  if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
+  hipFree(d);
+  if (ce != hipSuccess) return fail_(e, MSKG_EDEVICE, "frame wrong-site marks: %s", hipGetErrorString(ce));
   return MSKG_OK;
 }
 
@@ -2148,6 +2436,7 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
     RunState s0 = st;
     s0.tau = 0.0;  // k_sweep_end with tau = 0 only sets the next window
     s0.stop = 0;
+    s0.final_sweep = 0;
     memset(s0.slot, 0, sizeof s0.slot);
     s0.tau_policy = e->cfg.tau_policy;
     s0.events_per_pass = e->cfg.events_per_pass > 0 ? e->cfg.events_per_pass : 1.0;
@@ -2374,6 +2663,18 @@ int mskg_last_timing(mskg_engine *e, double *run_ms, double *sweep_ms, int64_t *
   if (sweep_ms) *sweep_ms = e->sweep_ms;
   if (sweep_points) *sweep_points = e->sweep_points;
   if (sweeps_done) *sweeps_done = e->sweeps_done;
+  return MSKG_OK;
+}
+
+int mskg_run_diagnostics(mskg_engine *e, int64_t *guard_exits, int64_t *sweeps, double *tau, int64_t *n_mc) {
+  if (!e || !e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  RunState st;
+  int rc = read_state(e, st);
+  if (rc) return rc;
+  if (guard_exits) *guard_exits = (int64_t)st.guard_exits;
+  if (sweeps) *sweeps = (int64_t)st.sweeps_run;
+  if (tau) *tau = st.tau;
+  if (n_mc) *n_mc = (int64_t)st.n_mc;
   return MSKG_OK;
 }
 
@@ -2780,67 +3081,31 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
 int mskg_frame(mskg_engine *e, int32_t *main_rec, int64_t cap_main, int32_t *void_rec, int64_t cap_void,
                int32_t *noepi_rec, int64_t cap_noepi, int64_t counts[5]) {
   if (!e || !e->loaded || !counts) return fail_(e, MSKG_EINVAL, "frame: no state loaded");
-  const int64_t n = e->box.npts;
-  const int nblk = (int)std::min<int64_t>(std::max<int64_t>(1, (n + 65535) / 65536), 65536);
-  const int64_t chunk = (n + nblk - 1) / nblk;
-  Lattice L = make_lattice(e);
-  unsigned long long *d_counts = nullptr, *d_offs = nullptr;
-  int4 *d_out[3] = {nullptr, nullptr, nullptr};
-  auto cleanup = [&]() {
-    hipFree(d_counts);
-    hipFree(d_offs);
-    for (int k = 0; k < 3; k++) hipFree(d_out[k]);
-  };
-  if (hipMalloc(&d_counts, (size_t)nblk * 5 * sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&d_offs, (size_t)nblk * 3 * sizeof(unsigned long long)) != hipSuccess) {
-    cleanup();
-    return fail_(e, MSKG_ENOMEM, "frame: no device memory for the block counts");
+  int32_t *const hosts[3] = {main_rec, void_rec, noepi_rec};
+  const int64_t caps[3] = {cap_main, cap_void, cap_noepi};
+  return frame_lists<FRAME_MAIN>(e, 0, hosts, caps, counts);
+}
+
+int mskg_frame_defects(mskg_engine *e, int32_t *vac_rec, int64_t cap_vac, int32_t *wrong_rec, int64_t cap_wrong,
+                       int64_t counts[2]) {
+  if (!e || !e->loaded || !counts) return fail_(e, MSKG_EINVAL, "frame: no state loaded");
+  // the wrong-occupation analysis runs for two crystal species one of which is C
+  const int sic = e->ncrystal == 2 &&
+                  (e->cfg.list_crystal[0] == 6 || e->cfg.list_crystal[1] == 6 || e->cfg.list_crystal[2] == 6);
+  std::vector<int4> flips;
+  if (sic) {
+    if (int rc = wrong_flips(e, flips)) return rc;
+    if (int rc = set_wrong_marks(e, flips, 1)) return rc;
   }
-  hipLaunchKernelGGL(k_frame_count, dim3(nblk), dim3(FRAME_THREADS), 0, e->stream, L, chunk, d_counts);
-  std::vector<unsigned long long> cnt((size_t)nblk * 5), offs((size_t)nblk * 3);
-  hipError_t ce = hipGetLastError();
-  if (ce == hipSuccess)
-    ce = hipMemcpyAsync(cnt.data(), d_counts, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                        e->stream);
-  if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
-  if (ce != hipSuccess) {
-    cleanup();
-    return fail_(e, MSKG_EDEVICE, "frame count: %s", hipGetErrorString(ce));
+  int32_t *const hosts[2] = {vac_rec, wrong_rec};
+  const int64_t caps[2] = {cap_vac, cap_wrong};
+  const int rc = frame_lists<FRAME_DEFECTS>(e, sic, hosts, caps, counts);
+  if (sic) {
+    const std::string msg = e->err;
+    if (int rc2 = set_wrong_marks(e, flips, 0)) return rc2;  // the lattice is left as it was
+    e->err = msg;
   }
-  unsigned long long tot[5] = {0, 0, 0, 0, 0};
-  for (int b = 0; b < nblk; b++)
-    for (int k = 0; k < 5; k++) {
-      if (k < 3) offs[(size_t)b * 3 + k] = tot[k];
-      tot[k] += cnt[(size_t)b * 5 + k];
-    }
-  for (int k = 0; k < 5; k++) counts[k] = (int64_t)tot[k];
-  if ((int64_t)tot[0] > cap_main || (int64_t)tot[1] > cap_void || (int64_t)tot[2] > cap_noepi) {
-    cleanup();
-    if ((main_rec && cap_main) || (void_rec && cap_void) || (noepi_rec && cap_noepi))
-      return fail_(e, MSKG_ECAPACITY, "frame: %lld / %lld / %lld records do not fit", (long long)tot[0],
-                   (long long)tot[1], (long long)tot[2]);
-    return MSKG_OK;  // sizing call
-  }
-  for (int k = 0; k < 3; k++)
-    if (hipMalloc(&d_out[k], std::max<unsigned long long>(1, tot[k]) * sizeof(int4)) != hipSuccess) {
-      cleanup();
-      return fail_(e, MSKG_ENOMEM, "frame: no device memory for the records");
-    }
-  ce = hipMemcpyAsync(d_offs, offs.data(), offs.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
-                      e->stream);
-  if (ce == hipSuccess) {
-    hipLaunchKernelGGL(k_frame_write, dim3(nblk), dim3(FRAME_THREADS), 0, e->stream, L, chunk, d_offs, d_out[0],
-                       d_out[1], d_out[2]);
-    ce = hipGetLastError();
-  }
-  int32_t *hosts[3] = {main_rec, void_rec, noepi_rec};
-  for (int k = 0; k < 3 && ce == hipSuccess; k++)
-    if (tot[k] && hosts[k])
-      ce = hipMemcpyAsync(hosts[k], d_out[k], tot[k] * sizeof(int4), hipMemcpyDeviceToHost, e->stream);
-  if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
-  cleanup();
-  if (ce != hipSuccess) return fail_(e, MSKG_EDEVICE, "frame write: %s", hipGetErrorString(ce));
-  return MSKG_OK;
+  return rc;
 }
 
 }  // extern "C"

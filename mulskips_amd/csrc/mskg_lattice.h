@@ -273,10 +273,8 @@ MSKG_HD U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
   }
   return c;
 }
-// Uniform in the open interval (0,1) from 32 random bits, and from the top
-// 24 bits in FP32.
+// Uniform in the open interval (0,1) from 32 random bits.
 MSKG_HD double u01_32(uint32_t a) { return ((double)a + 0.5) * (1.0 / 4294967296.0); }
-MSKG_HD float u01f(uint32_t a) { return ((float)(a >> 8) + 0.5f) * (1.0f / 16777216.0f); }
 // Uniform in the open interval (0,1) with 53 random bits.
 MSKG_HD double u01_open(uint32_t a, uint32_t b) {
   uint64_t k = (((uint64_t)a << 32) | b) >> 11;

@@ -209,3 +209,109 @@ def deck_from_tables(tables, init_st, lens, max_iter=10 ** 9, run_type="R", idum
     lc = [int(v) for v in tables["list_crystal"][:ncr]]
     return Deck(ncr, 0, lc, [], init_st, list(lens), float(tables["ptrans_zig"]), sf, "Iter", max_iter=max_iter,
                 out_molmol=max_iter, run_type=run_type, idum=idum, evap=evap, depo=depo)
+
+
+# ---- start.dat as pymulskips writes it --------------------------------------
+# The header comments and line formats of setuprun.py:292-389, kept
+# character for character: a deck written here is byte-identical to the one
+# pymulskips writes for the same run, so either can drive the host.
+INITSTAT_NOTE = (" ! Initstat: S Sphere, C Parellelepipid, F Flat SiC-3C(100) surface, FS Flat Si(100) surface, "
+                 "A APB, I inverted pyramid, D inverted pyramid of C, Z inverted pyramid of Si, J inverted pyramid "
+                 "with APB, Fw Flat SiC(100) surface with walls, Fa Flat SiC(100) surface with aperture, SL Si(100) "
+                 "liquid-solid interface, FF FinFET Si (100) liquid-solid, SS Sphere Si liquid-solid")
+PROB_FLOOR = 5e-12  # setuprun.py:355: every rate written is at least this
+
+
+def _fmt15(v):
+    return "{:.15f}".format(v)
+
+
+def pymulskips_start_dat(run):
+    """The text of the start.dat pymulskips' run_mulskips writes
+    (setuprun.py:292-389) for the run described by `run`, a dict of its
+    arguments and process data: Simulation, listcryZ, listcovZ, KMC_sf,
+    Seed_box, PtransZig, ExitStrategy with IterMax / OutMolMol or TotTime /
+    OutTime, RunType, IDUM, SaveFinalState (+ restartfilename), SaveCoo (+
+    coofilename), path (the working directory the file names are joined to),
+    the rate arrays PtransE (species, n_1, ..) / PtransD (species, 3) and for
+    coverage species PtransAbs / PtransDes (cov, 3) with the evaporation index
+    list `indices`; for 'LA' also driver, cadfilename, tempfilename, Tm, P0,
+    A, Tflex, LenVac, LenNuc, initialState, LenSiGe; for SiGe X0 (the Ge
+    fraction); for 'ST' useProbTable / ProbTable; for 'IN' / 'SH'
+    cadfilename.  Every rate is written floored at 5e-12 with Python's str()
+    of the float."""
+    sim = run["Simulation"]
+    cz, vz = list(run["listcryZ"]), list(run.get("listcovZ", []))
+    ncry, ncov = len(cz), len(vz)
+    path = run.get("path", ".")
+    driver = run.get("driver")
+    o = [f"{ncry} {ncov} ! NCrystal, NCov \n",
+         "{} ! Atomic numbers of crystal species \n".format(" ".join(str(x) for x in cz))]
+    if ncov:
+        o.append("{} ! Atomic numbers of coverage species \n".format(" ".join(str(x) for x in vz)))
+    o.append(sim + INITSTAT_NOTE + "\n")
+    if sim in ("IN", "SH"):
+        o.append(path + "/" + run["cadfilename"] + "\n")
+    elif sim == "LA":
+        if driver is None:
+            o += [path + "/" + run["cadfilename"] + "\n", path + "/" + run["tempfilename"] + "\n"]
+        else:
+            o += ["None\n", "None\n"]
+        o.append("{} ! Melting temperature in Kelvin \n".format(" ".join(str(x) for x in run["Tm"])))
+        o.append("{} ! prefactor P0 \n".format(" ".join(str(x) for x in run["P0"])))
+        o.append("{} ! errf param A \n".format(" ".join(str(x) for x in run["A"])))
+        o.append("{} ! errf param Tflex \n".format(" ".join(str(x) for x in run["Tflex"])))
+        o.append(_fmt15(run["LenVac"]) + " ! LenVac -> Top air thickness in Ang\n")
+        o.append(_fmt15(run["LenNuc"]) + " ! LenNuc -> Nucleus radius in Ang\n")
+        flag = {"homogeneous": "T", "inhomogeneous": "F", "tripleSF": "D", "stripe": "R"}[run["initialState"]]
+        o.append(flag + " ! Initial Nucleation State [homogeneous, inhomogeneous, tripleSF, stripe] \n")
+        if run["initialState"] == "tripleSF":
+            o.append(" ".join(str(v) for v in run["Seed_box"]) + " ! Len1 Len2 Len3 Len4 Len5 for configuring a triple SF\n")
+    else:
+        o.append(" ".join(str(v) for v in run["Seed_box"]) + " ! Len1 [Len2 Len3 Len4 Len5]\n")
+    if 14 in cz and 32 in cz:
+        o.append(str(run["X0"]) + " ! Ge fraction in substrate \n")
+        if sim == "LA":
+            o.append(_fmt15(run["LenSiGe"]) + " ! LenSiGe -> SiGe thickness in Ang\n")
+    o.append(str(run["PtransZig"]) + " ! PtransZig\n")
+    o.append(_fmt15(run["KMC_sf"]) + " ! KMC_sf -> KMC Super-Lattice parameter in Ang\n")
+    ex = run["ExitStrategy"]
+    o.append(ex + " ! ExitStrategy. Iter or Time. Decides whether to use iterations or time to check simulation "
+                  "end and output frequency\n")
+    if ex == "Iter":
+        o.append(str(run["IterMax"]) + " ! IterMax -> Max number of iterations\n")
+        o.append(str(run["OutMolMol"]) + " ! OutMolMol -> output frequency\n")
+    if ex == "Time":
+        o.append(_fmt15(run["TotTime"]) + " ! TotTime -> Max time of simulation in seconds\n")
+        o.append(_fmt15(run["OutTime"]) + " ! OutTime -> Delta Time between consecutive print, in seconds\n")
+    o.append(run["RunType"] + " ! RunType. R = truly random . T = not so random (use this for regression tests). "
+                              "C = continuation run\n")
+    o.append(str(run["IDUM"]) + " ! Random seed\n")
+    o.append(("T" if run.get("SaveFinalState") else "F") + " ! Save final state\n")
+    if run.get("SaveFinalState"):
+        o.append(path + "/" + run["restartfilename"] + "\n")
+    o.append(("T" if run.get("SaveCoo") else "F") + " ! Save final Coor\n")
+    if run.get("SaveCoo"):
+        o.append((path + "/" + run["coofilename"] if driver is None else "None") + "\n")
+    if sim == "ST" and run.get("useProbTable"):
+        o.append(run["ProbTable"] + "\n")
+    else:
+        pe = np.asarray(run["PtransE"], np.float64)
+        pd = np.asarray(run["PtransD"], np.float64)
+        for s in range(ncry):
+            for ind in np.asarray(run["indices"]):
+                digits = "".join(str(v) for v in ind)
+                prob = np.amax([PROB_FLOOR, pe[(s,) + tuple(int(v) for v in ind)]])
+                o.append("{} {} {} ! PtransE[{},{}] \n".format(s + 1, digits, prob, s + 1, ind))
+        for s in range(ncry):
+            for c in range(3):
+                o.append("{} {} {} ! PtransD[{},{}] \n".format(s + 1, c + 1, np.amax([PROB_FLOOR, pd[s, c]]), s + 1,
+                                                               c + 1))
+        if ncov:
+            for name in ("PtransAbs", "PtransDes"):
+                t = np.asarray(run[name], np.float64)
+                for s in range(ncry, ncry + ncov):
+                    for c in range(3):
+                        o.append("{} {} {} ! {}[{},{}] \n".format(s + 1, c + 1, np.amax([PROB_FLOOR, t[s - ncry, c]]),
+                                                                  name, s + 1, c + 1))
+    return "".join(o)

@@ -8,6 +8,7 @@ event log) and fetched back with `store_state`, through the C ABI of
 include/mskg.h.  Errors the reference turns into STOP become exceptions.
 """
 import ctypes as C
+import warnings
 
 import numpy as np
 
@@ -85,6 +86,7 @@ class Engine:
                 raise ValueError("second_factor needs 13 values (0..12 occupied second neighbours)")
             cfg.second_factor = self._arrays["second_factor"].ctypes.data
         self._cfg = cfg
+        self._guard_seen = 0
         h = C.c_void_p()
         rc = self.L.mskg_create(C.byref(cfg), C.byref(h))
         self.h = h
@@ -152,6 +154,16 @@ class Engine:
         self._check(self.L.mskg_frame(self.h, ptr(bufs[0]), int(c[0]), ptr(bufs[1]), int(c[1]), ptr(bufs[2]),
                                       int(c[2]), ptr(c)))
         return bufs[0][:c[0]], bufs[1][:c[1]], bufs[2][:c[2]], int(c[3]), int(c[4])
+
+    def frame_defects(self):
+        """The SiC defect frames' records (mskg_frame_defects): (vacancies
+        with the species sum of their neighbours, wrong occupations with the
+        flipped species), (x, y, z, code) rows in grid order."""
+        c = np.zeros(2, np.int64)
+        self._check(self.L.mskg_frame_defects(self.h, None, 0, None, 0, ptr(c)))
+        bufs = [np.zeros((max(int(c[k]), 1), 4), np.int32) for k in range(2)]
+        self._check(self.L.mskg_frame_defects(self.h, ptr(bufs[0]), int(c[0]), ptr(bufs[1]), int(c[1]), ptr(c)))
+        return bufs[0][:c[0]], bufs[1][:c[1]]
 
     def track_classes(self, on=True):
         """Keep the rate-class keys current through runs (likelihood segments)."""
@@ -268,12 +280,32 @@ class Engine:
         self._check(self.L.mskg_refresh_rates(self.h))
 
     def run(self, max_events, max_time=float("inf")):
+        """Synchronous-sublattice KMC until max_events events or the clock
+        reaches max_time (the last window is shortened to end there).
+        Returns (events, clock, stop reason).  Warns when a tile window hit
+        the per-window event cap (see diagnostics())."""
         n = C.c_int64()
         t = C.c_double()
         s = C.c_int32()
         mt = float(max_time) if np.isfinite(max_time) else 1e300
         self._check(self.L.mskg_run(self.h, int(max_events), mt, C.byref(n), C.byref(t), C.byref(s)))
+        g = self.diagnostics()["guard_exits"]
+        if g < self._guard_seen:  # a new state since: the count restarted
+            self._guard_seen = 0
+        if g > self._guard_seen:
+            warnings.warn(f"{g - self._guard_seen} tile windows ended at the per-window event cap: the window "
+                          "policy lets single tiles run far ahead (lower events_per_pass)", RuntimeWarning)
+            self._guard_seen = g
         return n.value, t.value, s.value
+
+    def diagnostics(self):
+        """Window diagnostics since the state was loaded: guard_exits (tile
+        windows cut at the per-window event cap), sweeps, the next window
+        tau and the MC particles at the last sweep."""
+        g, k, m = C.c_int64(), C.c_int64(), C.c_int64()
+        tau = C.c_double()
+        self._check(self.L.mskg_run_diagnostics(self.h, C.byref(g), C.byref(k), C.byref(tau), C.byref(m)))
+        return {"guard_exits": g.value, "sweeps": k.value, "tau": tau.value, "n_mc": m.value}
 
     def run_serial(self, max_events, max_time=float("inf")):
         """Serial BKL (one workgroup): see run_serial()."""

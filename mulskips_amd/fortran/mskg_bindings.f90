@@ -244,12 +244,29 @@ module mskg_bindings
       integer(c_int64_t), intent(out) :: counts(5)
     end function
 
+    ! the SiC defect frames (_v vacancy classes, _w wrong occupations); counts(2)
+    integer(c_int) function mskg_frame_defects(eng, vac_rec, cap_vac, wrong_rec, cap_wrong, counts) &
+        bind(c, name="mskg_frame_defects")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: eng, vac_rec, wrong_rec
+      integer(c_int64_t), value :: cap_vac, cap_wrong
+      integer(c_int64_t), intent(out) :: counts(2)
+    end function
+
     integer(c_int) function mskg_last_timing(eng, run_ms, sweep_ms, sweep_points, sweeps_done) &
         bind(c, name="mskg_last_timing")
       import :: c_int, c_int64_t, c_ptr, c_double
       type(c_ptr), value :: eng
       real(c_double), intent(out) :: run_ms, sweep_ms
       integer(c_int64_t), intent(out) :: sweep_points, sweeps_done
+    end function
+
+    integer(c_int) function mskg_run_diagnostics(eng, guard_exits, sweeps, tau, n_mc) &
+        bind(c, name="mskg_run_diagnostics")
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: eng
+      integer(c_int64_t), intent(out) :: guard_exits, sweeps, n_mc
+      real(c_double), intent(out) :: tau
     end function
   end interface
 

@@ -44,12 +44,14 @@ def film(latt, box, z0):
     height: mean of the column heights (highest atom in each 6x6 column);
     roughness: RMS deviation of the column heights;
     defects: fraction of the film's atoms on non-cubic (stacking-fault) sites;
-    coverage: covered points per surface site (atoms_per_layer)."""
+    coverage: covered points per surface site (atoms_per_layer);
+    species1, species2: the film's atoms of species 1 and 2."""
     lx, ly, lz = box
     occ = occupied(latt)
     z, y, x = np.nonzero(occ)
     keep = z >= z0
     z, y, x = z[keep], y[keep], x[keep]
+    sp = (latt[z, y, x] >> 27) & 7
     cols = np.full((ly // 6, lx // 6), z0 - 3, np.int64)
     np.maximum.at(cols, (y // 6, x // 6), z)
     defects = float(np.mean(~cubic_site(x, y, z))) if len(z) else 0.0
@@ -59,6 +61,8 @@ def film(latt, box, z0):
         "roughness": float(cols.std()),
         "defects": defects,
         "coverage": float(covered(latt).sum()) / atoms_per_layer(box),
+        "species1": int(np.count_nonzero(sp == 1)),
+        "species2": int(np.count_nonzero(sp == 2)),
     }
 
 

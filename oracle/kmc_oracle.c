@@ -76,6 +76,8 @@ struct orc_sim {
   /* clock and counters */
   double time; int64_t iter;
   int32_t count_crystal[3], count_crystal_old[3], count_cov[3];
+  int64_t executed[8];  /* events executed by type: deposits of species 1..3, evaporations,
+                           desorptions, absorptions of ListCov 1..3 (the engine's counters) */
   int last_arm;
   int force_arm;  /* >= 0: IF_Arm_Choice returns this instead of comparing/drawing */
   int frozen_below_z;  /* engine option restated: atoms below this z never evaporate */
@@ -1563,7 +1565,57 @@ static const char *sym_of(int z) {
   }
 }
 
+/* The vacancy frame (inpout/WriteMolMolXYZFile.f:222-260): NumVoids, then
+ * every vacancy in ListVoid order -- for SiC PVD (NCov 0, NCrystal 2,
+ * ListCrystal(2) = 6) classified by the species sum of its four NextNXYZ
+ * points (4 CV, 8 SV, 7 SAV, 5 CAV, 6 XV, anything else stops), otherwise as
+ * an 'O' line. */
+static void write_vacancy_frame(orc_sim *s, FILE *f, const double alat[3]) {
+  char e1[32], e2[32], e3[32];
+  fprintf(f, " %8d angstroem\n", s->n_vd);
+  fort_e(e1, alat[0], 15, 8); fort_e(e2, alat[1], 15, 8); fort_e(e3, alat[2], 15, 8);
+  fprintf(f, " periodic %s %s %s\n", e1, e2, e3);
+  const int sic_pvd = s->ncov == 0 && s->ncrystal == 2 && s->list_crystal[1] == 6;
+  for (int i = 1; i <= s->n_vd; i++) {
+    const int32_t *p = s->vd[i].xyz;
+    const double fx = s->sf * p[0], fy = s->sf * p[1], fz = s->sf * p[2];
+    if (!sic_pvd) {
+      fprintf(f, "O %10.5f %10.5f %10.5f  \n", fx, fy, fz);
+      continue;
+    }
+    int iat = 0;
+    for (int j = 0; j < 4; j++) iat += ibits(LC(s, s->vd[i].nn[j]), POS_SIC, LEN_SIC);
+    switch (iat) {
+      case 4: fprintf(f, "Mg %10.5f %10.5f %10.5f # CV (C vacancy)\n", fx, fy, fz); break;
+      case 8: fprintf(f, "He %10.5f %10.5f %10.5f # SV (Si vacancy)\n", fx, fy, fz); break;
+      case 7: fprintf(f, "Ne %10.5f %10.5f %10.5f # SAV (Si antisite vac)\n", fx, fy, fz); break;
+      case 5: fprintf(f, "Be %10.5f %10.5f %10.5f # CAV (C antisite vac)\n", fx, fy, fz); break;
+      case 6: fprintf(f, "Ar %10.5f %10.5f %10.5f # XV (near 2C and 2Si)\n", fx, fy, fz); break;
+      default: fail(s, "ERROR: Weird Defect type was formed...ABORTING");
+    }
+  }
+}
+
+/* NCoorCorr of WriteMolMolXYZFile.f:284-296: the first neighbours of a bulk
+ * atom of the other species (C bits around a Si atom, Si bits around a C). */
+static int ncoor_corr(orc_sim *s, int32_t w, const int32_t site[3]) {
+  const at_rec *b = atr(s, s->lind[gidx(s, site[0], site[1], site[2])]);
+  const int iat = ibits(w, POS_SIC, LEN_SIC);
+  int corr = 0;
+  if (iat == 1) for (int i = 0; i < 4; i++) corr += ibits(LC(s, b->nn[i]), POS_C, LEN_C);
+  else if (iat == 2) for (int i = 0; i < 4; i++) corr += ibits(LC(s, b->nn[i]), POS_SI, LEN_SI);
+  return corr;
+}
+
+int orc_write_frames(orc_sim *s, const char *xyz_path, const char *d_path, const char *v_path, const char *w_path,
+                     int64_t iter);
+
 int orc_write_frame(orc_sim *s, const char *xyz_path, const char *d_path, int64_t iter) {
+  return orc_write_frames(s, xyz_path, d_path, NULL, NULL, iter);
+}
+
+int orc_write_frames(orc_sim *s, const char *xyz_path, const char *d_path, const char *v_path, const char *w_path,
+                     int64_t iter) {
   if (setjmp(s->jb)) { s->have_jb = 0; return 2; }
   s->have_jb = 1;
   int lx = s->lx, ly = s->ly, lz = s->lz;
@@ -1635,6 +1687,12 @@ int orc_write_frame(orc_sim *s, const char *xyz_path, const char *d_path, int64_
               s->sf * s->vd[i].xyz[2]);
     fclose(f);
   }
+  if (v_path) {
+    FILE *fv = fopen(v_path, "w");
+    if (!fv) fail(s, "cannot write %s", v_path);
+    write_vacancy_frame(s, fv, alat);
+    fclose(fv);
+  }
   /* SiC "wrong" analysis and its side effect on LattCoo (:262-398): the
    * restore loop reads ListAtBuffer(NumIsWrong) instead of entry i, so
    * the species flip is only undone correctly when all flipped atoms had
@@ -1670,6 +1728,38 @@ int orc_write_frame(orc_sim *s, const char *xyz_path, const char *d_path, int64_
             s->abuf[nwrong].type = iat == 1 ? 1 : 2;
           }
         }
+    /* the wrong-occupation frame (:300-380), read while the isolated
+     * wrong atoms are flipped: bulk atoms with 3 neighbours of the other
+     * species, written as Si or C */
+    if (w_path) {
+      int64_t nat_wrong = 0;
+      for (int pass = 0; pass < 2; pass++) {
+        FILE *fw = NULL;
+        if (pass == 1) {
+          fw = fopen(w_path, "w");
+          if (!fw) fail(s, "cannot write %s", w_path);
+          fprintf(fw, " %8lld angstroem\n", (long long)nat_wrong);
+          fort_e(e1, alat[0], 15, 8); fort_e(e2, alat[1], 15, 8); fort_e(e3, alat[2], 15, 8);
+          fprintf(fw, " periodic %s %s %s\n", e1, e2, e3);
+        }
+        for (int z = 1; z <= lz - 1; z++)
+          for (int y = 0; y < ly; y++)
+            for (int x = 0; x < lx; x++) {
+              const int32_t w = s->latt[gidx(s, x, y, z)];
+              if (!(ibits(w, POS_OCC, LEN_OCC) == 1 && ibits(w, POS_COOR, LEN_COOR) == 4)) continue;
+              const int32_t site[3] = {x, y, z};
+              if (ncoor_corr(s, w, site) != 3) continue;
+              if (pass == 0) {
+                nat_wrong++;
+              } else {
+                const int iat = ibits(w, POS_SIC, LEN_SIC);
+                if (iat == 1) fprintf(fw, "Si %10.5f %10.5f %10.5f  \n", s->sf * x, s->sf * y, s->sf * z);
+                if (iat == 2) fprintf(fw, "C %10.5f %10.5f %10.5f  \n", s->sf * x, s->sf * y, s->sf * z);
+              }
+            }
+        if (fw) fclose(fw);
+      }
+    }
     for (int i = 1; i <= nwrong; i++) {
       int32_t *w = LCp(s, s->abuf[i].xyz);
       if (s->abuf[nwrong].type == 1) {
@@ -1729,6 +1819,10 @@ static void refresh_second(orc_sim *s, const int32_t site[3]) {
 
 static void do_execute(orc_sim *s, int32_t ind, int32_t itrans) {
   s->last_arm = -1;
+  {
+    const int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
+    if (slot >= 0 && slot < 8) s->executed[slot]++;
+  }
   int32_t site[3];
   cp3(site, s->ad[ind].xyz);
   if (itrans <= 3) {
@@ -1781,6 +1875,22 @@ int orc_run(orc_sim *s, int64_t max_events, double max_time, int64_t *done) {
   return rc;
 }
 
+/* The state at the KMC time t_end: every event whose clock (Time + 1/R,
+ * main.f:313) stays at or below t_end is executed, the first one past it is
+ * not -- unlike the reference's loop, which exits after executing that
+ * event (main.f:363-366).  Used to compare states sampled at one time. */
+int orc_run_to(orc_sim *s, int64_t max_events, double t_end, int64_t *done) {
+  int64_t n = 0;
+  int rc = 0;
+  while (n < max_events && s->tree && s->tree[1] > 0.0 && s->time + 1.0 / s->tree[1] <= t_end) {
+    rc = orc_step(s, NULL);
+    if (rc) break;
+    n++;
+  }
+  if (done) *done = n;
+  return rc;
+}
+
 int orc_apply(orc_sim *s, const int32_t site[3], int32_t itrans, int32_t arm) {
   if (setjmp(s->jb)) { s->have_jb = 0; s->force_arm = -1; return 2; }
   s->have_jb = 1;
@@ -1792,6 +1902,48 @@ int orc_apply(orc_sim *s, const int32_t site[3], int32_t itrans, int32_t arm) {
   do_execute(s, ind, itrans);
   s->force_arm = -1;
   s->iter++;
+  s->have_jb = 0;
+  return 0;
+}
+
+/* A state handed over from elsewhere (the engine's store_state layout, the
+ * same hand-over as the reference's builders leave at main.f:298): LattCoo,
+ * then the three lists with their NextNXYZ, added in the given order with
+ * AddMC / AddBulk / AddVoid (Add.f:24-101) so LattInd and the tree follow.
+ * A listed MC particle without a rate (an atom of a frozen substrate) takes
+ * the vanishing rate 1e-300, as orc_set_frozen gives it; orc_refresh_all
+ * re-evaluates every rate afterwards. */
+int orc_load_state(orc_sim *s, const int32_t *latt, int32_t n_ad, const int32_t *ad_xyz, const int32_t *ad_nn,
+                   const int32_t *ad_ev, const double *ad_prob, int32_t n_bulk, const int32_t *bulk_xyz,
+                   const int32_t *bulk_nn, int32_t n_void, const int32_t *void_xyz, const int32_t *void_nn,
+                   double time) {
+  if (setjmp(s->jb)) { s->have_jb = 0; return 2; }
+  s->have_jb = 1;
+  if (!s->tree) allocate_tree(s);
+  if (n_ad >= NUM_MC_MAX || n_void >= NUM_VD_MAX || n_bulk >= NUM_AT_MAX) fail(s, "orc_load_state: list too long");
+  memcpy(s->latt, latt, (size_t)s->npts * sizeof(int32_t));
+  memset(s->lind, 0, (size_t)s->npts * sizeof(int32_t));
+  memset(s->tree, 0, ((size_t)s->size_tree + 2) * sizeof(double));
+  memset(s->ad, 0, (size_t)(NUM_MC_MAX + 1) * sizeof(mc_rec));
+  memset(s->vd, 0, (size_t)(NUM_VD_MAX + 1) * sizeof(at_rec));
+  if (s->at) memset(s->at, 0, (size_t)s->cap_at * sizeof(at_rec));
+  s->n_ad = s->n_at = s->n_vd = 0;
+  s->last_key = -1;
+  int32_t nn[4][3];
+  for (int32_t i = 0; i < n_ad; i++) {
+    memcpy(nn, ad_nn + 12 * (int64_t)i, sizeof nn);
+    add_mc(s, ad_xyz + 3 * (int64_t)i, nn, ad_ev[i], ad_prob[i] > 0.0 ? ad_prob[i] : 1e-300);
+  }
+  for (int32_t i = 0; i < n_bulk; i++) {
+    memcpy(nn, bulk_nn + 12 * (int64_t)i, sizeof nn);
+    add_bulk(s, bulk_xyz + 3 * (int64_t)i, nn);
+  }
+  for (int32_t i = 0; i < n_void; i++) {
+    memcpy(nn, void_nn + 12 * (int64_t)i, sizeof nn);
+    add_void(s, void_xyz + 3 * (int64_t)i, nn);
+  }
+  s->time = time;
+  s->iter = 1;
   s->have_jb = 0;
   return 0;
 }
@@ -2151,6 +2303,7 @@ int32_t *orc_lattind(orc_sim *s) { return s->lind; }
 void orc_dims(const orc_sim *s, int *lx, int *ly, int *lz) { *lx = s->lx; *ly = s->ly; *lz = s->lz; }
 void orc_counts(const orc_sim *s, int32_t *a, int32_t *b, int32_t *c) { *a = s->n_ad; *b = s->n_at; *c = s->n_vd; }
 double orc_time(const orc_sim *s) { return s->time; }
+void orc_executed(const orc_sim *s, int64_t out[8]) { memcpy(out, s->executed, sizeof s->executed); }
 double orc_total_rate(const orc_sim *s) { return s->tree ? s->tree[1] : 0.0; }
 int64_t orc_iter(const orc_sim *s) { return s->iter; }
 void orc_count_crystal(const orc_sim *s, int32_t cc[3], int32_t cov[3]) {

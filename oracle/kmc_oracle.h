@@ -107,12 +107,28 @@ int orc_step(orc_sim *s, orc_event *ev);
  * first event at or past max_time, as main.f:363-366 does).  Returns as
  * orc_step; *done receives the events executed. */
 int orc_run(orc_sim *s, int64_t max_events, double max_time, int64_t *done);
+/* As orc_run, but stopping before the first event whose clock would pass
+ * t_end: the state at the KMC time t_end. */
+int orc_run_to(orc_sim *s, int64_t max_events, double t_end, int64_t *done);
+/* Events executed since creation, by type (deposits of species 1..3,
+ * evaporations, desorptions, absorptions of ListCov 1..3). */
+void orc_executed(const orc_sim *s, int64_t out[8]);
 
 /* Execute one given event (Ind_Event itrans at the MC particle at site, with
  * the stacking choice forced to arm when >= 0) with the reference's update
  * rules: replays an event log produced elsewhere, e.g. by the engine's
  * parallel run.  Returns 0, or 2 if the site is not an MC particle. */
 int orc_apply(orc_sim *s, const int32_t site[3], int32_t itrans, int32_t arm);
+
+/* Replace the state by a handed-over one: LattCoo (len_x*len_y*(len_z+1),
+ * x fastest) and the MC / bulk / void lists with NextNXYZ (3x4 ints per
+ * record), Ind_Event and ProbTrans for MC particles -- the layout of the
+ * engine's mskg_store_state.  The tables and options stay as they are;
+ * call orc_refresh_all to re-evaluate the rates in FP64. */
+int orc_load_state(orc_sim *s, const int32_t *latt, int32_t n_ad, const int32_t *ad_xyz, const int32_t *ad_nn,
+                   const int32_t *ad_ev, const double *ad_prob, int32_t n_bulk, const int32_t *bulk_xyz,
+                   const int32_t *bulk_nn, int32_t n_void, const int32_t *void_xyz, const int32_t *void_nn,
+                   double time);
 
 /* The engine's frozen-substrate option: atoms with z < z never evaporate.
  * Here they stay MC particles with the vanishing rate 1e-300, so the
@@ -162,6 +178,11 @@ int orc_adjoint_stats(orc_sim *s, const orc_event *evs, int64_t n, const double 
  * wrong-site side effect on LattCoo) exactly as the reference does. */
 int orc_write_frame(orc_sim *s, const char *xyz_path, const char *d_path,
                     int64_t iter);
+/* The same, plus the SiC defect frames of WriteMolMolXYZFile.f: the vacancy
+ * frame (_v, :222-260) and the wrong-occupation frame (_w, :264-398; only
+ * for NCrystal 2 with C).  Either path may be NULL. */
+int orc_write_frames(orc_sim *s, const char *xyz_path, const char *d_path, const char *v_path,
+                     const char *w_path, int64_t iter);
 
 /* ---- state access (for building device states and comparing) --------- */
 /* The laser-annealing coupling maps (layout 0: DIMENSION(LenZ, LenX*LenY),

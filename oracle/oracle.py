@@ -44,6 +44,7 @@ def _load():
         L.orc_step.argtypes = [P, P]
         L.orc_apply.argtypes = [P, P, I, I]
         L.orc_set_frozen.argtypes = [P, I]
+        L.orc_load_state.argtypes = [P, P, I, P, P, P, P, I, P, P, I, P, P, D]
         L.orc_set_second_factor.argtypes = [P, P]
         L.orc_refresh_all.argtypes = [P]
         L.orc_set_la.argtypes = [P, D, D, D, D]
@@ -51,8 +52,11 @@ def _load():
         L.orc_la_counts.argtypes = [P, P, P]
         L.orc_set_st_tables.argtypes = [P, I, P, P, P, P, P]
         L.orc_run.argtypes = [P, I64, D, P]
+        L.orc_run_to.argtypes = [P, I64, D, P]
+        L.orc_executed.argtypes = [P, P]
         L.orc_adjoint_stats.argtypes = [P, P, I64, P, P, P, P]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
+        L.orc_write_frames.argtypes = [P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, I64]
         L.orc_set_temperature_field.argtypes = [P, P, I]
         L.orc_phases.argtypes = [P, P, P, I]
         L.orc_lattcoo.restype = C.POINTER(C.c_int32)
@@ -113,6 +117,25 @@ class Oracle:
     def load_deck(self, path):
         self._check(self.L.orc_load_deck(self.h, os.fsencode(path)), "orc_load_deck")
 
+    def load_state(self, lattcoo, mc, bulk, void, kmc_time=0.0):
+        """Replace the state by a handed-over one (the engine's store_state
+        layout): LattCoo (LenZ+1, LenY, LenX) and the three list dicts. The
+        tables and options stay; refresh_all() re-evaluates the rates."""
+        lat = np.ascontiguousarray(lattcoo, np.int32)
+        assert lat.shape == (self.dims[2] + 1, self.dims[1], self.dims[0]), lat.shape
+
+        def arr(d, k, dt, shape):
+            a = np.ascontiguousarray(d[k], dt)
+            return a if len(a) else np.zeros(shape, dt)
+        keep = [arr(mc, "xyz", np.int32, (1, 3)), arr(mc, "nn", np.int32, (1, 4, 3)),
+                arr(mc, "event", np.int32, (1,)), arr(mc, "prob", np.float64, (1,)),
+                arr(bulk, "xyz", np.int32, (1, 3)), arr(bulk, "nn", np.int32, (1, 4, 3)),
+                arr(void, "xyz", np.int32, (1, 3)), arr(void, "nn", np.int32, (1, 4, 3))]
+        rc = self.L.orc_load_state(self.h, _p(lat), len(mc["xyz"]), *[_p(a) for a in keep[:4]],
+                                   len(bulk["xyz"]), _p(keep[4]), _p(keep[5]), len(void["xyz"]), _p(keep[6]),
+                                   _p(keep[7]), float(kmc_time))
+        self._check(rc, "orc_load_state")
+
     def run_main(self, outdir=None, log_cap=0):
         log = np.zeros(max(log_cap, 1), EVENT_DTYPE)
         n = C.c_int64(0)
@@ -147,15 +170,23 @@ class Oracle:
             rc = self.L.orc_set_second_factor(self.h, _p(self._f2))
         self._check(rc, "orc_set_second_factor")
 
-    def run(self, max_events, max_time=float("inf")):
+    def run(self, max_events, max_time=float("inf"), at_time=False):
         """Serial loop until max_events or the clock reaches max_time;
-        returns (events, stop) with stop 0 limit, 1 few MC particles."""
+        returns (events, stop) with stop 0 limit, 1 few MC particles.
+        at_time=True stops before the first event past max_time instead of
+        after it (the state at the KMC time max_time)."""
         done = C.c_int64(0)
-        rc = self.L.orc_run(self.h, int(max_events), float(max_time) if np.isfinite(max_time) else 1e300,
-                            C.byref(done))
+        f = self.L.orc_run_to if at_time else self.L.orc_run
+        rc = f(self.h, int(max_events), float(max_time) if np.isfinite(max_time) else 1e300, C.byref(done))
         if rc == 2:
             self._check(rc, "orc_run")
         return done.value, rc
+
+    def executed(self):
+        """Events executed by type (the engine's counters layout)."""
+        out = np.zeros(8, np.int64)
+        self.L.orc_executed(self.h, _p(out))
+        return out
 
     def set_la(self, tm, p0, sigma, tflex):
         """Laser-annealing rates from the T bits; the tables hold energies."""
@@ -300,6 +331,12 @@ class Oracle:
     def write_frame(self, xyz_path, d_path=None, it=0):
         self._check(self.L.orc_write_frame(self.h, os.fsencode(xyz_path), os.fsencode(d_path) if d_path else None,
                                            it), "orc_write_frame")
+
+    def write_frames(self, xyz_path=None, d_path=None, v_path=None, w_path=None, it=0):
+        """The main, _d, _v and _w frames of WriteMolMolXYZFile.f (any path may be None)."""
+        enc = lambda p: os.fsencode(p) if p else None  # noqa: E731
+        self._check(self.L.orc_write_frames(self.h, enc(xyz_path), enc(d_path), enc(v_path), enc(w_path), it),
+                    "orc_write_frames")
 
 
 def find_nn(s1, s2, sc, lx, ly):

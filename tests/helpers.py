@@ -51,9 +51,14 @@ def oracle_state(o):
 
 
 def parity_mask(box):
+    """The lattice points of the grid (x = y = z mod 2), as a bool array of
+    LattCoo's shape (LenZ+1, LenY, LenX), built by broadcasting so that a
+    large grid costs one byte per point."""
     lx, ly, lz = box
-    z, y, x = np.meshgrid(np.arange(lz + 1), np.arange(ly), np.arange(lx), indexing="ij")
-    return (((x ^ y) & 1) == 0) & (((x ^ z) & 1) == 0)
+    z = (np.arange(lz + 1, dtype=np.int8) & 1)[:, None, None]
+    y = (np.arange(ly, dtype=np.int8) & 1)[None, :, None]
+    x = (np.arange(lx, dtype=np.int8) & 1)[None, None, :]
+    return (x == y) & (x == z)
 
 
 def list_map(d):
@@ -62,22 +67,27 @@ def list_map(d):
 
 
 def oracle_run_to_time(o, t_end):
-    """Run the oracle's serial loop until its clock reaches t_end (or it
-    stops); returns the number of events."""
-    n, _ = o.run(10 ** 15, t_end)
+    """Run the oracle's serial loop to the KMC time t_end (or until it
+    stops): the state at t_end, every event up to t_end executed and none
+    after it, as the engine stops its last window at t_end.  Returns the
+    number of events."""
+    n, _ = o.run(10 ** 15, t_end, at_time=True)
     return n
 
 
 def oracle_observables(args):
     """Worker (separate process): run one deck through the oracle to t_end
-    with RunType 'R' and seed idum, return the film observables."""
+    with RunType 'R' and seed idum, return the film observables, the events
+    and the depositions by species.  args: (path, box, idum, t_end, z0,
+    frozen[, second_factor])."""
     import os as _os
     import tempfile as _tf
 
     from mulskips_amd import deck as _deck
     from mulskips_amd import observables as _ob
     from oracle.oracle import Oracle as _Oracle
-    path, box, idum, t_end, z0, frozen = args
+    path, box, idum, t_end, z0, frozen = args[:6]
+    f2 = args[6] if len(args) > 6 else None
     d = _deck.read_start_dat(path)
     d.run_type, d.idum = "R", int(idum)
     with _tf.TemporaryDirectory() as tmp:
@@ -87,9 +97,14 @@ def oracle_observables(args):
         o.load_deck(p)
         if frozen:
             o.set_frozen(frozen)
+        if f2 is not None:
+            o.set_second_factor(f2)
         n = oracle_run_to_time(o, t_end)
         r = _ob.film(o.lattcoo(), box, z0)
         r["events"] = n
         r["time"] = o.time()
+        ex = o.executed()
+        r["dep1"], r["dep2"], r["evap"] = int(ex[0]), int(ex[1]), int(ex[3])
+
         o.close()
     return r

@@ -82,3 +82,41 @@ def test_capacity_beyond_lds_is_refused():
     from mulskips_amd.engine import Engine
     with pytest.raises(EngineError, match="tile_capacity 7000"):
         Engine((96, 96, 192), models.sic_c2_tables(), tile=24, tile_capacity=7000)
+
+
+def test_cache_key_holds_the_request(tmp_path):
+    """A choice made under allow_unvalidated (k = 8) or on a restricted tile
+    list is not returned for a default request; the result names the list
+    capacity the chosen trial ran with."""
+    tab = models.sic_c2_tables()
+    cache = str(tmp_path / "tune.json")
+    fast8 = Runner({(24, 4.0): 1.0, (24, 8.0): 5.0, (48, 4.0): 2.0, (48, 8.0): 3.0})
+    out = tune((96, 96, 192), tab, setup=None, events_per_pass=(4.0, 8.0), cache=cache, device="dev",
+               allow_unvalidated=True, runner=fast8)
+    assert (out["tile"], out["events_per_pass"], out["tile_capacity"]) == (24, 8.0, 1024)
+    assert all(t["lds_workgroups_per_cu"] == min(12, 160 * 1024 // (9 * t["tile_capacity"] + 2048))
+               for t in out["trials"])
+    r = Runner({(24, 4.0): 1.0, (48, 4.0): 2.0})
+    out2 = tune((96, 96, 192), tab, setup=None, events_per_pass=(4.0, 8.0)[:1], cache=cache, device="dev", runner=r)
+    assert not out2["cached"] and out2["events_per_pass"] == 4.0 and out2["tile"] == 48 and len(r.calls) == 2
+    assert out2["tile_capacity"] == tile_capacity(48)
+    r24 = Runner({(24, 4.0): 1.0})
+    out3 = tune((96, 96, 192), tab, setup=None, tiles=[24], events_per_pass=(4.0,), cache=cache, device="dev",
+                runner=r24)
+    assert not out3["cached"] and out3["tile"] == 24 and len(r24.calls) == 1
+    # the same request again comes from the cache; another starting state does not
+    assert tune((96, 96, 192), tab, setup=None, tiles=[24], events_per_pass=(4.0,), cache=cache, device="dev",
+                runner=Runner({}))["cached"]
+    r4 = Runner({(24, 4.0): 1.0})
+    assert not tune((96, 96, 192), tab, setup=None, tiles=[24], events_per_pass=(4.0,), cache=cache, device="dev",
+                    setup_tag="thicker slab", runner=r4)["cached"]
+    # an entry edited to an unvalidated window is refused for a default request
+    with open(cache) as f:
+        store = json.load(f)
+    k3 = out3["key"]
+    store[k3]["events_per_pass"] = 8.0
+    with open(cache, "w") as f:
+        json.dump(store, f)
+    r5 = Runner({(24, 4.0): 1.0})
+    assert not tune((96, 96, 192), tab, setup=None, tiles=[24], events_per_pass=(4.0,), cache=cache, device="dev",
+                    runner=r5)["cached"]

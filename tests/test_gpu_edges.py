@@ -3,7 +3,7 @@ every kernel still ends normally:
 * an empty lattice stops at once with the reference's "few MC particles"
   reason (main.f:307-310: NumAdAtom <= 20);
 * a run of zero events and the replay of an empty log change nothing;
-* a time limit stops the run at the first sweep that reaches it;
+* a time limit stops the run with the clock on the limit;
 * a logged event at a point off the lattice is reported, not executed;
 * a tile list longer than tile_capacity is reported, and so is a
   store_state buffer that is too small.
@@ -65,11 +65,16 @@ def test_zero_events_and_empty_replay_change_nothing(tmp_path):
 
 
 def test_time_limit(tmp_path):
+    """The last window of a run to a time limit is shortened to end there:
+    the clock stops on the limit exactly, and a limit far below one
+    window's length executes only the events due before it."""
     eng, box = loaded(tmp_path, tile=24, events_per_pass=4.0, tau_policy=1)
-    n1, t1, s1 = eng.run(10 ** 12, max_time=1e-3)
-    assert s1 == _lib.MSKG_STOP_LIMIT and n1 > 0 and t1 >= 1e-3
-    n2, t2, s2 = eng.run(10 ** 12, max_time=t1 + 1e-3)  # the limit is on the clock, not the call
-    assert s2 == _lib.MSKG_STOP_LIMIT and n2 > 0 and t2 >= t1 + 1e-3
+    n0, t0, s0 = eng.run(10 ** 12, max_time=1e-3)  # ~5e-6 events expected at the deck's 0.0054/s
+    assert s0 == _lib.MSKG_STOP_LIMIT and n0 == 0 and t0 == 1e-3
+    n1, t1, s1 = eng.run(10 ** 12, max_time=5000.0)  # ~27 events expected
+    assert s1 == _lib.MSKG_STOP_LIMIT and n1 > 0 and t1 == 5000.0
+    n2, t2, s2 = eng.run(10 ** 12, max_time=t1 + 5000.0)  # the limit is on the clock, not the call
+    assert s2 == _lib.MSKG_STOP_LIMIT and n2 > 0 and t2 == t1 + 5000.0
     assert eng.counters(lists=False)[1] == t2
     eng.close()
 

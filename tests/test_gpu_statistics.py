@@ -11,12 +11,19 @@ difference:
   with a frozen substrate, E = 1.0 eV per bond at 700 K, PtransZig 0.5;
 * growth and stacking-fault (non-cubic site) fraction: the cube-zigzag
   regression deck (PtransZig 0.8) on a 144^3 grid;
-* growth and coverage: the CVD-Si-SiH2 regression deck (H coverage).
+* growth and coverage: the CVD-Si-SiH2 regression deck (H coverage);
+* the benchmark's SiC model with its 0.05 eV second-neighbour term, at the
+  benchmark's tile edge 24 and 4 events per tile and pass: at 1600 K on a
+  frozen SiC substrate (the benchmark's own steady state: adatom population
+  and deposits by species) and at 900 K on the SiC parallelepiped of the
+  cube-zigzag deck (growth, roughness, stacking).
 Both window sizes the engine is run with are covered: about 1 and 4 events
-per tile and colour pass (tau policy 1).  The seeds are fixed and both
-sides repeat a run bit for bit, so the test is deterministic.
+per tile and colour pass (tau policy 1).  The event count is asserted like
+the observables.  The engine's last window ends at the time limit, so both
+sides stop at the same KMC time.  The seeds are fixed and both sides repeat
+a run bit for bit, so the test is deterministic.
 
-A case compares several observables, and the six cases hold about 24
+A case compares several observables, and the eight cases hold about 35
 comparisons, so a correct engine puts one of them outside 2 standard
 errors in most realisations.  The test is therefore sequential: an
 observable outside 2 standard errors at 32 seeds is measured again on 128
@@ -48,27 +55,58 @@ def si_pvd_deck(tmp_path):
     return path, d
 
 
+def sic2_deck(tmp_path, name):
+    """The SiC model of the large-replica benchmark (pair energies Si-Si 0.6 /
+    Si-C 1.1 / C-C 0.9 eV, 0.05 eV per occupied second-neighbour position,
+    Si and C fluxes 2 and 1.8 ML/s):
+    sic2-bench: at its 1600 K on a flat SiC slab over a frozen substrate --
+      every surface class evaporates faster than the flux arrives, so the
+      run is the flicker of adatoms on the frozen surface that the benchmark
+      times: its deposits of each species and its evaporations;
+    sic2-growth: at 900 K, where it grows, on the cube-zigzag deck's SiC
+      parallelepiped (PtransZig 0.8): growth, roughness and stacking."""
+    if name == "sic2-bench":
+        tab = models.sic_c2_tables()
+        d = deckmod.deck_from_tables(tab, "F", [72, 120, 120])
+    else:
+        tab = models.sic_c2_tables(temperature=900.0)
+        base, box, lens = SMALL["cube-zigzag"]
+        d = deckmod.deck_from_tables(tab, "C", list(lens), sf=0.38583333333333333)
+        d.ptrans_zig = 0.8
+    tab["ptrans_zig"] = d.ptrans_zig
+    path = os.path.join(str(tmp_path), f"start-{name}.dat")
+    deckmod.write_start_dat(d, path)
+    return path, d, tab
+
+
 def case(tmp_path, name):
-    """(deck path, deck, box, z0, frozen, t_end, observables)"""
+    """(deck path, engine tables, box, z0, frozen, t_end, observables)"""
     if name == "si-pvd":
         path, d = si_pvd_deck(tmp_path)
-        return path, d, (96, 96, 240), 36, 36, 60.0, ("atoms", "height", "roughness")
+        return path, d.tables(), (96, 96, 240), 36, 36, 60.0, ("atoms", "height", "roughness")
+    if name == "sic2-bench":
+        path, d, tab = sic2_deck(tmp_path, name)
+        return path, tab, (144, 144, 240), 60, 61, 4.0, ("dep1", "dep2", "evap")
+    if name == "sic2-growth":
+        path, d, tab = sic2_deck(tmp_path, name)
+        return path, tab, SMALL["cube-zigzag"][1], 0, 0, 4.0, ("atoms", "species1", "height", "roughness",
+                                                                "defects")
     base, box, lens = SMALL[name]
     path, d = small_deck(tmp_path, base, lens)
     if name == "cube-zigzag":
-        return path, d, box, 0, 0, 3.0e5, ("atoms", "defects")
+        return path, d.tables(), box, 0, 0, 3.0e5, ("atoms", "defects")
     if name == "CVD-Si-SiH2":
-        return path, d, box, 0, 0, 5.0, ("atoms", "coverage", "roughness")
+        return path, d.tables(), box, 0, 0, 5.0, ("atoms", "coverage", "roughness")
     raise KeyError(name)
 
 
-def serial_runs(path, box, t_end, z0, frozen, seeds):
+def serial_runs(path, box, t_end, z0, frozen, seeds, f2=None):
     ctx = mp.get_context("spawn")
     with ctx.Pool(min(8, len(seeds))) as pool:
-        return pool.map(oracle_observables, [(path, box, s, t_end, z0, frozen) for s in seeds])
+        return pool.map(oracle_observables, [(path, box, s, t_end, z0, frozen, f2) for s in seeds])
 
 
-def parallel_runs(path, d, box, t_end, z0, frozen, epp, seeds):
+def parallel_runs(path, tab, box, t_end, z0, frozen, epp, seeds):
     o = Oracle(*box)
     o.load_deck(path)
     if frozen:
@@ -77,13 +115,19 @@ def parallel_runs(path, d, box, t_end, z0, frozen, epp, seeds):
     o.close()
     out = []
     for s in seeds:
-        eng = Engine(box, d.tables(), seed=s, tile=24, tau_policy=1, events_per_pass=epp, frozen_below_z=frozen)
+        eng = Engine(box, tab, seed=s, tile=24, tau_policy=1, events_per_pass=epp, frozen_below_z=frozen)
         eng.load_state(lat0, mc0, bulk0, void0)
+        if tab.get("second_factor") is not None:
+            eng.refresh_rates()  # the listed atoms' rates with the second-neighbour factor
         n, t, stop = eng.run(10 ** 12, max_time=t_end)
-        assert stop == 0 and t >= t_end
+        # the last window ends at the time limit: the engine stops exactly there
+        assert stop == 0 and t == t_end, (stop, t)
+        assert eng.diagnostics()["guard_exits"] == 0
         lat, _, _, _ = eng.store_state()
         r = observables.film(lat, box, z0)
         r["events"] = n
+        counts = eng.counters(lists=False)[0]
+        r["dep1"], r["dep2"], r["evap"] = int(counts[0]), int(counts[1]), int(counts[3])
         out.append(r)
         eng.close()
         if len(out) % 32 == 0:
@@ -97,23 +141,37 @@ def report(title, res):
         print(f"  {k:10s} {ma:14.5g} {mb:14.5g}  diff {dd:+.4g}  2se {bound:.4g}  {'ok' if ok else 'OUT'}", flush=True)
 
 
-@pytest.mark.parametrize("epp", [1.0, 4.0])
-@pytest.mark.parametrize("name", ["si-pvd", "cube-zigzag", "CVD-Si-SiH2"])
-def test_sublattice_matches_serial_statistics(tmp_path, name, epp):
-    path, d, box, z0, frozen, t_end, keys = case(tmp_path, name)
+def check_statistics(tmp_path, name, epp):
+    path, tab, box, z0, frozen, t_end, keys = case(tmp_path, name)
+    keys = keys + ("events",)
+    f2 = tab.get("second_factor")
     # serial reference: the oracle, one process per seed; the engine from the same initial lattice
-    serial = serial_runs(path, box, t_end, z0, frozen, [1001 + 2 * s for s in range(SEEDS)])
-    parallel, lat0 = parallel_runs(path, d, box, t_end, z0, frozen, epp, [7000 + s for s in range(SEEDS)])
-    res = observables.compare(parallel, serial, keys + ("events",), nsigma=2.0)
+    serial = serial_runs(path, box, t_end, z0, frozen, [1001 + 2 * s for s in range(SEEDS)], f2)
+    parallel, lat0 = parallel_runs(path, tab, box, t_end, z0, frozen, epp, [7000 + s for s in range(SEEDS)])
+    res = observables.compare(parallel, serial, keys, nsigma=2.0)
     report(f"\n[{name}, {epp} events per tile and pass] t_end={t_end}, {SEEDS} seeds each (parallel vs serial):", res)
     out = [k for k in keys if not res[k][4]]
     if out:  # second stage on fresh seeds
         n2 = 4 * SEEDS
-        serial2 = serial_runs(path, box, t_end, z0, frozen, [50001 + 2 * s for s in range(n2)])
-        parallel2, _ = parallel_runs(path, d, box, t_end, z0, frozen, epp, [17000 + s for s in range(n2)])
+        serial2 = serial_runs(path, box, t_end, z0, frozen, [50001 + 2 * s for s in range(n2)], f2)
+        parallel2, _ = parallel_runs(path, tab, box, t_end, z0, frozen, epp, [17000 + s for s in range(n2)])
         res2 = observables.compare(parallel2, serial2, tuple(out), nsigma=2.0)
         report(f"[{name}, {epp}] second stage, {n2} fresh seeds each, for {', '.join(out)}:", res2)
         for k in out:
             assert res2[k][4], f"{name}: {k} differs by more than 2 standard errors at 32 and at {n2} seeds: {res2[k]}"
     # the comparison is not vacuous: the films changed
     assert np.mean([r["atoms"] for r in serial]) != observables.film(lat0, box, z0)["atoms"]
+    assert np.mean([r["events"] for r in serial]) > 1000
+
+
+@pytest.mark.parametrize("epp", [1.0, 4.0])
+@pytest.mark.parametrize("name", ["si-pvd", "cube-zigzag", "CVD-Si-SiH2"])
+def test_sublattice_matches_serial_statistics(tmp_path, name, epp):
+    check_statistics(tmp_path, name, epp)
+
+
+@pytest.mark.parametrize("name", ["sic2-bench", "sic2-growth"])
+def test_benchmark_sic_model_matches_serial_statistics(tmp_path, name):
+    """The benchmark's SiC model with its second-neighbour term, at the
+    benchmark's tile edge 24 and window of 4 events per tile and pass."""
+    check_statistics(tmp_path, name, 4.0)
