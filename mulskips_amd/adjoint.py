@@ -24,10 +24,20 @@ autograd carries the gradient to the physical parameters:
              nu_Si, nu_C [1/s])
 
 for the two-species model (species 1 Si, 2 C) at a fixed temperature.
-Isolated atoms (rate 1e5*R, Get_Prob.f:305) and the placeholder deposition
-(1e-8, Get_Prob.f:183) have rates that do not depend on theta; they enter
-log L as constants.
+The placeholder deposition (1e-8, Get_Prob.f:183) does not depend on theta.
+An isolated atom's rate is 1e5 R (Get_Prob.f:305), R the total rate when it
+was evaluated, so it does depend on theta through R.  Its terms -- log(1e5 R)
+per isolated-atom event and the integral of 1e5 R over the time the atom
+stays isolated -- enter log L here as the constants the replay measured
+(scalars 1 and 2 of the kernel), not as functions of theta: the gradient
+leaves out sum_e [sel_e / R_e - 1e5 dur_e] dR_e/dtheta over the isolated
+episodes e.  Each such episode ends after about one event (the atom's rate
+is 1e5 times everything else), so the omitted terms average to zero; a
+trajectory with isolated-atom events gets a warning and reports their count
+(TrajectoryStats.isolated_events).
 """
+import warnings
+
 import numpy as np
 import torch
 
@@ -114,10 +124,14 @@ class TrajectoryStats:
         S = np.atleast_2d(S).sum(axis=0)
         A = np.atleast_2d(A).sum(axis=0)
         sc = np.atleast_2d(scalars).sum(axis=0)
+        self.isolated_events = int(S[KEY_ISO])
         if S[KEY_ISO] or sc[1]:
-            self.iso_const = sc[2] - sc[1]  # isolated atoms: constant in theta
+            self.iso_const = sc[2] - sc[1]  # isolated atoms: taken as constant in theta
         else:
             self.iso_const = 0.0
+        if self.isolated_events:
+            warnings.warn(f"{self.isolated_events} isolated-atom events: their log-likelihood terms are taken as "
+                          "constant in theta (their rate 1e5 R depends on theta through R)", RuntimeWarning)
         a = np.zeros(ADJ_NS)
         a[:KEY_N] = A
         a[KEY_ISO] = 0.0

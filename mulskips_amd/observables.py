@@ -38,14 +38,18 @@ def cubic_site(x, y, z):
     return ok & ((m == 0) | (m == 3))
 
 
-def film(latt, box, z0):
+def film(latt, box, z0, sf=None):
     """Observables of the film above height z0 (grid units):
     atoms: atoms with z >= z0;
     height: mean of the column heights (highest atom in each 6x6 column);
     roughness: RMS deviation of the column heights;
     defects: fraction of the film's atoms on non-cubic (stacking-fault) sites;
     coverage: covered points per surface site (atoms_per_layer);
-    species1, species2: the film's atoms of species 1 and 2."""
+    species1, species2: the film's atoms of species 1 and 2;
+    and with the super-lattice step sf (Angstrom) the observables pymulskips
+    reads off the main xyz frame (pymulskips/analyze.py:124-207, :418-483):
+    surface: its surface height in Angstrom; cov_h: its H coverage per
+    surface atom (0 without surface atoms)."""
     lx, ly, lz = box
     occ = occupied(latt)
     z, y, x = np.nonzero(occ)
@@ -55,7 +59,7 @@ def film(latt, box, z0):
     cols = np.full((ly // 6, lx // 6), z0 - 3, np.int64)
     np.maximum.at(cols, (y // 6, x // 6), z)
     defects = float(np.mean(~cubic_site(x, y, z))) if len(z) else 0.0
-    return {
+    out = {
         "atoms": int(len(z)),
         "height": float(cols.mean()),
         "roughness": float(cols.std()),
@@ -64,6 +68,13 @@ def film(latt, box, z0):
         "species1": int(np.count_nonzero(sp == 1)),
         "species2": int(np.count_nonzero(sp == 2)),
     }
+    if sf is not None:
+        from . import analysis
+        names, zf = analysis.main_frame(latt, box, sf)
+        out["surface"] = analysis.surface_height(zf, sf * lz) if len(zf) else 0.0
+        under = sum(1 for n in names if n == "X") - 4 * sum(1 for n in names if n == "O")
+        out["cov_h"] = analysis.coverage(names)["H"][0] if under > 0 else 0.0
+    return out
 
 
 def compare(a, b, keys, nsigma=2.0):

@@ -78,8 +78,9 @@ def oracle_run_to_time(o, t_end):
 def oracle_observables(args):
     """Worker (separate process): run one deck through the oracle to t_end
     with RunType 'R' and seed idum, return the film observables, the events
-    and the depositions by species.  args: (path, box, idum, t_end, z0,
-    frozen[, second_factor])."""
+    and the executed events by type.  args: (path, box, idum, t_end, z0,
+    frozen[, second_factor[, sf]]); with the super-lattice step sf the
+    observables include pymulskips' surface height and H coverage."""
     import os as _os
     import tempfile as _tf
 
@@ -88,6 +89,7 @@ def oracle_observables(args):
     from oracle.oracle import Oracle as _Oracle
     path, box, idum, t_end, z0, frozen = args[:6]
     f2 = args[6] if len(args) > 6 else None
+    sf = args[7] if len(args) > 7 else None
     d = _deck.read_start_dat(path)
     d.run_type, d.idum = "R", int(idum)
     with _tf.TemporaryDirectory() as tmp:
@@ -100,7 +102,7 @@ def oracle_observables(args):
         if f2 is not None:
             o.set_second_factor(f2)
         n = oracle_run_to_time(o, t_end)
-        r = _ob.film(o.lattcoo(), box, z0)
+        r = _ob.film(o.lattcoo(), box, z0, sf=sf)
         r["events"] = n
         r["time"] = o.time()
         ex = o.executed()

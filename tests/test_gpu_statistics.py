@@ -7,11 +7,14 @@ trajectories must agree in distribution with the reference's serial BKL
 are run to the same KMC time from the same initial lattice, and the means of
 the film observables must agree within 2 standard errors of their
 difference:
-* growth (atoms), RMS roughness and mean height: Si PVD on a flat (001) slab
-  with a frozen substrate, E = 1.0 eV per bond at 700 K, PtransZig 0.5;
+* growth (atoms), RMS roughness, mean height and pymulskips' surface height
+  (pymulskips/analyze.py:124-207, whose change over the run is its growth
+  rate): Si PVD on a flat (001) slab with a frozen substrate, E = 1.0 eV per
+  bond at 700 K, PtransZig 0.5;
 * growth and stacking-fault (non-cubic site) fraction: the cube-zigzag
   regression deck (PtransZig 0.8) on a 144^3 grid;
-* growth and coverage: the CVD-Si-SiH2 regression deck (H coverage);
+* growth and coverage: the CVD-Si-SiH2 regression deck (H coverage, also
+  as pymulskips' get_coverage, analyze.py:418-483, and its surface height);
 * the benchmark's SiC model with its 0.05 eV second-neighbour term, at the
   benchmark's tile edge 24 and 4 events per tile and pass: at 1600 K on a
   frozen SiC substrate (the benchmark's own steady state: adatom population
@@ -80,33 +83,34 @@ def sic2_deck(tmp_path, name):
 
 
 def case(tmp_path, name):
-    """(deck path, engine tables, box, z0, frozen, t_end, observables)"""
+    """(deck path, engine tables, box, z0, frozen, t_end, super-lattice step,
+    observables)"""
     if name == "si-pvd":
         path, d = si_pvd_deck(tmp_path)
-        return path, d.tables(), (96, 96, 240), 36, 36, 60.0, ("atoms", "height", "roughness")
+        return path, d.tables(), (96, 96, 240), 36, 36, 60.0, d.sf, ("atoms", "height", "roughness", "surface")
     if name == "sic2-bench":
         path, d, tab = sic2_deck(tmp_path, name)
-        return path, tab, (144, 144, 240), 60, 61, 4.0, ("dep1", "dep2", "evap")
+        return path, tab, (144, 144, 240), 60, 61, 4.0, d.sf, ("dep1", "dep2", "evap")
     if name == "sic2-growth":
         path, d, tab = sic2_deck(tmp_path, name)
-        return path, tab, SMALL["cube-zigzag"][1], 0, 0, 4.0, ("atoms", "species1", "height", "roughness",
-                                                                "defects")
+        return path, tab, SMALL["cube-zigzag"][1], 0, 0, 4.0, d.sf, ("atoms", "species1", "height", "roughness",
+                                                                      "defects")
     base, box, lens = SMALL[name]
     path, d = small_deck(tmp_path, base, lens)
     if name == "cube-zigzag":
-        return path, d.tables(), box, 0, 0, 3.0e5, ("atoms", "defects")
+        return path, d.tables(), box, 0, 0, 3.0e5, d.sf, ("atoms", "defects")
     if name == "CVD-Si-SiH2":
-        return path, d.tables(), box, 0, 0, 5.0, ("atoms", "coverage", "roughness")
+        return path, d.tables(), box, 0, 0, 5.0, d.sf, ("atoms", "coverage", "roughness", "surface", "cov_h")
     raise KeyError(name)
 
 
-def serial_runs(path, box, t_end, z0, frozen, seeds, f2=None):
+def serial_runs(path, box, t_end, z0, frozen, seeds, f2=None, sf=None):
     ctx = mp.get_context("spawn")
     with ctx.Pool(min(8, len(seeds))) as pool:
-        return pool.map(oracle_observables, [(path, box, s, t_end, z0, frozen, f2) for s in seeds])
+        return pool.map(oracle_observables, [(path, box, s, t_end, z0, frozen, f2, sf) for s in seeds])
 
 
-def parallel_runs(path, tab, box, t_end, z0, frozen, epp, seeds):
+def parallel_runs(path, tab, box, t_end, z0, frozen, epp, seeds, sf=None):
     o = Oracle(*box)
     o.load_deck(path)
     if frozen:
@@ -124,7 +128,7 @@ def parallel_runs(path, tab, box, t_end, z0, frozen, epp, seeds):
         assert stop == 0 and t == t_end, (stop, t)
         assert eng.diagnostics()["guard_exits"] == 0
         lat, _, _, _ = eng.store_state()
-        r = observables.film(lat, box, z0)
+        r = observables.film(lat, box, z0, sf=sf)
         r["events"] = n
         counts = eng.counters(lists=False)[0]
         r["dep1"], r["dep2"], r["evap"] = int(counts[0]), int(counts[1]), int(counts[3])
@@ -142,19 +146,19 @@ def report(title, res):
 
 
 def check_statistics(tmp_path, name, epp):
-    path, tab, box, z0, frozen, t_end, keys = case(tmp_path, name)
+    path, tab, box, z0, frozen, t_end, sf, keys = case(tmp_path, name)
     keys = keys + ("events",)
     f2 = tab.get("second_factor")
     # serial reference: the oracle, one process per seed; the engine from the same initial lattice
-    serial = serial_runs(path, box, t_end, z0, frozen, [1001 + 2 * s for s in range(SEEDS)], f2)
-    parallel, lat0 = parallel_runs(path, tab, box, t_end, z0, frozen, epp, [7000 + s for s in range(SEEDS)])
+    serial = serial_runs(path, box, t_end, z0, frozen, [1001 + 2 * s for s in range(SEEDS)], f2, sf)
+    parallel, lat0 = parallel_runs(path, tab, box, t_end, z0, frozen, epp, [7000 + s for s in range(SEEDS)], sf)
     res = observables.compare(parallel, serial, keys, nsigma=2.0)
     report(f"\n[{name}, {epp} events per tile and pass] t_end={t_end}, {SEEDS} seeds each (parallel vs serial):", res)
     out = [k for k in keys if not res[k][4]]
     if out:  # second stage on fresh seeds
         n2 = 4 * SEEDS
-        serial2 = serial_runs(path, box, t_end, z0, frozen, [50001 + 2 * s for s in range(n2)], f2)
-        parallel2, _ = parallel_runs(path, tab, box, t_end, z0, frozen, epp, [17000 + s for s in range(n2)])
+        serial2 = serial_runs(path, box, t_end, z0, frozen, [50001 + 2 * s for s in range(n2)], f2, sf)
+        parallel2, _ = parallel_runs(path, tab, box, t_end, z0, frozen, epp, [17000 + s for s in range(n2)], sf)
         res2 = observables.compare(parallel2, serial2, tuple(out), nsigma=2.0)
         report(f"[{name}, {epp}] second stage, {n2} fresh seeds each, for {', '.join(out)}:", res2)
         for k in out:
