@@ -337,6 +337,21 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
 int mskg_frame(mskg_engine *e, int32_t *main_rec, int64_t cap_main, int32_t *void_rec, int64_t cap_void,
                int32_t *noepi_rec, int64_t cap_noepi, int64_t counts[5]);
 
+/* mskg_frame without a single allocation: no reference counterpart; for
+ * callers that cannot call hipMalloc -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation.  The records go to the
+ * caller's DEVICE buffers (int32 quadruples, capacities in records; a list
+ * longer than its capacity is cut there, records beyond it are not
+ * written), and counts is a DEVICE array of 5 int64 receiving the full
+ * totals (main, voids, non-epitaxial records, the two header counts), so
+ * counts[k] > cap_k tells the caller the list was cut.  The block counts and
+ * offsets of the compaction live in a fixed device array of the library
+ * (16384 blocks of at most 2^30 lattice points each; larger lattices are
+ * rejected), one per GPU, shared by every call on that GPU: calls that
+ * overlap on one GPU race.  Returns when the records are written. */
+int mskg_frame_device(mskg_engine *e, int32_t *main_rec, int64_t cap_main, int32_t *void_rec, int64_t cap_void,
+                      int32_t *noepi_rec, int64_t cap_noepi, int64_t *counts);
+
 /* The SiC defect frames of the xyz output (inpout/WriteMolMolXYZFile.f),
  * gathered on the device as (x, y, z, code) int32 quadruples in the
  * reference's grid order (points with z <= LenZ-1):

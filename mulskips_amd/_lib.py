@@ -21,7 +21,7 @@ EXPORTS = [
     "mskg_build_slab", "mskg_set_event_log", "mskg_get_event_log", "mskg_run_serial", "mskg_adjoint", "mskg_build_alloy_slab",
     "mskg_track_classes", "mskg_copy_state", "mskg_frame", "mskg_set_temperature_field", "mskg_phases",
     "mskg_set_la_variant", "mskg_la_counts", "mskg_set_st_tables", "mskg_species_counts",
-    "mskg_run_diagnostics", "mskg_frame_defects",
+    "mskg_run_diagnostics", "mskg_frame_defects", "mskg_frame_device",
 ]
 MSKG_FIELD_SOCKET, MSKG_FIELD_PLANES = 0, 1  # layouts of the laser-annealing maps
 
@@ -85,6 +85,7 @@ def lib():
     L.mskg_copy_state.argtypes = [P, P]
     L.mskg_frame.argtypes = [P, P, I64, P, I64, P, I64, P]
     L.mskg_frame_defects.argtypes = [P, P, I64, P, I64, P]
+    L.mskg_frame_device.argtypes = [P, P, I64, P, I64, P, I64, P]
     L.mskg_set_temperature_field.argtypes = [P, P, I32, I32]
     L.mskg_phases.argtypes = [P, P, P, I32, I32]
     L.mskg_set_la_variant.argtypes = [P, P, P, P, P, P, P]

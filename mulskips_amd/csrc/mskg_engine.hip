@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1457,7 +1458,8 @@ __global__ void __launch_bounds__(FRAME_THREADS) k_frame_count(Lattice L, int64_
 template <int MODE>
 __global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_t chunk, int sic,
                                                                const unsigned long long *offs, int4 *out0,
-                                                               int4 *out1, int4 *out2) {
+                                                               int4 *out1, int4 *out2, int64_t cap0, int64_t cap1,
+                                                               int64_t cap2) {
   constexpr int NL_ = FrameShape<MODE>::NLIST, NC = FrameShape<MODE>::NCNT;
   __shared__ unsigned int wsum[NL_][FRAME_THREADS / WAVE];
   const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(lo + chunk, L.box.npts);
@@ -1465,6 +1467,7 @@ __global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_
   unsigned long long base[NL_];
   for (int k = 0; k < NL_; k++) base[k] = offs[(int64_t)blockIdx.x * NL_ + k];
   int4 *const outs[3] = {out0, out1, out2};
+  const unsigned long long caps[3] = {(unsigned long long)cap0, (unsigned long long)cap1, (unsigned long long)cap2};
   for (int64_t p0 = lo; p0 < hi; p0 += FRAME_THREADS) {
     const int64_t p = p0 + threadIdx.x;
     bool in[NL_];
@@ -1489,12 +1492,59 @@ __global__ void __launch_bounds__(FRAME_THREADS) k_frame_write(Lattice L, int64_
         if (q < w) before += wsum[k][q];
         total += wsum[k][q];
       }
-      if (in[k]) outs[k][base[k] + before + rank[k]] = make_int4(x, y, z, code[k]);
+      const unsigned long long at = base[k] + before + rank[k];
+      if (in[k] && at < caps[k]) outs[k][at] = make_int4(x, y, z, code[k]);
       base[k] += total;
     }
     __syncthreads();
   }
 }
+
+// The block offsets of a frame from the block counts, on the device: one
+// block scans the counts of every list in order and writes the offsets and
+// the totals (NCNT of them).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_frame_scan(const unsigned long long *counts, int nblk,
+                                                    unsigned long long *offs, int64_t *totals) {
+  constexpr int NL_ = FrameShape<MODE>::NLIST, NC = FrameShape<MODE>::NCNT;
+  __shared__ unsigned long long part[256];
+  for (int k = 0; k < NC; k++) {
+    // each thread sums a contiguous run of blocks, then the runs are scanned
+    const int per = (nblk + 255) / 256;
+    const int lo = min((int)threadIdx.x * per, nblk), hi = min(lo + per, nblk);
+    unsigned long long s = 0;
+    for (int b = lo; b < hi; b++) s += counts[(int64_t)b * NC + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long acc = 0;
+      for (int q = 0; q < 256; q++) {
+        const unsigned long long v = part[q];
+        part[q] = acc;
+        acc += v;
+      }
+      totals[k] = (int64_t)acc;
+    }
+    __syncthreads();
+    if (k < NL_) {
+      unsigned long long acc = part[threadIdx.x];
+      for (int b = lo; b < hi; b++) {
+        offs[(int64_t)b * NL_ + k] = acc;
+        acc += counts[(int64_t)b * NC + k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Scratch of the allocation-free frame entry point (mskg_frame_device): the
+// block counts (5 per block) and offsets (3 per block) of one frame.  Every
+// GPU the library runs on holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress gives for the
+// device (noalloc_scratch), never through the array's name.
+constexpr int NOALLOC_FRAME_BLOCKS = 16384;
+constexpr int64_t NOALLOC_MAX_CHUNK = 1ll << 30;  // lattice points per block
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_FRAME_BLOCKS * 8];
 
 // ---- census: atoms by species and covered points by coverage slot ---------
 // Every deposition adds one atom of its species and every evaporation
@@ -1667,6 +1717,7 @@ struct mskg_engine {
   Tables *d_tab = nullptr;
   RunState *d_st = nullptr;
   int *d_err = nullptr;
+  int device = 0;          // the HIP device the engine lives on
   int tile = 0, ntx = 0, nty = 0, ntz = 0, ntiles = 0, cap = 1024, capg = 512;
   bool lists_dirty = true;
   int pass_grid = 4096;  // workgroups per colour pass (they stride over the window)
@@ -1871,8 +1922,10 @@ static int frame_lists(mskg_engine *e, int sic, int32_t *const hosts[], const in
   ce = hipMemcpyAsync(d_offs, offs.data(), offs.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
                       e->stream);
   if (ce == hipSuccess) {
+    int64_t capv[3] = {0, 0, 0};
+    for (int k = 0; k < NL_; k++) capv[k] = (int64_t)tot[k];
     hipLaunchKernelGGL(k_frame_write<MODE>, dim3(nblk), dim3(FRAME_THREADS), 0, e->stream, L, chunk, sic, d_offs,
-                       d_out[0], d_out[1], d_out[2]);
+                       d_out[0], d_out[1], d_out[2], capv[0], capv[1], capv[2]);
     ce = hipGetLastError();
   }
   for (int k = 0; k < NL_ && ce == hipSuccess; k++)
@@ -1960,6 +2013,25 @@ static int set_wrong_marks(mskg_engine *e, const std::vector<int4> &flips, int s
   return MSKG_OK;
 }
 
+// The device address of `scratch` on each device, taken once per device
+// with that device current (mskg_create).
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned long long *g_scratch[MAX_DEVICES] = {};
+
+static int noalloc_scratch_init(mskg_engine *e) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (e->device < 0 || e->device >= MAX_DEVICES)
+    return fail_(e, MSKG_EDEVICE, "device %d beyond the %d the library tracks", e->device, MAX_DEVICES);
+  if (g_scratch[e->device]) return MSKG_OK;
+  void *ptr = nullptr;
+  const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (rc != hipSuccess || !ptr)
+    return fail_(e, MSKG_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(rc));
+  g_scratch[e->device] = (unsigned long long *)ptr;
+  return MSKG_OK;
+}
+
 extern "C" {
 
 const char *mskg_last_error(const mskg_engine *e) { return e ? e->err.c_str() : "null engine"; }
@@ -2028,6 +2100,8 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
 
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail_(e, MSKG_EDEVICE, "no HIP device available");
+  HIPCHK(e, hipGetDevice(&e->device));
+  if (int rc = noalloc_scratch_init(e)) return rc;
   HIPCHK(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   HIPCHK(e, hipEventCreate(&e->ev0));
   HIPCHK(e, hipEventCreate(&e->ev1));
@@ -3084,6 +3158,31 @@ int mskg_frame(mskg_engine *e, int32_t *main_rec, int64_t cap_main, int32_t *voi
   int32_t *const hosts[3] = {main_rec, void_rec, noepi_rec};
   const int64_t caps[3] = {cap_main, cap_void, cap_noepi};
   return frame_lists<FRAME_MAIN>(e, 0, hosts, caps, counts);
+}
+
+int mskg_frame_device(mskg_engine *e, int32_t *main_rec, int64_t cap_main, int32_t *void_rec, int64_t cap_void,
+                      int32_t *noepi_rec, int64_t cap_noepi, int64_t *counts) {
+  if (!e || !e->loaded || !counts) return fail_(e, MSKG_EINVAL, "frame_device: no state loaded or no counts");
+  if (cap_main < 0 || cap_void < 0 || cap_noepi < 0) return fail_(e, MSKG_EINVAL, "frame_device: negative capacity");
+  const int64_t n = e->box.npts;
+  const int nblk = (int)std::min<int64_t>(std::max<int64_t>(1, (n + 65535) / 65536), NOALLOC_FRAME_BLOCKS);
+  const int64_t chunk = (n + nblk - 1) / nblk;
+  if (chunk > NOALLOC_MAX_CHUNK)
+    return fail_(e, MSKG_EINVAL, "frame_device: %lld lattice points exceed the scratch's %d blocks of %lld points",
+                 (long long)n, NOALLOC_FRAME_BLOCKS, (long long)NOALLOC_MAX_CHUNK);
+  unsigned long long *sc = g_scratch[e->device];
+  if (!sc) return fail_(e, MSKG_EDEVICE, "frame_device: the device's scratch is not set up");
+  unsigned long long *d_counts = sc, *d_offs = sc + (size_t)nblk * FrameShape<FRAME_MAIN>::NCNT;
+  Lattice L = make_lattice(e);
+  hipLaunchKernelGGL(k_frame_count<FRAME_MAIN>, dim3(nblk), dim3(FRAME_THREADS), 0, e->stream, L, chunk, 0,
+                     d_counts);
+  hipLaunchKernelGGL(k_frame_scan<FRAME_MAIN>, dim3(1), dim3(256), 0, e->stream, d_counts, nblk, d_offs, counts);
+  hipLaunchKernelGGL(k_frame_write<FRAME_MAIN>, dim3(nblk), dim3(FRAME_THREADS), 0, e->stream, L, chunk, 0, d_offs,
+                     (int4 *)main_rec, (int4 *)void_rec, (int4 *)noepi_rec, cap_main, cap_void, cap_noepi);
+  hipError_t ce = hipGetLastError();
+  if (ce == hipSuccess) ce = hipStreamSynchronize(e->stream);
+  if (ce != hipSuccess) return fail_(e, MSKG_EDEVICE, "frame_device: %s", hipGetErrorString(ce));
+  return MSKG_OK;
 }
 
 int mskg_frame_defects(mskg_engine *e, int32_t *vac_rec, int64_t cap_vac, int32_t *wrong_rec, int64_t cap_wrong,

@@ -244,6 +244,14 @@ module mskg_bindings
       integer(c_int64_t), intent(out) :: counts(5)
     end function
 
+    ! mskg_frame without an allocation: device buffers and a device counts(5)
+    integer(c_int) function mskg_frame_device(eng, main_rec, cap_main, void_rec, cap_void, noepi_rec, cap_noepi, &
+                                              counts) bind(c, name="mskg_frame_device")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: eng, main_rec, void_rec, noepi_rec, counts
+      integer(c_int64_t), value :: cap_main, cap_void, cap_noepi
+    end function
+
     ! the SiC defect frames (_v vacancy classes, _w wrong occupations); counts(2)
     integer(c_int) function mskg_frame_defects(eng, vac_rec, cap_vac, wrong_rec, cap_wrong, counts) &
         bind(c, name="mskg_frame_defects")
