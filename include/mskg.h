@@ -203,10 +203,27 @@ int mskg_refresh_rates(mskg_engine *e);
 int mskg_run(mskg_engine *e, int64_t max_events, double max_time,
              int64_t *events_done, double *kmc_time, int32_t *stop_reason);
 
+/* How mskg_run orders the eight colour passes of a sweep (no reference
+ * counterpart; the reference's loop is serial).  MSKG_SCHEDULE_COLOURS (the
+ * default): one launch per colour, each waiting for the whole previous
+ * colour.  MSKG_SCHEDULE_TILES: one launch per sweep in which a tile's pass
+ * starts as soon as the neighbour tiles of lower colours have finished
+ * theirs (a queue of tiles in colour order, per-tile completion flags with
+ * agent-scope release / acquire).  Both give the same lattice, clock and
+ * counters bit for bit: a tile sees the same state, and draws the same
+ * random numbers, in either order.  On MI355X the tile schedule removes the
+ * idle time at the colour barriers but each acquire invalidates the L2 of
+ * the waiting wave's XCD, which the other waves of that XCD then refill:
+ * the barrier schedule is the faster one there. */
+#define MSKG_SCHEDULE_TILES 1
+#define MSKG_SCHEDULE_COLOURS 0
+int mskg_set_schedule(mskg_engine *e, int32_t schedule);
+
 /* Record every event mskg_run executes (capacity events; 0 switches the log
  * off).  The order is a valid serial order of the parallel run: events of
- * one tile are in execution order, and events of different tiles of the same
- * colour pass touch disjoint sites, so they commute.  mskg_get_event_log
+ * one tile are in execution order, a tile's events follow those of its
+ * neighbour tiles that ran before it in the sweep, and events of tiles that
+ * are not neighbours touch disjoint sites, so they commute.  mskg_get_event_log
  * copies up to cap events and reports how many were recorded. */
 int mskg_set_event_log(mskg_engine *e, int64_t capacity);
 int mskg_get_event_log(mskg_engine *e, mskg_event *out, int64_t cap, int64_t *n);

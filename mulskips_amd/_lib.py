@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libmskg.so")
 
 MSKG_OK, MSKG_EINVAL, MSKG_ESTATE, MSKG_ECAPACITY, MSKG_EDEVICE, MSKG_ENOMEM = range(6)
 MSKG_STOP_LIMIT, MSKG_STOP_FEW_MC, MSKG_STOP_NO_EVENTS = 0, 1, 3  # stop_reason of mskg_run
+MSKG_SCHEDULE_COLOURS, MSKG_SCHEDULE_TILES = 0, 1  # mskg_set_schedule
 
 EXPORTS = [
     "mskg_create", "mskg_destroy", "mskg_last_error", "mskg_load_state", "mskg_set_temperature",
@@ -21,7 +22,7 @@ EXPORTS = [
     "mskg_build_slab", "mskg_set_event_log", "mskg_get_event_log", "mskg_run_serial", "mskg_adjoint", "mskg_build_alloy_slab",
     "mskg_track_classes", "mskg_copy_state", "mskg_frame", "mskg_set_temperature_field", "mskg_phases",
     "mskg_set_la_variant", "mskg_la_counts", "mskg_set_st_tables", "mskg_species_counts",
-    "mskg_run_diagnostics", "mskg_frame_defects", "mskg_frame_device",
+    "mskg_run_diagnostics", "mskg_frame_defects", "mskg_frame_device", "mskg_set_schedule",
 ]
 MSKG_FIELD_SOCKET, MSKG_FIELD_PLANES = 0, 1  # layouts of the laser-annealing maps
 
@@ -76,6 +77,7 @@ def lib():
     L.mskg_counters.argtypes = [P, P, P, P, P, P]
     L.mskg_last_timing.argtypes = [P, P, P, P, P]
     L.mskg_run_diagnostics.argtypes = [P, P, P, P, P]
+    L.mskg_set_schedule.argtypes = [P, I32]
     L.mskg_build_slab.argtypes = [P, I32, I32]
     L.mskg_set_event_log.argtypes = [P, I64]
     L.mskg_get_event_log.argtypes = [P, P, I64, P]

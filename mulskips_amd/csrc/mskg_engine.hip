@@ -13,6 +13,9 @@
 //                  one tile for the window tau, selecting events with a
 //                  wavefront prefix scan over the tile's FP32 rates held in
 //                  LDS, FP64 partial sums and Philox-4x32-10 draws.
+//   k_sweep        the eight colours of a sweep in one launch, each tile
+//                  starting when its lower-colour neighbours are done (same
+//                  result as eight k_pass launches, without their barriers).
 //   k_rsum         fixed-order FP64 sum of the tile rate totals;
 //   k_sweep_end    -> next tau and the clock.
 //   k_serial       serial BKL over a batch of lattices (one wave each).
@@ -100,6 +103,11 @@ struct PassParams {
   int cap;
   uint32_t key0, key1;
   int colour;
+  // dependency-scheduled sweep: per-tile completion stamps, and sched[0]
+  // the sweep's stamp, [1] queue head, [2] abort flag, [3] / [4] the first /
+  // last tile layer holding candidates when the sweep starts
+  unsigned int *done;
+  unsigned int *sched;
 };
 
 __device__ inline double wave_sum(double v) {
@@ -418,44 +426,29 @@ __device__ inline void sort_by_offset(int32_t *lk, float *lr, uint8_t *lc, int l
   }
 }
 
-// Three waves per SIMD (168 VGPRs): 3072 tiles in flight on 256 CUs, against
-// the ~4096 tiles of a colour pass at the 512^3 workload.
-template <bool LA, bool COV>
-__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) k_pass(PassParams P) {
-  extern __shared__ unsigned char smem[];
-  const int lane = threadIdx.x;
-  Lattice L = P.L;
-  const int cap = P.cap;
-  int32_t *lk = (int32_t *)smem;                  // tile-local point offsets
-  float *lr = (float *)(lk + cap);                // rates
-  uint8_t *lc = (uint8_t *)(lr + cap);            // classes
-  __shared__ int s_stop, s_nt;
-  __shared__ P3 s_tp[MAX_TOUCHED];
-  __shared__ int s_tk[MAX_TOUCHED];
-  __shared__ float s_tr[MAX_TOUCHED];
-  __shared__ uint8_t s_tc[MAX_TOUCHED];
-  __shared__ int s_found[MAX_TOUCHED];
-  __shared__ unsigned long long s_cnt[8];  // events by type (lane 0)
-  __shared__ long long s_dcc[3];           // crystal count changes (laser-annealing build's form)
-  __shared__ unsigned long long s_des[3];  // desorptions by coverage slot
-  __shared__ EvScratch sc;
+// The LDS of one tile's pass besides its candidate list (which is dynamic).
+struct PassLds {
+  int stop, nt;
+  P3 tp[MAX_TOUCHED];
+  int tk[MAX_TOUCHED];
+  float tr[MAX_TOUCHED];
+  uint8_t tc[MAX_TOUCHED];
+  int found[MAX_TOUCHED];
+  unsigned long long cnt[8];  // events by type (lane 0)
+  long long dcc[3];           // crystal count changes (laser-annealing build's form)
+  unsigned long long des[3];  // desorptions by coverage slot
+  EvScratch sc;
+};
 
-  RunState *st = P.st;
-  const double tau = st->tau;
-  if (!(tau > 0.0) || st->err || st->stop) return;
-  // tiles of this colour inside the window of active tile layers
-  const int c0 = P.colour & 1, c1 = (P.colour >> 1) & 1, c2 = (P.colour >> 2) & 1;
-  const int zlo = max(L.win[4 * P.colour], 0), zhi = min(L.win[4 * P.colour + 1], P.ntz - 1);
-  if (zlo > zhi) return;  // no tile of this colour holds candidates
-  const int zfirst = zlo + ((c2 - zlo) & 1);
-  const int nlayers = zfirst <= zhi ? (zhi - zfirst) / 2 + 1 : 0;
-  const int hx = L.ntx / 2, per_layer = hx * (L.nty / 2);
-  const int nwin = nlayers * per_layer;
-  for (int wi = blockIdx.x; wi < nwin; wi += gridDim.x) {
-  const int layer = wi / per_layer, rr = wi - layer * per_layer;
-  const int t = ((zfirst + 2 * layer) * L.nty + 2 * (rr / hx) + c1) * L.ntx + 2 * (rr % hx) + c0;
+// One tile's share of a colour pass: its candidate list into LDS, rejection-
+// free KMC inside the tile for the window tau, the list back.  Returns the
+// error that ends the wave's work (0: none).
+template <bool LA, bool COV>
+__device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, const double tau, const int t,
+                                const int colour, PassLds &S, int32_t *lk, float *lr, uint8_t *lc, const int lane) {
+  const int cap = P.cap;
   const int n_in = L.tl_n[t];
-  if (n_in == 0) continue;
+  if (n_in == 0) return 0;
   L.cur_tile = t;
   TileGeom g = tile_geom(t, L.tile, L.ntx, L.nty, L.box.lz);
 
@@ -493,7 +486,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
       P3 p = tile_point(L.box, g, L.tile, 0);
       st->err_site[0] = p.x; st->err_site[1] = p.y; st->err_site[2] = p.z;
     }
-    return;
+    return ERR_CAPACITY;
   }
   __syncthreads();
   sort_by_offset(lk, lr, lc, nh, n, lane);
@@ -502,13 +495,13 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   const double rtot = st->rtot;
   double tloc = 0.0;
   unsigned long long nev = 0;
-  if (lane < 8) s_cnt[lane] = 0;
-  if (lane < 3) s_dcc[lane] = 0;
-  if (COV && lane < 3) s_des[lane] = 0;
+  if (lane < 8) S.cnt[lane] = 0;
+  if (lane < 3) S.dcc[lane] = 0;
+  if (COV && lane < 3) S.des[lane] = 0;
   U4 ctr;
   ctr.x = (uint32_t)t;
   ctr.y = (uint32_t)st->sweeps;
-  ctr.z = (uint32_t)(st->sweeps >> 32) ^ ((uint32_t)P.colour << 29);
+  ctr.z = (uint32_t)(st->sweeps >> 32) ^ ((uint32_t)colour << 29);
   ctr.w = 0;
   int err = 0;
   // every wave ends its pass: the window tau bounds the expected number of
@@ -566,7 +559,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
       const int coor0 = P.log ? s_coor(L.site[s.i]) : 0;
       __syncthreads();  // every lane has read the site before it changes
       EvResult res;
-      execute<LA, COV>(L, rtot, s, itrans, ch, res, sc, lane);
+      execute<LA, COV>(L, rtot, s, itrans, ch, res, S.sc, lane);
       __syncthreads();  // neighbour updates by lanes 1..3 are visible to lane 0
       if (lane == 0) {
         err = res.err;
@@ -585,36 +578,36 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
           }
         }
         int slot = itrans <= 3 ? itrans - 1 : (itrans == 4 ? 3 : (itrans == 5 ? 4 : itrans - 1));
-        if (slot >= 0 && slot < 8) s_cnt[slot]++;
-        if (LA && L.cl && res.species && itrans <= 4) s_dcc[res.species - 1] += itrans <= 3 ? 1 : -1;
-        if (COV && itrans == 5 && res.species) s_des[res.species - 1]++;
-        for (int q = 0; q < sc.ntouched; q++) s_tp[q] = sc.touched[q];
-        s_nt = sc.ntouched;
-        s_stop = err;
+        if (slot >= 0 && slot < 8) S.cnt[slot]++;
+        if (LA && L.cl && res.species && itrans <= 4) S.dcc[res.species - 1] += itrans <= 3 ? 1 : -1;
+        if (COV && itrans == 5 && res.species) S.des[res.species - 1]++;
+        for (int q = 0; q < S.sc.ntouched; q++) S.tp[q] = S.sc.touched[q];
+        S.nt = S.sc.ntouched;
+        S.stop = err;
       }
     }
     nev++;
     __syncthreads();
-    if (s_stop) {
-      err = s_stop;
+    if (S.stop) {
+      err = S.stop;
       break;
     }
     // the touched points' new rates, one lane each, as execute() left them
     // on chip; a point that got no new rate keeps its list entry as it is
-    const int nt = s_nt;
+    const int nt = S.nt;
     if (lane < nt) {
-      const P3 tp = s_tp[lane];
-      const int ts = sc.tset[lane];
+      const P3 tp = S.tp[lane];
+      const int ts = S.sc.tset[lane];
       const int kl = ts ? tile_local(g, L.tile, tp) : -1;
-      s_tk[lane] = kl;
-      s_found[lane] = -1;
+      S.tk[lane] = kl;
+      S.found[lane] = -1;
       if (kl >= 0) {
         if (ts == 1) {
-          s_tr[lane] = sc.trate[lane];
-          s_tc[lane] = sc.tcls[lane];
+          S.tr[lane] = S.sc.trate[lane];
+          S.tc[lane] = S.sc.tcls[lane];
         } else {
-          s_tr[lane] = L.rate[tp.i];
-          s_tc[lane] = L.cls[tp.i];
+          S.tr[lane] = L.rate[tp.i];
+          S.tc[lane] = L.cls[tp.i];
         }
       }
     }
@@ -625,7 +618,7 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
     for (int j = lane; j < n; j += WAVE) {
       const int kj = lk[j];
       for (int q = 0; q < nt; q++)
-        if (s_tk[q] == kj) s_found[q] = j;
+        if (S.tk[q] == kj) S.found[q] = j;
     }
     __syncthreads();
     // ... updates in place, and appends the new MC particles (first
@@ -633,16 +626,16 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
     bool app = false;
     int kl = -1;
     if (lane < nt) {
-      kl = s_tk[lane];
+      kl = S.tk[lane];
       if (kl >= 0) {
-        const int f = s_found[lane];
+        const int f = S.found[lane];
         if (f >= 0) {
-          lr[f] = s_tr[lane];
-          lc[f] = s_tc[lane];
-        } else if (s_tc[lane]) {
+          lr[f] = S.tr[lane];
+          lc[f] = S.tc[lane];
+        } else if (S.tc[lane]) {
           app = true;
           for (int q = 0; q < lane; q++)
-            if (s_tk[q] == kl) app = false;
+            if (S.tk[q] == kl) app = false;
         }
       }
     }
@@ -651,8 +644,8 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
       const int slot = n + __popcll(am & ((1ull << lane) - 1ull));
       if (slot < cap) {
         lk[slot] = kl;
-        lr[slot] = s_tr[lane];
-        lc[slot] = s_tc[lane];
+        lr[slot] = S.tr[lane];
+        lc[slot] = S.tc[lane];
       }
     }
     if (n + __popcll(am) > cap) {  // the list keeps its first cap entries
@@ -681,16 +674,16 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
     L.tl_n[t] = nout;
     L.tl_own[t] = nout;
     L.tile_r[t] = R;
-    slot_add(st, t, R, (unsigned long long)nout, nev, s_cnt);
+    slot_add(st, t, R, (unsigned long long)nout, nev, S.cnt);
     if (LA && L.cl)
       for (int k = 0; k < 3; k++)
-        if (s_dcc[k]) atomicAdd(&st->dcc[k], (unsigned long long)s_dcc[k]);
+        if (S.dcc[k]) atomicAdd(&st->dcc[k], (unsigned long long)S.dcc[k]);
     if (COV)
       for (int k = 0; k < 3; k++)
-        if (s_des[k]) atomicAdd(&st->desorbed[k], s_des[k]);
+        if (S.des[k]) atomicAdd(&st->desorbed[k], S.des[k]);
     if (nout) {
-      if (g.tz < L.win[4 * P.colour + 2]) atomic_min_i32(&L.win[4 * P.colour + 2], g.tz);
-      if (g.tz > L.win[4 * P.colour + 3]) atomic_max_i32(&L.win[4 * P.colour + 3], g.tz);
+      if (g.tz < L.win[4 * colour + 2]) atomic_min_i32(&L.win[4 * colour + 2], g.tz);
+      if (g.tz > L.win[4 * colour + 3]) atomic_max_i32(&L.win[4 * colour + 3], g.tz);
     }
     if (err) {
       st->err = err;
@@ -698,9 +691,178 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
       st->err_site[0] = p.x; st->err_site[1] = p.y; st->err_site[2] = p.z;
     }
   }
-  if (err) return;
+  err = __builtin_amdgcn_readfirstlane(err);  // the same in every lane
+  if (err) return err;
   __syncthreads();
-  }  // window loop
+  return 0;
+}
+
+// One colour pass: the tiles of `colour` in the window of tile layers that
+// hold candidates, each workgroup (one wave) striding over them.  Three
+// waves per SIMD (168 VGPRs): 3072 tiles in flight on 256 CUs, against the
+// ~4096 tiles of a colour pass at the 512^3 workload.
+template <bool LA, bool COV>
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) k_pass(PassParams P) {
+  extern __shared__ unsigned char smem[];
+  __shared__ PassLds S;
+  const int lane = threadIdx.x;
+  Lattice L = P.L;
+  int32_t *lk = (int32_t *)smem;        // tile-local point offsets
+  float *lr = (float *)(lk + P.cap);    // rates
+  uint8_t *lc = (uint8_t *)(lr + P.cap);  // classes
+  RunState *st = P.st;
+  const double tau = st->tau;
+  if (!(tau > 0.0) || st->err || st->stop) return;
+  const int c0 = P.colour & 1, c1 = (P.colour >> 1) & 1, c2 = (P.colour >> 2) & 1;
+  const int zlo = max(L.win[4 * P.colour], 0), zhi = min(L.win[4 * P.colour + 1], P.ntz - 1);
+  if (zlo > zhi) return;  // no tile of this colour holds candidates
+  const int zfirst = zlo + ((c2 - zlo) & 1);
+  const int nlayers = zfirst <= zhi ? (zhi - zfirst) / 2 + 1 : 0;
+  const int hx = L.ntx / 2, per_layer = hx * (L.nty / 2);
+  const int nwin = nlayers * per_layer;
+  for (int wi = blockIdx.x; wi < nwin; wi += gridDim.x) {
+    const int layer = wi / per_layer, rr = wi - layer * per_layer;
+    const int t = ((zfirst + 2 * layer) * L.nty + 2 * (rr / hx) + c1) * L.ntx + 2 * (rr % hx) + c0;
+    if (pass_tile<LA, COV>(P, L, st, tau, t, P.colour, S, lk, lr, lc, lane)) return;
+  }
+}
+
+// ---- a sweep scheduled by tile dependencies ------------------------------
+// The eight colour passes of a sweep in one launch without a barrier
+// between colours: a tile's pass starts as soon as its neighbour tiles of
+// the lower colours have finished theirs in this sweep (tiles of the higher
+// colours finished theirs in the previous launch).  A tile's events read
+// and write only its own tile and the 24-unit shell of its 26 neighbour
+// tiles, so every tile sees exactly the state it sees with a barrier after
+// each colour, and the sweep's result -- lattice, lists, totals, random
+// streams -- is the same bit for bit.  Tiles that do not neighbour each
+// other (same colour, or farther apart) run concurrently as before.
+//
+// Work distribution: a queue of the sweep's tiles in colour order; every
+// wave takes the next entry with one atomic, waits for the entry's lower-
+// colour neighbours, runs the tile and publishes its completion.  A wave
+// only waits on entries of lower colours, which were taken earlier by waves
+// that are running, so the queue always drains.  The completion flag of a
+// tile is the sweep's stamp (sched[0], advanced by k_sweep_end), stored
+// with agent-scope release after the tile's writes; a waiting wave loads it
+// at agent scope and takes an agent-scope acquire before it reads the
+// neighbours' records, so the tiles' records travel between XCDs.
+
+// The tile layers of colours with layer parity `odd` that a sweep's queue
+// holds, for candidates in layers [zlo, zhi] when the sweep starts: the
+// even colours (0-3) run first, so their layers outside the window stay
+// empty until they have run; the odd colours (4-7) run after the even
+// layers next to them, whose events can enlist points one layer out.
+__device__ inline void queue_layers(int zlo, int zhi, int odd, int ntz, int &first, int &count) {
+  const int lo = max(odd ? zlo - 1 : zlo, 0), hi = min(odd ? zhi + 1 : zhi, ntz - 1);
+  first = lo + ((odd - lo) & 1);
+  count = first <= hi ? (hi - first) / 2 + 1 : 0;
+}
+
+// A wave waits at most this long (wall-clock ticks of 10 ns) for a
+// neighbour tile; the run then stops with ERR_SCHED instead of hanging.
+constexpr long long SCHED_WAIT_TICKS = 200000000ll;
+
+template <bool LA, bool COV>
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3))) k_sweep(PassParams P) {
+  extern __shared__ unsigned char smem[];
+  __shared__ PassLds S;
+  const int lane = threadIdx.x;
+  Lattice L = P.L;
+  int32_t *lk = (int32_t *)smem;
+  float *lr = (float *)(lk + P.cap);
+  uint8_t *lc = (uint8_t *)(lr + P.cap);
+  RunState *st = P.st;
+  const double tau = st->tau;
+  if (!(tau > 0.0) || st->err || st->stop) return;
+  // the window as the sweep starts (L.win's first words widen during it)
+  const int zlo = (int)P.sched[3], zhi = (int)P.sched[4];
+  if (zlo > zhi) return;
+  int first[2], count[2];
+  queue_layers(zlo, zhi, 0, P.ntz, first[0], count[0]);
+  queue_layers(zlo, zhi, 1, P.ntz, first[1], count[1]);
+  const int hx = L.ntx / 2, per_layer = hx * (L.nty / 2);
+  const int per_even = count[0] * per_layer, per_odd = count[1] * per_layer;
+  const int total = 4 * (per_even + per_odd);
+  const unsigned int stamp = P.sched[0];
+  unsigned int *const head = P.sched + 1;
+  // neighbour k of 26 (lanes 0..25): offsets -1..1 along each axis
+  const int kk = lane + (lane >= 13 ? 1 : 0);
+  const int ndx = kk % 3 - 1, ndy = (kk / 3) % 3 - 1, ndz = kk / 9 - 1;
+  const int nflip = (ndx & 1) | ((ndy & 1) << 1) | ((ndz & 1) << 2);
+  // Every decision of this loop is wave-uniform by construction (scalar
+  // values from readfirstlane / ballot) and no memory access sits under a
+  // lane-dependent branch: the compiler may otherwise restructure the loop
+  // so that lane 0, which would own such accesses, runs a different
+  // iteration than the other lanes.
+  for (;;) {
+    // the next entry: every lane adds 1 (one atomic of 64 for the wave), so
+    // the head counts entries in units of 64
+    const unsigned int took = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int q = __builtin_amdgcn_readfirstlane((int)(took >> 6));
+    if (q >= total) break;
+    // entry -> colour, tile
+    int colour, wi;
+    if (q < 4 * per_even) {
+      colour = q / per_even;
+      wi = q - colour * per_even;
+    } else {
+      const int r = q - 4 * per_even;
+      colour = 4 + r / per_odd;
+      wi = r - (colour - 4) * per_odd;
+    }
+    const int odd = colour >> 2;
+    const int layer = wi / per_layer, rr = wi - layer * per_layer;
+    const int tz = first[odd] + 2 * layer, ty = 2 * (rr / hx) + ((colour >> 1) & 1), tx = 2 * (rr % hx) + (colour & 1);
+    const int t = (tz * L.nty + ty) * L.ntx + tx;
+    // the neighbours of lower colours that this sweep's queue holds
+    bool dep = false;
+    int n = 0;
+    if (lane < 26) {
+      const int nc = colour ^ nflip, nz = tz + ndz;
+      if (nc < colour && nz >= 0 && nz < P.ntz) {
+        const int nodd = nc >> 2;
+        const int rel = nz - first[nodd];
+        if (rel >= 0 && rel < 2 * count[nodd]) {
+          int nx = tx + ndx, ny = ty + ndy;
+          nx += nx < 0 ? L.ntx : (nx >= L.ntx ? -L.ntx : 0);
+          ny += ny < 0 ? L.nty : (ny >= L.nty ? -L.nty : 0);
+          n = (nz * L.nty + ny) * L.ntx + nx;
+          dep = true;
+        }
+      }
+    }
+    const unsigned int *const flag = P.done + (dep ? n : t);  // a lane without a neighbour reads its own tile's
+    bool abort = false;
+    if (__ballot(dep) != 0ull) {
+      const long long t0 = wall_clock64();
+      for (;;) {
+        const unsigned int f = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__ballot(dep && f != stamp) == 0ull) break;
+        const unsigned int ab = __hip_atomic_load(&P.sched[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool late = wall_clock64() - t0 > SCHED_WAIT_TICKS;
+        if (__builtin_amdgcn_readfirstlane((int)ab) != 0 || late) {
+          if (late) {  // every lane stores the same values
+            __hip_atomic_store(&P.sched[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atomicCAS(&st->err, 0, ERR_SCHED);
+          }
+          abort = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    const int err_now = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&st->err, __ATOMIC_RELAXED,
+                                                                         __HIP_MEMORY_SCOPE_AGENT));
+    const int n_in = __builtin_amdgcn_readfirstlane(L.tl_n[t]);
+    if (!abort && !err_now && n_in > 0) {
+      pass_tile<LA, COV>(P, L, st, tau, t, colour, S, lk, lr, lc, lane);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the tile's writes before its flag
+    }
+    // every lane stores the same stamp
+    __hip_atomic_store(&P.done[t], stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // The fixed-order sum of the tile totals: block b sums its chunk of tiles
@@ -724,7 +886,8 @@ __global__ void __launch_bounds__(256) k_rsum(const double *tile_r, int ntiles, 
 // (operator splitting of the master equation over the colours), the next
 // window is events_per_pass / (largest tile rate), and the reference's
 // early stop applies when at most 20 MC particles remain (main.f:307-310).
-__global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, const int *list_err) {
+__global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, const int *list_err,
+                                                     unsigned int *sched) {
   __shared__ double srsum[NSLOT];
   __shared__ unsigned long long srmax[NSLOT], sn[NSLOT], stl[NSLOT], sev[NSLOT], scnt[8][NSLOT];
   const int i = threadIdx.x;
@@ -759,12 +922,21 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
       st->cl[k] = st->cinit[k] - st->cc[k];
     }
   // the passes of the next sweep visit the layers that held candidates
+  int zlo = 0x7fffffff, zhi = -1;
   for (int c = 0; c < 8; c++) {
     win[4 * c] = win[4 * c + 2];
     win[4 * c + 1] = win[4 * c + 3];
     win[4 * c + 2] = 0x7fffffff;
     win[4 * c + 3] = -1;
+    zlo = min(zlo, win[4 * c]);
+    zhi = max(zhi, win[4 * c + 1]);
   }
+  // the dependency-scheduled sweep: a new stamp, an empty queue, the window
+  sched[0]++;
+  sched[1] = 0u;
+  sched[2] = 0u;
+  sched[3] = (unsigned int)zlo;
+  sched[4] = (unsigned int)zhi;
   if (*list_err && !st->err) st->err = *list_err;
   if (st->tau > 0.0 && !st->err && !st->stop) {
     // a window shortened to end at the time limit lands on it exactly
@@ -1713,6 +1885,10 @@ struct mskg_engine {
   int64_t census0[6] = {0, 0, 0, 0, 0, 0};
   int *d_list_err = nullptr;
   int *d_win = nullptr;
+  // per-tile completion stamps of the dependency-scheduled sweep, then its
+  // stamp / queue / window words (PassParams::done, ::sched)
+  unsigned int *d_done = nullptr;
+  int schedule = MSKG_SCHEDULE_COLOURS;
   Geometry *d_geo = nullptr;
   Tables *d_tab = nullptr;
   RunState *d_st = nullptr;
@@ -1830,6 +2006,7 @@ static const char *err_name(int c) {
     case ERR_SUMS: return "serial selection found no event inside a positive rate sum";
     case ERR_LIQUID: return "a species' CountLiquid is negative: its solidification rate would be negative";
     case ERR_ST_RANGE: return "a site's temperature lies outside the InitSt 'ST' reference temperatures";
+    case ERR_SCHED: return "a tile waited more than 2 s for its neighbour tiles in a dependency-scheduled sweep";
     default: return "unknown";
   }
 }
@@ -2116,6 +2293,7 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   HIPCHK(e, hipMalloc(&e->d_tile_r, e->ntiles * sizeof(double)));
   HIPCHK(e, hipMalloc(&e->d_list_err, sizeof(int)));
   HIPCHK(e, hipMalloc(&e->d_win, 32 * sizeof(int)));
+  HIPCHK(e, hipMalloc(&e->d_done, ((size_t)e->ntiles + 8) * sizeof(unsigned int)));
   HIPCHK(e, hipMalloc(&e->d_geo, sizeof(Geometry)));
   HIPCHK(e, hipMalloc(&e->d_tab, sizeof(Tables)));
   HIPCHK(e, hipMalloc(&e->d_st, sizeof(RunState)));
@@ -2130,6 +2308,8 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   HIPCHK(e, hipMemsetAsync(e->d_tl_own, 0, e->ntiles * sizeof(int32_t), e->stream));
   HIPCHK(e, hipMemsetAsync(e->d_tile_r, 0, e->ntiles * sizeof(double), e->stream));
   HIPCHK(e, hipMemsetAsync(e->d_list_err, 0, sizeof(int), e->stream));
+  // completion stamps: 0 is no sweep's (the first k_sweep_end makes it 1)
+  HIPCHK(e, hipMemsetAsync(e->d_done, 0, ((size_t)e->ntiles + 8) * sizeof(unsigned int), e->stream));
   RunState st;
   memset(&st, 0, sizeof st);
   HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
@@ -2145,6 +2325,7 @@ void mskg_destroy(mskg_engine *e) {
   if (e->graph) hipGraphDestroy(e->graph);
   hipFree(e->d_inlist); hipFree(e->d_tl_idx); hipFree(e->d_tl_n); hipFree(e->d_tl_own); hipFree(e->d_tile_r); hipFree(e->d_list_err);
   hipFree(e->d_field[0]); hipFree(e->d_field[1]); hipFree(e->d_offwall); hipFree(e->d_st_tables); hipFree(e->d_win);
+  hipFree(e->d_done);
   hipFree(e->d_geo); hipFree(e->d_tab); hipFree(e->d_st); hipFree(e->d_err); hipFree(e->d_log);
   hipFree(e->d_tsum); hipFree(e->d_bsum); hipFree(e->d_key); hipFree(e->d_key_seen); hipFree(e->d_rate_seen);
   if (e->ev0) hipEventDestroy(e->ev0);
@@ -2522,29 +2703,48 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
   hipLaunchKernelGGL(k_tile_rates, dim3(tblocks), dim3(256), 0, e->stream, L, e->ntiles, e->d_st);
   HIPCHK(e, hipGetLastError());
   hipLaunchKernelGGL(k_rsum, dim3(NPART), dim3(256), 0, e->stream, e->d_tile_r, e->ntiles, e->d_st);
-  hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err);
+  hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err,
+                       e->d_done + e->ntiles);
   HIPCHK(e, hipGetLastError());
   const size_t shm = (size_t)e->cap * (4 + 4 + 1) + 16;
-  // one sweep = eight colour passes + the window/clock update, captured once
-  // as a graph and replayed
+  // one sweep = eight colour passes (or one dependency-scheduled launch) +
+  // the window/clock update, captured once as a graph and replayed
   if (!e->graph_exec) {
-    const int grid = std::max(1, std::min(e->ntiles / 8 + 1, e->pass_grid));
+    PassParams P;
+    P.L = L;
+    P.st = e->d_st;
+    P.log = e->d_log;
+    P.log_cap = (unsigned long long)e->log_cap;
+    P.ntz = e->ntz;
+    P.cap = e->cap;
+    P.key0 = (uint32_t)e->cfg.seed;
+    P.key1 = (uint32_t)(e->cfg.seed >> 32);
+    P.colour = 0;
+    P.done = e->d_done;
+    P.sched = e->d_done + e->ntiles;
+    const bool la = e->tab.la_mode || e->tab.st_n;
+    int grid = std::max(1, std::min(e->ntiles / 8 + 1, e->pass_grid));
+    if (e->schedule == MSKG_SCHEDULE_TILES) {
+      // every wave the device holds at once: the waves take tiles from the
+      // sweep's queue until it is empty
+      int per_cu = 0, cus = 0;
+      HIPCHK(e, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, reinterpret_cast<const void *>(MSKG_VARIANT(k_sweep, la, e->cov_model)), WAVE, shm));
+      HIPCHK(e, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
+      grid = std::max(1, per_cu * cus);
+    }
     HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    for (int c = 0; c < 8; c++) {
-      PassParams P;
-      P.L = L;
-      P.st = e->d_st;
-      P.log = e->d_log;
-      P.log_cap = (unsigned long long)e->log_cap;
-      P.ntz = e->ntz;
-      P.cap = e->cap;
-      P.key0 = (uint32_t)e->cfg.seed;
-      P.key1 = (uint32_t)(e->cfg.seed >> 32);
-      P.colour = c;
-      hipLaunchKernelGGL(MSKG_VARIANT(k_pass, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(grid), dim3(WAVE), shm, e->stream, P);
+    if (e->schedule == MSKG_SCHEDULE_TILES) {
+      hipLaunchKernelGGL(MSKG_VARIANT(k_sweep, la, e->cov_model), dim3(grid), dim3(WAVE), shm, e->stream, P);
+    } else {
+      for (int c = 0; c < 8; c++) {
+        P.colour = c;
+        hipLaunchKernelGGL(MSKG_VARIANT(k_pass, la, e->cov_model), dim3(grid), dim3(WAVE), shm, e->stream, P);
+      }
     }
     hipLaunchKernelGGL(k_rsum, dim3(NPART), dim3(256), 0, e->stream, e->d_tile_r, e->ntiles, e->d_st);
-    hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err);
+    hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err,
+                       e->d_done + e->ntiles);
     HIPCHK(e, hipStreamEndCapture(e->stream, &e->graph));
     HIPCHK(e, hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
   }
@@ -2561,10 +2761,12 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
     int64_t done = (int64_t)(st.events - ev_start);
     if (st.stop) { stop = st.stop == STOP_AT_LIMIT ? MSKG_STOP_LIMIT : st.stop; break; }
     if (done >= max_events || st.clock >= max_time) break;
-    // size the next batch of sweeps from the observed events per sweep
+    // size the next batch of sweeps from the observed events per sweep, at
+    // most doubling the sweeps run so far: sweeps queued past the stop
+    // still cost a launch each
     double per = (double)done / (double)sweeps;
     int64_t want = per > 0 ? (int64_t)((double)(max_events - done) / per * 0.9) : 16;
-    batch = (int)std::max<int64_t>(1, std::min<int64_t>(want, 512));
+    batch = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(want, 512), sweeps));
   }
   HIPCHK(e, hipEventRecord(e->ev1, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
@@ -2737,6 +2939,18 @@ int mskg_last_timing(mskg_engine *e, double *run_ms, double *sweep_ms, int64_t *
   if (sweep_ms) *sweep_ms = e->sweep_ms;
   if (sweep_points) *sweep_points = e->sweep_points;
   if (sweeps_done) *sweeps_done = e->sweeps_done;
+  return MSKG_OK;
+}
+
+int mskg_set_schedule(mskg_engine *e, int32_t schedule) {
+  if (!e) return MSKG_EINVAL;
+  if (schedule != MSKG_SCHEDULE_TILES && schedule != MSKG_SCHEDULE_COLOURS)
+    return fail_(e, MSKG_EINVAL, "schedule %d: MSKG_SCHEDULE_TILES (1) or MSKG_SCHEDULE_COLOURS (0)", schedule);
+  if (schedule != e->schedule) {
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    e->schedule = schedule;
+  }
   return MSKG_OK;
 }
 

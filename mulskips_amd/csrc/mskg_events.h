@@ -25,7 +25,8 @@ enum : int {
   ERR_TILE_LIST = 7,   // a tile's candidate list overflowed
   ERR_SUMS = 8,        // serial selection: the rate sums disagree with the lists
   ERR_LIQUID = 9,      // negative CountLiquid (laser-annealing build's form)
-  ERR_ST_RANGE = 10    // a temperature outside the interpolation tables (InitSt 'ST')
+  ERR_ST_RANGE = 10,   // a temperature outside the interpolation tables (InitSt 'ST')
+  ERR_SCHED = 11       // a tile waited too long for its neighbour tiles (dependency-scheduled sweep)
 };
 
 // Per-tile candidate lists (synchronous-sublattice mode): every MC particle

@@ -298,6 +298,13 @@ class Engine:
             self._guard_seen = g
         return n.value, t.value, s.value
 
+    def set_schedule(self, schedule):
+        """Order of a sweep's colour passes (mskg_set_schedule): "colours" (a
+        barrier after each colour; the default) or "tiles" (each tile as soon
+        as its lower-colour neighbours are done).  Same results bit for bit."""
+        code = {"tiles": _lib.MSKG_SCHEDULE_TILES, "colours": _lib.MSKG_SCHEDULE_COLOURS}[schedule]
+        self._check(self.L.mskg_set_schedule(self.h, code))
+
     def diagnostics(self):
         """Window diagnostics since the state was loaded: guard_exits (tile
         windows cut at the per-window event cap), sweeps, the next window

@@ -17,6 +17,7 @@ module mskg_bindings
                                MSKG_ECAPACITY = 3, MSKG_EDEVICE = 4, MSKG_ENOMEM = 5
   integer(c_int), parameter :: MSKG_STOP_LIMIT = 0, MSKG_STOP_FEW_MC = 1, MSKG_STOP_NO_EVENTS = 3
   integer(c_int32_t), parameter :: MSKG_FIELD_SOCKET = 0, MSKG_FIELD_PLANES = 1
+  integer(c_int32_t), parameter :: MSKG_SCHEDULE_COLOURS = 0, MSKG_SCHEDULE_TILES = 1
   integer(c_int), parameter :: MSKG_KEY_ADS = 2496, MSKG_KEY_PH = 2499, MSKG_KEY_ISO = 2500, MSKG_KEY_N = 2501, &
                                MSKG_ADJ_NS = 2510
 
@@ -275,6 +276,13 @@ module mskg_bindings
       type(c_ptr), value :: eng
       integer(c_int64_t), intent(out) :: guard_exits, sweeps, n_mc
       real(c_double), intent(out) :: tau
+    end function
+
+    ! MSKG_SCHEDULE_COLOURS (0, the default) or MSKG_SCHEDULE_TILES (1)
+    integer(c_int) function mskg_set_schedule(eng, schedule) bind(c, name="mskg_set_schedule")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: eng
+      integer(c_int32_t), value :: schedule
     end function
   end interface
 
