@@ -17,10 +17,12 @@ window for which the 32-seed statistical comparison with serial KMC stays
 within 2 standard errors).
 
 The JSON line also carries
-* roofline: the rate-update sweep kernel (Get_Prob over every lattice point,
-  21 algorithmic bytes per point: the 16-byte site record read, the FP32
-  rate and the class byte written), timed with HIP events on the engine's
-  stream, against the MI355X HBM peak of 8 TB/s;
+* roofline: the rate-update sweep kernel (Get_Prob over the stored points of
+  the tile layers that hold MC particles, 21 algorithmic bytes per point: the
+  16-byte site record read, the FP32 rate and the class byte written), timed
+  with HIP events on the engine's stream, against the MI355X HBM peak of
+  8 TB/s; the points swept, their fraction of the lattice and the same sweep
+  over every stored point (full_box) beside it;
 * cpu_baseline: the CPU oracle (the reference engine restated in C, serial,
   1 core) on the same model at a 480 x 480 x 240 grid for a bounded time.
 """
@@ -51,13 +53,15 @@ def measured_pass_traffic_per_event():
     return None
 
 
-def measured_traffic_per_point():
+def measured_traffic_per_point(key):
     """HBM bytes per lattice point of the sweep kernel from the committed
-    counter profile (FETCH_SIZE + WRITE_SIZE, calibrated against a copy)."""
+    counter profile (FETCH_SIZE + WRITE_SIZE, calibrated against a copy):
+    key traffic_bytes_per_point (every stored point) or
+    active_traffic_bytes_per_point (the tile layers holding MC particles)."""
     try:
         with open(PMC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_point ="):
+                if line.startswith(key + " ="):
                     return float(line.split("=")[1])
     except OSError:
         pass
@@ -362,16 +366,23 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     counts, t_kmc, _ = eng.counters(lists=False)
-    # the rate-update sweep, timed with HIP events on the engine's stream
-    sweeps = []
-    for _ in range(5):
-        eng.refresh_rates()
-        tm = eng.last_timing()
-        sweeps.append(tm["sweep_ms"])
-    sweep_ms = sorted(sweeps)[len(sweeps) // 2]
-    npts = tm["sweep_points"]
+    # the rate-update sweep, timed with HIP events on the engine's stream:
+    # over the tile layers that hold MC particles (the roofline), then over
+    # every stored point (the full-box figure; it leaves the lists to be
+    # rebuilt, so it comes second)
+    def sweep_median(active):
+        ms = []
+        for _ in range(5):
+            eng.refresh_rates(active=active)
+            tm = eng.last_timing()
+            ms.append(tm["sweep_ms"])
+        return sorted(ms)[len(ms) // 2], tm["sweep_points"]
+    sweep_ms, npts = sweep_median(True)
+    full_ms, full_pts = sweep_median(False)
     achieved = npts * SWEEP_BYTES_PER_POINT / (sweep_ms * 1e-3) / 1e9
-    tpp = measured_traffic_per_point()
+    full_achieved = full_pts * SWEEP_BYTES_PER_POINT / (full_ms * 1e-3) / 1e9
+    tpp = measured_traffic_per_point("active_traffic_bytes_per_point")
+    tpp_full = measured_traffic_per_point("traffic_bytes_per_point")
     bpe = measured_pass_traffic_per_event()
     eng.close()
     probe = stream_probe()
@@ -394,12 +405,17 @@ def main():
                                "fluxes 2 / 1.8 ML/s, T=1600 K",
                    "box": list(box), "tile": a.tile, "colours": 8, "tau_policy": a.tau_policy,
                    "events_per_pass": a.events_per_pass, "events_per_step": eps},
-        "roofline": {"kernel": "k_rate_sweep (rate-update sweep)", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "k_rate_sweep over the tile layers holding MC particles (mskg_refresh_active)",
+                     "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": None if tpp is None else tpp * npts,
                      "traffic_source": "profiles/pmc_rate_sweep_sic512.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, "
-                                       "per point, times the points of this launch)",
-                     "points": npts, "bytes_per_point": SWEEP_BYTES_PER_POINT, "ms": sweep_ms,
+                                       "per point of the active-layer sweep, times the points of this launch)",
+                     "points_swept": npts, "points_total": full_pts, "active_fraction": npts / full_pts,
+                     "bytes_per_point": SWEEP_BYTES_PER_POINT, "ms": sweep_ms,
+                     "full_box": {"points": full_pts, "ms": full_ms, "achieved": full_achieved,
+                                  "frac": full_achieved / HBM_PEAK_GBPS,
+                                  "traffic": None if tpp_full is None else tpp_full * full_pts},
                      "stream_probe": probe, "frac_of_triad": achieved / probe["triad_GBps"]},
         # the kernel with most of the device time: bound by the latency of its
         # dependent per-event work; its measured HBM traffic for scale

@@ -194,8 +194,19 @@ int mskg_set_temperature_field(mskg_engine *e, const int32_t *field, int32_t lay
 int mskg_phases(mskg_engine *e, int32_t *coor, int32_t *species, int32_t layout, int32_t on_device);
 
 /* Rate-update sweep: re-evaluates Get_Prob (Get_Prob.f:54-331) for every
- * MC particle of the lattice from the current state.  */
+ * MC particle of the lattice from the current state, reading every stored
+ * point.  */
 int mskg_refresh_rates(mskg_engine *e);
+
+/* The same sweep over the tile layers that hold MC particles only (no
+ * reference counterpart: Get_Prob.f:54-331 over the MC particles, which
+ * lie in those layers).  The engine knows those layers while its tile
+ * lists are current (after mskg_run, mskg_run_serial or a previous
+ * mskg_refresh_active); otherwise -- after an upload, a builder, a replay
+ * or mskg_refresh_rates -- it sweeps every point.  The rates and classes
+ * it leaves equal mskg_refresh_rates' bit for bit, and mskg_last_timing
+ * reports the points it read. */
+int mskg_refresh_active(mskg_engine *e);
 
 /* Advances the lattice with the parallel synchronous-sublattice KMC until
  * max_events events have executed or the clock passes max_time (whichever

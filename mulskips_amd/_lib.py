@@ -23,6 +23,7 @@ EXPORTS = [
     "mskg_track_classes", "mskg_copy_state", "mskg_frame", "mskg_set_temperature_field", "mskg_phases",
     "mskg_set_la_variant", "mskg_la_counts", "mskg_set_st_tables", "mskg_species_counts",
     "mskg_run_diagnostics", "mskg_frame_defects", "mskg_frame_device", "mskg_set_schedule",
+    "mskg_refresh_active",
 ]
 MSKG_FIELD_SOCKET, MSKG_FIELD_PLANES = 0, 1  # layouts of the laser-annealing maps
 
@@ -70,6 +71,7 @@ def lib():
     L.mskg_load_state.argtypes = [P, P, I32, P, P, P, P, I32, P, P, I32, P, P, D]
     L.mskg_set_temperature.argtypes = [P, P]
     L.mskg_refresh_rates.argtypes = [P]
+    L.mskg_refresh_active.argtypes = [P]
     L.mskg_run.argtypes = [P, I64, D, P, P, P]
     L.mskg_replay.argtypes = [P, P, I64, I32]
     L.mskg_store_state.argtypes = [P, P, I32, P, P, P, P, P, I32, P, P, P, I32, P, P, P]

@@ -217,6 +217,41 @@ __device__ inline void sweep_point(const Lattice &L, double rtot, int64_t p, con
 }
 
 // LA, COV: the rate model's laser-annealing and coverage branches are compiled in.
+// Four consecutive points from p0, or the points p0..n-1 at the end.
+template <bool LA, bool COV>
+__device__ inline void sweep_group(const Lattice &L, double rtot, int *err, int atoms_only, int64_t p0, int64_t n) {
+  if (p0 + 4 <= n) {
+    Site sp[4];
+    for (int k = 0; k < 4; k++) sp[k] = L.site[p0 + k];
+    float r[4];
+    uint8_t c[4];
+    bool keep[4];
+    for (int k = 0; k < 4; k++) sweep_point<LA, COV>(L, rtot, p0 + k, sp[k], atoms_only, err, r[k], c[k], keep[k]);
+    if (!(keep[0] || keep[1] || keep[2] || keep[3])) {
+      *(float4 *)(L.rate + p0) = make_float4(r[0], r[1], r[2], r[3]);
+      *(uchar4 *)(L.cls + p0) = make_uchar4(c[0], c[1], c[2], c[3]);
+    } else {
+      for (int k = 0; k < 4; k++)
+        if (!keep[k]) {
+          L.rate[p0 + k] = r[k];
+          L.cls[p0 + k] = c[k];
+        }
+    }
+  } else {
+    for (int64_t q = p0; q < n; q++) {
+      float r;
+      uint8_t c;
+      bool keep;
+      sweep_point<LA, COV>(L, rtot, q, L.site[q], atoms_only, err, r, c, keep);
+      if (!keep) {
+        L.rate[q] = r;
+        L.cls[q] = c;
+      }
+    }
+  }
+}
+
+// LA, COV: the rate model's laser-annealing and coverage branches are compiled in.
 template <bool LA, bool COV>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_rate_sweep(Lattice L, double rtot, int *err, int atoms_only) {
@@ -253,6 +288,19 @@ k_rate_sweep(Lattice L, double rtot, int *err, int atoms_only) {
       }
     }
   }
+}
+
+// The same over the points [first, end) of the stored order (first a
+// multiple of 4), in groups of four counted in 32 bits (the largest lattice
+// holds fewer than 2^31 groups).  A kernel of its own: the whole-lattice
+// sweep keeps its registers without the range.
+template <bool LA, bool COV>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_rate_sweep_range(Lattice L, double rtot, int *err, int64_t first, int64_t end) {
+  const int g_first = (int)(first >> 2), g_end = (int)((end + 3) >> 2);
+  const int gstride = gridDim.x * blockDim.x;
+  for (int g = g_first + blockIdx.x * blockDim.x + threadIdx.x; g < g_end; g += gstride)
+    sweep_group<LA, COV>(L, rtot, err, 0, (int64_t)g * 4, end);
 }
 
 // ---- event-log replay -------------------------------------------------------
@@ -921,13 +969,18 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
       st->dcc[k] = 0ull;
       st->cl[k] = st->cinit[k] - st->cc[k];
     }
-  // the passes of the next sweep visit the layers that held candidates
+  // the passes of the next sweep visit the layers that held candidates; a
+  // sweep whose passes were skipped (the run had stopped) collected none,
+  // and the windows stay those of the last sweep that ran
+  const bool passes_ran = !st->stop;
   int zlo = 0x7fffffff, zhi = -1;
   for (int c = 0; c < 8; c++) {
-    win[4 * c] = win[4 * c + 2];
-    win[4 * c + 1] = win[4 * c + 3];
-    win[4 * c + 2] = 0x7fffffff;
-    win[4 * c + 3] = -1;
+    if (passes_ran) {
+      win[4 * c] = win[4 * c + 2];
+      win[4 * c + 1] = win[4 * c + 3];
+      win[4 * c + 2] = 0x7fffffff;
+      win[4 * c + 3] = -1;
+    }
     zlo = min(zlo, win[4 * c]);
     zhi = max(zhi, win[4 * c + 1]);
   }
@@ -2308,6 +2361,14 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   HIPCHK(e, hipMemsetAsync(e->d_tl_own, 0, e->ntiles * sizeof(int32_t), e->stream));
   HIPCHK(e, hipMemsetAsync(e->d_tile_r, 0, e->ntiles * sizeof(double), e->stream));
   HIPCHK(e, hipMemsetAsync(e->d_list_err, 0, sizeof(int), e->stream));
+  {  // empty windows: lists built before any run widen them
+    int w0[32];
+    for (int c = 0; c < 8; c++) {
+      w0[4 * c] = w0[4 * c + 2] = 0x7fffffff;
+      w0[4 * c + 1] = w0[4 * c + 3] = -1;
+    }
+    HIPCHK(e, hipMemcpyAsync(e->d_win, w0, sizeof w0, hipMemcpyHostToDevice, e->stream));
+  }
   // completion stamps: 0 is no sweep's (the first k_sweep_end makes it 1)
   HIPCHK(e, hipMemsetAsync(e->d_done, 0, ((size_t)e->ntiles + 8) * sizeof(unsigned int), e->stream));
   RunState st;
@@ -2607,20 +2668,33 @@ int mskg_la_counts(mskg_engine *e, int64_t count[3], int64_t liquid[3]) {
   return MSKG_OK;
 }
 
-int mskg_refresh_rates(mskg_engine *e) {
-  if (!e || !e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+// The rate sweep over the stored points [first, end).
+// A sweep of the whole lattice recomputes the classes from the records, so
+// the tile lists are rebuilt at the next run; a sweep of the layers that
+// hold the MC particles of engine-kept lists changes rates only (a point's
+// class is zero exactly when its record makes it no MC particle, which a
+// sweep does not change), so the lists stay.
+static int refresh_range(mskg_engine *e, int64_t first, int64_t end, bool lists_stay) {
   RunState st;
   int rc = read_state(e, st);
   if (rc) return rc;
   HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
   Lattice L = make_lattice(e);
   L.inlist = nullptr;
-  e->lists_dirty = true;
-  int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
+  if (!lists_stay) e->lists_dirty = true;
   HIPCHK(e, hipEventRecord(e->ev0, e->stream));
-  hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
-  HIPCHK(e, hipGetLastError());
-  if (e->track) e->keys_valid = true;
+  const bool la = e->tab.la_mode || e->tab.st_n;
+  if (first == 0 && end == e->box.npts) {
+    const int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, la, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, st.rtot,
+                       e->d_err, 0);
+    HIPCHK(e, hipGetLastError());
+  } else if (end > first) {
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((end - first + 1023) / 1024, 256 * 64));
+    hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep_range, la, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L,
+                       st.rtot, e->d_err, first, end);
+    HIPCHK(e, hipGetLastError());
+  }
   HIPCHK(e, hipEventRecord(e->ev1, e->stream));
   int err = 0;
   HIPCHK(e, hipMemcpyAsync(&err, e->d_err, sizeof(int), hipMemcpyDeviceToHost, e->stream));
@@ -2628,9 +2702,44 @@ int mskg_refresh_rates(mskg_engine *e) {
   float ms = 0;
   HIPCHK(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
   e->sweep_ms = ms;
-  e->sweep_points = e->box.npts;
+  e->sweep_points = end - first;
   if (err) return fail_(e, MSKG_ESTATE, "rate sweep: %s", err_name(err));
   return MSKG_OK;
+}
+
+int mskg_refresh_rates(mskg_engine *e) {
+  if (!e || !e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  const int rc = refresh_range(e, 0, e->box.npts, false);
+  if (!rc && e->track) e->keys_valid = true;
+  return rc;
+}
+
+int mskg_refresh_active(mskg_engine *e) {
+  if (!e || !e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  // the tile layers that hold MC particles: every tile layer that held a
+  // list entry since the lists were built is in the window of some colour
+  // (windows only widen between sweeps, and a sweep keeps the layers its
+  // tiles and their appends touched).  With lists that are not current (an
+  // upload, a replay, a whole-lattice sweep since) every point is swept.
+  if (e->lists_dirty) return mskg_refresh_rates(e);
+  int zlo = 0, zhi = e->ntz - 1;
+  {
+    int w[32];
+    HIPCHK(e, hipMemcpyAsync(w, e->d_win, sizeof w, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    zlo = 0x7fffffff;
+    zhi = -1;
+    for (int c = 0; c < 8; c++) {
+      zlo = std::min(zlo, w[4 * c]);
+      zhi = std::max(zhi, w[4 * c + 1]);
+    }
+    if (zlo > zhi) zlo = 0, zhi = -1;  // no MC particle
+    zlo = std::max(zlo, 0);
+    zhi = std::min(zhi, e->ntz - 1);
+  }
+  const Box &b = e->box;
+  const int64_t z0 = (int64_t)zlo * e->tile, z1 = zhi < zlo ? z0 : std::min<int64_t>((int64_t)(zhi + 1) * e->tile, b.lz + 1);
+  return refresh_range(e, z0 * b.plane, z1 * b.plane, true);
 }
 
 int mskg_replay(mskg_engine *e, const mskg_event *log, int64_t n, int32_t clock_mode) {

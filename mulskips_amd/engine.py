@@ -276,8 +276,11 @@ class Engine:
         lx, ly, lz = self.box
         return (lx * ly, lz) if code == MSKG_FIELD_SOCKET else (lz, ly, lx)
 
-    def refresh_rates(self):
-        self._check(self.L.mskg_refresh_rates(self.h))
+    def refresh_rates(self, active=False):
+        """Get_Prob for every MC particle: over every stored point, or with
+        active=True over the tile layers that hold MC particles (the whole
+        lattice when the engine's tile lists are not current)."""
+        self._check(self.L.mskg_refresh_active(self.h) if active else self.L.mskg_refresh_rates(self.h))
 
     def run(self, max_events, max_time=float("inf")):
         """Synchronous-sublattice KMC until max_events events or the clock

@@ -139,6 +139,11 @@ module mskg_bindings
       type(c_ptr), value :: eng
     end function
 
+    integer(c_int) function mskg_refresh_active(eng) bind(c, name="mskg_refresh_active")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng
+    end function
+
     integer(c_int) function mskg_run(eng, max_events, max_time, events_done, kmc_time, stop_reason) &
         bind(c, name="mskg_run")
       import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
