@@ -226,12 +226,17 @@ def la_leg(cycles, events_per_cycle=100_000, seed=2026):
                         "k = 4 as in the main leg"}
 
 
+HIST_EVERY = 10_000
+
+
 def adjoint_leg(n_traj, n_events, segment):
     """Adjoint gradients/s: n_traj SiC trajectories (144^3 grid, second-
     neighbour model, frozen substrate) recorded by the serial kernel with a
-    checkpoint every `segment` events; the adjoint kernel replays all
-    segments of all trajectories in one launch, and each trajectory's
-    8-parameter log-likelihood gradient follows from its summed statistics."""
+    lattice checkpoint every `segment` events; the adjoint kernel replays all
+    segments of all trajectories in one launch, writing the class
+    populations (the class-keyed rate tables) every 10^4 events, and each
+    trajectory's 8-parameter log-likelihood gradient follows from its summed
+    statistics."""
     import torch
 
     from mulskips_amd import adjoint
@@ -252,7 +257,7 @@ def adjoint_leg(n_traj, n_events, segment):
     t_rec = time.perf_counter() - t0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    stats = adjoint.segmented_stats(segments)
+    stats, hist = adjoint.segmented_stats(segments, every=HIST_EVERY)
     grads = [adjoint.gradient(theta0, adjoint.TrajectoryStats(S, A, sc), 1600.0)[1] for S, A, sc in stats]
     t_adj = time.perf_counter() - t0
     for traj in segments:
@@ -261,6 +266,7 @@ def adjoint_leg(n_traj, n_events, segment):
     g = torch.stack(grads).numpy()
     return {"value": n_traj * len(theta0) / t_adj, "unit": "gradients/s (trajectories x parameters / s)",
             "trajectories": n_traj, "events_per_trajectory": n_events, "checkpoint_every": segment,
+            "class_checkpoints_every": HIST_EVERY, "class_checkpoints": int(sum(len(h) for h in hist)),
             "parameters": list(adjoint.PARAMS), "adjoint_s": t_adj, "replayed_events_per_s": n_traj * n_events / t_adj,
             "record_s": t_rec, "mean_gradient": [float(v) for v in g.mean(axis=0)],
             "workload": "3C-SiC second-neighbour model, 144^3 grid, frozen substrate, 1600 K; trajectories "

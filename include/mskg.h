@@ -323,7 +323,8 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
  * of coordination c: MSKG_KEY_ADS + c - 1 (its A; its selections are split
  * by deposited species in S[MSKG_KEY_N + 3*(s-1) + c-1]); the placeholder
  * deposition MSKG_KEY_PH (rate 1e-8); isolated atoms MSKG_KEY_ISO, whose
- * rates 1e5*R are per point and enter only through scalars.
+ * rates 1e5*R are per point and enter only through scalars (their A is the
+ * integral of their number).
  * S: n x MSKG_ADJ_NS, A: n x MSKG_KEY_N, scalars: n x 4 = (sum of dt,
  * integral of the isolated-atom rates, sum of their log rates, events
  * replayed).  Models without coverage species and without LA rates only.
@@ -344,6 +345,15 @@ int mskg_track_classes(mskg_engine *e, int32_t on);
 int mskg_copy_state(mskg_engine *dst, const mskg_engine *src);
 int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const *logs, const double *const *dts,
                  const int64_t *n_events, double *S, double *A, double *scalars);
+
+/* mskg_adjoint with checkpoints (no reference counterpart): with every > 0,
+ * hist[i] receives, for replica i, the class populations n_c (MSKG_KEY_N
+ * int32 each, isolated atoms included) after events every, 2*every, ...:
+ * n_events[i] / every rows.  The populations at 1e4-event intervals are the
+ * rate tables of a class-keyed model at those times (R = sum_c n_c r_c). */
+int mskg_adjoint_checkpointed(mskg_engine *const *engines, int32_t n, const mskg_event *const *logs,
+                              const double *const *dts, const int64_t *n_events, double *S, double *A, double *scalars,
+                              int64_t every, int32_t *const *hist);
 
 /* Frame compaction for the xyz output (inpout/WriteMolMolXYZFile.f:77-260):
  * one pass over the lattice writes, as (x, y, z, code) int32 quadruples in

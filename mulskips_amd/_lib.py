@@ -23,7 +23,7 @@ EXPORTS = [
     "mskg_track_classes", "mskg_copy_state", "mskg_frame", "mskg_set_temperature_field", "mskg_phases",
     "mskg_set_la_variant", "mskg_la_counts", "mskg_set_st_tables", "mskg_species_counts",
     "mskg_run_diagnostics", "mskg_frame_defects", "mskg_frame_device", "mskg_set_schedule",
-    "mskg_refresh_active",
+    "mskg_refresh_active", "mskg_adjoint_checkpointed",
 ]
 MSKG_FIELD_SOCKET, MSKG_FIELD_PLANES = 0, 1  # layouts of the laser-annealing maps
 
@@ -97,6 +97,7 @@ def lib():
     L.mskg_set_st_tables.argtypes = [P, I32, P, P, P, P, P]
     L.mskg_species_counts.argtypes = [P, P, P]
     L.mskg_adjoint.argtypes = [P, I32, P, P, P, P, P, P]
+    L.mskg_adjoint_checkpointed.argtypes = [P, I32, P, P, P, P, P, P, I64, P]
     L.mskg_geometry_find_nn.argtypes = [P, P, P, I32, I32, P]
     L.mskg_geometry_find_sn.argtypes = [P, P, P, I32, I32, P, P]
     _lib = L

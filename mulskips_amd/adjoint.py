@@ -156,11 +156,12 @@ def gradient(theta, stats, temperature):
     return ll.detach(), g
 
 
-def trajectory_stats(engines, logs, dts):
-    """Run the adjoint kernel on engines loaded with the initial states."""
+def trajectory_stats(engines, logs, dts, every=0):
+    """Run the adjoint kernel on engines loaded with the initial states; with
+    every > 0 also the class-population checkpoints after every `every`
+    events (one int32 array (len // every, KEY_N) per trajectory)."""
     from .engine import adjoint_stats
-    S, A, sc = adjoint_stats(engines, logs, dts)
-    return S, A, sc
+    return adjoint_stats(engines, logs, dts, every=every)
 
 
 def waiting_times(log, t0=0.0):
@@ -206,14 +207,27 @@ def record_segmented(make_engine, seeds, n_events, segment):
     return logs, dts, segments
 
 
-def segmented_stats(segments):
+def segmented_stats(segments, every=0):
     """Adjoint statistics of segmented trajectories in one launch; returns
-    per-trajectory (S, A, scalars) summed over each trajectory's segments."""
+    per-trajectory (S, A, scalars) summed over each trajectory's segments.
+    With every > 0 (a divisor of every segment's length) it also returns,
+    per trajectory, the class populations after every `every` events along
+    the whole trajectory (the segments' checkpoints in order), as
+    (per_trajectory, checkpoints)."""
     flat = [seg for traj in segments for seg in traj]
-    S, A, sc = trajectory_stats([c for c, _, _ in flat], [lg for _, lg, _ in flat], [d for _, _, d in flat])
-    out, k = [], 0
+    if every > 0:
+        for _, lg, _ in flat:
+            if len(lg) % every:
+                raise ValueError(f"a segment of {len(lg)} events is not a multiple of the checkpoint interval {every}")
+        S, A, sc, hist = trajectory_stats([c for c, _, _ in flat], [lg for _, lg, _ in flat],
+                                          [d for _, _, d in flat], every=every)
+    else:
+        S, A, sc = trajectory_stats([c for c, _, _ in flat], [lg for _, lg, _ in flat], [d for _, _, d in flat])
+    out, ck, k = [], [], 0
     for traj in segments:
         m = len(traj)
         out.append((S[k:k + m].sum(axis=0), A[k:k + m].sum(axis=0), sc[k:k + m].sum(axis=0)))
+        if every > 0:
+            ck.append(np.concatenate(hist[k:k + m], axis=0))
         k += m
-    return out
+    return (out, ck) if every > 0 else out

@@ -1348,6 +1348,8 @@ struct AdjDesc {
   double *A;                 // [KEY_N]
   double *scal;              // [4]: sum dt, integral of isolated rates, sum of their log rates, events
   RunState *st;
+  int32_t *hist;             // may be null: [n / every][KEY_N] class populations after every `every` events
+  long long every;
 };
 
 __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
@@ -1446,8 +1448,8 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
         if (o == -3) continue;
         if (nw == KEY_UNSUPPORTED) { bad = ERR_COVERAGE; break; }
         if (o != nw) {
-          if (o >= 0 && o != KEY_ISO) { A[o] += cnt[o] * (t - tau[o]); tau[o] = t; cnt[o]--; }
-          if (nw >= 0 && nw != KEY_ISO) { A[nw] += cnt[nw] * (t - tau[nw]); tau[nw] = t; cnt[nw]++; }
+          if (o >= 0) { A[o] += cnt[o] * (t - tau[o]); tau[o] = t; cnt[o]--; }
+          if (nw >= 0) { A[nw] += cnt[nw] * (t - tau[nw]); tau[nw] = t; cnt[nw]++; }
         }
         const double ro = o == KEY_ISO ? (double)s_ro[q] : 0.0;
         const double rn = nw == KEY_ISO ? (double)s_rn[q] : 0.0;
@@ -1461,6 +1463,10 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
     }
     __syncthreads();
     if (s_bad) { err = s_bad; k++; break; }
+    if (D.hist && (k + 1) % D.every == 0) {  // checkpoint: the class populations after event k
+      int32_t *row = D.hist + ((k + 1) / D.every - 1) * KEY_N;
+      for (int c = lane; c < KEY_N; c += WAVE) row[c] = cnt[c];
+    }
     if ((k + 1) % ADJ_EPOCH == 0) {  // restart the accounting clock
       for (int c = lane; c < KEY_N; c += WAVE) {
         A[c] += cnt[c] * (t - tau[c]);
@@ -1489,7 +1495,7 @@ __global__ void __launch_bounds__(256) k_key_hist(Lattice L, int *cnt, double *i
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const int k = L.key[p];
     if (k == KEY_ISO) atomicAdd(iso, (double)L.rate[p]);
-    else if (k >= 0) atomicAdd(&cnt[k], 1);
+    if (k >= 0) atomicAdd(&cnt[k], 1);
     else if (k == KEY_UNSUPPORTED) atomicAdd(&cnt[KEY_N], 1);
   }
 }
@@ -3349,7 +3355,14 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
 
 int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const *logs, const double *const *dts,
                  const int64_t *n_events, double *S, double *A, double *scalars) {
+  return mskg_adjoint_checkpointed(engines, n, logs, dts, n_events, S, A, scalars, 0, nullptr);
+}
+
+int mskg_adjoint_checkpointed(mskg_engine *const *engines, int32_t n, const mskg_event *const *logs,
+                              const double *const *dts, const int64_t *n_events, double *S, double *A, double *scalars,
+                              int64_t every, int32_t *const *hist) {
   if (!engines || n <= 0 || !logs || !dts || !n_events || !S || !A || !scalars) return MSKG_EINVAL;
+  if (every < 0 || (every > 0 && !hist) || (every == 0 && hist)) return fail_(engines[0], MSKG_EINVAL, "checkpoints: every %lld", (long long)every);
   mskg_engine *e0 = engines[0];
   if (!e0) return MSKG_EINVAL;
   std::vector<AdjDesc> descs(n);
@@ -3400,7 +3413,9 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
     double *d_S = (double *)dev(ADJ_NS * sizeof(double));
     double *d_A = (double *)dev(KEY_N * sizeof(double));
     double *d_sc = (double *)dev(4 * sizeof(double));
-    if (!d_cnt || !d_iso || !d_log || !d_dt || !d_S || !d_A || !d_sc) {
+    const int64_t rows = every > 0 ? n_events[i] / every : 0;
+    int32_t *d_hist = rows > 0 ? (int32_t *)dev((size_t)rows * KEY_N * sizeof(int32_t)) : nullptr;
+    if (!d_cnt || !d_iso || !d_log || !d_dt || !d_S || !d_A || !d_sc || (rows > 0 && !d_hist)) {
       cleanup();
       return fail_(e, MSKG_ENOMEM, "adjoint: no device memory for the log");
     }
@@ -3441,6 +3456,8 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
     D.A = d_A;
     D.scal = d_sc;
     D.st = e->d_st;
+    D.hist = d_hist;
+    D.every = every > 0 ? every : 1;
   }
   AdjDesc *d_descs = (AdjDesc *)dev(n * sizeof(AdjDesc));
   if (!d_descs) { cleanup(); return fail_(e0, MSKG_ENOMEM, "adjoint: descriptors"); }
@@ -3463,6 +3480,9 @@ int mskg_adjoint(mskg_engine *const *engines, int32_t n, const mskg_event *const
     ce = hipMemcpy(S + (size_t)i * ADJ_NS, descs[i].S, ADJ_NS * sizeof(double), hipMemcpyDeviceToHost);
     if (ce == hipSuccess) ce = hipMemcpy(A + (size_t)i * KEY_N, descs[i].A, KEY_N * sizeof(double), hipMemcpyDeviceToHost);
     if (ce == hipSuccess) ce = hipMemcpy(scalars + (size_t)i * 4, descs[i].scal, 4 * sizeof(double), hipMemcpyDeviceToHost);
+    if (ce == hipSuccess && descs[i].hist)
+      ce = hipMemcpy(hist[i], descs[i].hist, (size_t)(n_events[i] / every) * KEY_N * sizeof(int32_t),
+                     hipMemcpyDeviceToHost);
     if (ce != hipSuccess) { cleanup(); return fail_(e, MSKG_EDEVICE, "adjoint download: %s", hipGetErrorString(ce)); }
     RunState st;
     int rc = read_state(e, st);

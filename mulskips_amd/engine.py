@@ -415,12 +415,15 @@ def run_serial(engines, max_events, max_time=float("inf")):
     return done, clock, stop
 
 
-def adjoint_stats(engines, logs, dts):
+def adjoint_stats(engines, logs, dts, every=0):
     """Likelihood statistics of recorded trajectories (mskg_adjoint): engine i
     holds the initial state of trajectory i, logs[i] its events and dts[i]
     the waiting time before each.  Returns S (n, ADJ_NS), A (n, KEY_N) and
     scalars (n, 4) = (sum dt, isolated-rate integral, isolated log-rate sum,
-    events replayed).  The engines end in the final states."""
+    events replayed); with every > 0 also the checkpoints
+    (mskg_adjoint_checkpointed): for each trajectory an int32 array
+    (len(log) // every, KEY_N) of the class populations after every
+    `every` events.  The engines end in the final states."""
     L = _lib.lib()
     n = len(engines)
     logs = [np.ascontiguousarray(lg, EVENT_DTYPE) for lg in logs]
@@ -434,8 +437,16 @@ def adjoint_stats(engines, logs, dts):
     S = np.zeros((n, _lib.ADJ_NS))
     A = np.zeros((n, _lib.KEY_N))
     sc = np.zeros((n, 4))
-    rc = L.mskg_adjoint(_engine_array(engines), n, lp, dp, ptr(ne), ptr(S), ptr(A), ptr(sc))
+    if every > 0:
+        hist = [np.zeros((len(lg) // every, _lib.KEY_N), np.int32) for lg in logs]
+        hp = (C.c_void_p * n)(*[h.ctypes.data for h in hist])
+        rc = L.mskg_adjoint_checkpointed(_engine_array(engines), n, lp, dp, ptr(ne), ptr(S), ptr(A), ptr(sc),
+                                         int(every), hp)
+    else:
+        rc = L.mskg_adjoint(_engine_array(engines), n, lp, dp, ptr(ne), ptr(S), ptr(A), ptr(sc))
     if rc:
         msg = "; ".join(L.mskg_last_error(e.h).decode() for e in engines if L.mskg_last_error(e.h))
         raise EngineError(rc, msg or "mskg_adjoint failed")
+    if every > 0:
+        return S, A, sc, hist
     return S, A, sc

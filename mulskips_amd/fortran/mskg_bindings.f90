@@ -230,6 +230,17 @@ module mskg_bindings
       real(c_double), intent(out) :: S(*), A(*), scalars(*)
     end function
 
+    ! ... with class-population checkpoints every `every` events (hist: n pointers)
+    integer(c_int) function mskg_adjoint_checkpointed(engines, n, logs, dts, n_events, S, A, scalars, every, hist) &
+        bind(c, name="mskg_adjoint_checkpointed")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), intent(in) :: engines(*), logs(*), dts(*), hist(*)
+      integer(c_int32_t), value :: n
+      integer(c_int64_t), intent(in) :: n_events(*)
+      real(c_double), intent(out) :: S(*), A(*), scalars(*)
+      integer(c_int64_t), value :: every
+    end function
+
     integer(c_int) function mskg_track_classes(eng, on) bind(c, name="mskg_track_classes")
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: eng

@@ -2193,6 +2193,14 @@ static double kval(const ksum *k) { return k->sum + k->c; }
 
 int orc_adjoint_stats(orc_sim *s, const orc_event *evs, int64_t n, const double *dt, double *S, double *A,
                       double *scal) {
+  return orc_adjoint_checkpoints(s, evs, n, dt, S, A, scal, 0, NULL);
+}
+
+/* The same with the class populations after every `every` events
+ * (every > 0): hist holds n / every rows of KEY_N counts, the MC particles
+ * by class key (isolated atoms included) after events every, 2*every, ... */
+int orc_adjoint_checkpoints(orc_sim *s, const orc_event *evs, int64_t n, const double *dt, double *S, double *A,
+                            double *scal, int64_t every, int32_t *hist) {
   if (setjmp(s->jb)) { s->have_jb = 0; s->force_arm = -1; return 2; }
   s->have_jb = 1;
   memset(S, 0, sizeof(double) * ADJ_NS);
@@ -2204,7 +2212,7 @@ int orc_adjoint_stats(orc_sim *s, const orc_event *evs, int64_t n, const double 
     for (int32_t i = 1; i <= s->n_ad; i++) {
       const int key = s->ad[i].key;
       if (key == KEY_ISO) kadd(&iso_int, s->ad[i].prob * d);
-      else if (key >= 0) A[key] += d;
+      if (key >= 0) A[key] += d;
       else if (key == -2) fail(s, "orc_adjoint_stats: a rate class the likelihood does not cover");
     }
     kadd(&logl, -s->tree[1] * d);
@@ -2236,6 +2244,12 @@ int orc_adjoint_stats(orc_sim *s, const orc_event *evs, int64_t n, const double 
     do_execute(s, ind, it);
     s->force_arm = -1;
     s->iter++;
+    if (every > 0 && (k + 1) % every == 0) {
+      int32_t *row = hist + ((k + 1) / every - 1) * KEY_N;
+      memset(row, 0, sizeof(int32_t) * KEY_N);
+      for (int32_t i = 1; i <= s->n_ad; i++)
+        if (s->ad[i].key >= 0) row[s->ad[i].key]++;
+    }
   }
   scal[0] = kval(&ttot);
   scal[1] = kval(&iso_int);

@@ -173,6 +173,10 @@ int orc_refresh_all(orc_sim *s);
  * coverage species and LA rates.  No reference counterpart. */
 int orc_adjoint_stats(orc_sim *s, const orc_event *evs, int64_t n, const double *dt, double *S, double *A,
                       double *scal);
+/* ... and the class populations after every `every` events (hist: n / every
+ * rows of KEY_N counts, isolated atoms included). */
+int orc_adjoint_checkpoints(orc_sim *s, const orc_event *evs, int64_t n, const double *dt, double *S, double *A,
+                            double *scal, int64_t every, int32_t *hist);
 
 /* Write the main xyz frame of WriteMolMolXYZFile.f (and its SiC
  * wrong-site side effect on LattCoo) exactly as the reference does. */

@@ -55,6 +55,7 @@ def _load():
         L.orc_run_to.argtypes = [P, I64, D, P]
         L.orc_executed.argtypes = [P, P]
         L.orc_adjoint_stats.argtypes = [P, P, I64, P, P, P, P]
+        L.orc_adjoint_checkpoints.argtypes = [P, P, I64, P, P, P, P, I64, P]
         L.orc_write_frame.argtypes = [P, C.c_char_p, C.c_char_p, I64]
         L.orc_write_frames.argtypes = [P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, I64]
         L.orc_set_temperature_field.argtypes = [P, P, I]
@@ -219,16 +220,23 @@ class Oracle:
         """Get_Prob for every MC particle (fresh rates and class keys)."""
         self._check(self.L.orc_refresh_all(self.h), "orc_refresh_all")
 
-    def adjoint_stats(self, log, dt):
+    def adjoint_stats(self, log, dt, every=0):
         """Brute-force likelihood statistics of a trajectory replayed from the
         current state: S (2510,), A (2501,), scal (4,) = (sum dt, isolated-rate
-        integral, isolated log-rate sum, direct log-likelihood)."""
+        integral, isolated log-rate sum, direct log-likelihood); with every > 0
+        also the class populations after every `every` events, (len // every,
+        2501) int32."""
         log = np.ascontiguousarray(log, EVENT_DTYPE)
         dt = np.ascontiguousarray(dt, np.float64)
         assert len(dt) == len(log)
         S = np.zeros(2510)
         A = np.zeros(2501)
         sc = np.zeros(4)
+        if every > 0:
+            hist = np.zeros((len(log) // every, 2501), np.int32)
+            self._check(self.L.orc_adjoint_checkpoints(self.h, _p(log), len(log), _p(dt), _p(S), _p(A), _p(sc),
+                                                       int(every), _p(hist)), "orc_adjoint_checkpoints")
+            return S, A, sc, hist
         self._check(self.L.orc_adjoint_stats(self.h, _p(log), len(log), _p(dt), _p(S), _p(A), _p(sc)),
                     "orc_adjoint_stats")
         return S, A, sc
