@@ -76,7 +76,18 @@ typedef struct {
                                positions (+-6,+-6,0 and permutations), e.g. exp(-n2*E2/kT)
                                for a second-neighbour bond energy E2; NULL = off.  Needs
                                tile >= 24. */
-  int32_t reserved[4];
+  int32_t stored_len_z;     /* engine option without a reference counterpart: the grid planes
+                               z = 0..stored_len_z-1 are held in device memory, the planes
+                               above are vacuum the run never reaches (a film that grows to
+                               within 24 planes of the stored top stops the run with an
+                               error, before any event reads or writes above it); 0 = all
+                               len_z+1 planes.  A box of tall vacuum then costs no memory:
+                               a stored plane holds len_x*len_y/4 lattice points of 22 B.
+                               With it, mskg_load_state reads (and mskg_store_state writes)
+                               only the stored planes of LattCoo, the laser-annealing maps
+                               and forms are unavailable, and every run, replay, rate and
+                               frame result equals that of the engine storing all planes. */
+  int32_t reserved[3];
 } mskg_config;
 
 /* One logged event, as written by the reference loop instrumented at
@@ -304,6 +315,27 @@ int mskg_run_diagnostics(mskg_engine *e, int64_t *guard_exits, int64_t *sweeps, 
  * runs in. */
 int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_events, const double *max_time,
                     int64_t *events_done, double *kmc_time, int32_t *stop_reason);
+
+/* The batched ensemble (no reference counterpart; the reference runs one
+ * lattice per process, main.f:304-371): the synchronous-sublattice sweep of
+ * mskg_run on n independent lattices at once.  Each colour pass is one
+ * launch over the tiles of that colour of every replica still running,
+ * taken from one queue by persistent waves, so small replicas share the
+ * GPU with large ones and the passes' slowest tiles overlap other
+ * replicas' work; the per-replica window, clock and stop follow after each
+ * sweep in one launch, and a sweep (8 passes, tile sums, sweep end, the
+ * next sweep's plan) is one captured graph.  Replica i runs to
+ * max_events[i] events or max_time[i] (as mskg_run) with its own tables,
+ * tile edge, window policy, seed and event log; its lattice, lists,
+ * counters and clock after the call equal, bit for bit, those of mskg_run
+ * on that engine alone with the same limits.  Engines must be distinct,
+ * live on the current device and share the rate-model kernel (all or none
+ * laser-annealing / 'ST', all or none with coverage rates); at most 1024.
+ * The launches use engines[0]'s stream.  Stop reasons as mskg_run's; on a
+ * replica's error the others finish and the call returns MSKG_ESTATE with
+ * that replica's message in mskg_last_error(engines[0]). */
+int mskg_run_ensemble(mskg_engine *const *engines, int32_t n, const int64_t *max_events, const double *max_time,
+                      int64_t *events_done, double *kmc_time, int32_t *stop_reason);
 
 /* Trajectory likelihood statistics (no reference counterpart; for parameter
  * calibration).  For each engine i, loaded with the initial state of a

@@ -23,7 +23,7 @@ EXPORTS = [
     "mskg_track_classes", "mskg_copy_state", "mskg_frame", "mskg_set_temperature_field", "mskg_phases",
     "mskg_set_la_variant", "mskg_la_counts", "mskg_set_st_tables", "mskg_species_counts",
     "mskg_run_diagnostics", "mskg_frame_defects", "mskg_frame_device", "mskg_set_schedule",
-    "mskg_refresh_active", "mskg_adjoint_checkpointed",
+    "mskg_refresh_active", "mskg_adjoint_checkpointed", "mskg_run_ensemble",
 ]
 MSKG_FIELD_SOCKET, MSKG_FIELD_PLANES = 0, 1  # layouts of the laser-annealing maps
 
@@ -42,7 +42,8 @@ class Config(C.Structure):
         ("la_mode", C.c_int32), ("tm", C.c_double), ("p0", C.c_double), ("sigma", C.c_double),
         ("tflex", C.c_double), ("seed", C.c_uint64), ("tile", C.c_int32), ("tile_capacity", C.c_int32),
         ("events_per_pass", C.c_double), ("tau_policy", C.c_int32),
-        ("frozen_below_z", C.c_int32), ("second_factor", C.c_void_p), ("reserved", C.c_int32 * 4),
+        ("frozen_below_z", C.c_int32), ("second_factor", C.c_void_p), ("stored_len_z", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -84,6 +85,7 @@ def lib():
     L.mskg_set_event_log.argtypes = [P, I64]
     L.mskg_get_event_log.argtypes = [P, P, I64, P]
     L.mskg_run_serial.argtypes = [P, I32, P, P, P, P, P]
+    L.mskg_run_ensemble.argtypes = [P, I32, P, P, P, P, P]
     L.mskg_build_alloy_slab.argtypes = [P, I32, D]
     L.mskg_track_classes.argtypes = [P, I32]
     L.mskg_copy_state.argtypes = [P, P]

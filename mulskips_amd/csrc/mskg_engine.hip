@@ -52,6 +52,10 @@ constexpr int NSLOT = 128;
 constexpr int STOP_AT_LIMIT = 100;
 // Events one tile may execute in one window (a bound on the wave's work).
 constexpr int PASS_EVENT_CAP = 1 << 16;
+// With only the bottom planes stored (stored_len_z), an event's site stays
+// this many planes below the top stored plane (Lattice::ztop): an event
+// reads at most 12 planes and writes at most 6 planes away from its site.
+constexpr int TOP_MARGIN = 24;
 struct Slot {
   unsigned long long rmax;  // bit pattern of the largest (non-negative) tile rate
   unsigned long long n, tiles, events, counts[8];
@@ -103,6 +107,7 @@ struct PassParams {
   int cap;
   uint32_t key0, key1;
   int colour;
+  int rsum_chunk;           // tiles per block of the fixed-order tile sum (k_rsum)
   // dependency-scheduled sweep: per-tile completion stamps, and sched[0]
   // the sweep's stamp, [1] queue head, [2] abort flag, [3] / [4] the first /
   // last tile layer holding candidates when the sweep starts
@@ -745,6 +750,26 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
   return 0;
 }
 
+// The tiles of `colour` in the colour's window of tile layers that hold
+// candidates (win[4c], win[4c+1], fixed during a sweep): their number, and
+// zfirst, the lowest of those layers with the colour's layer parity.
+__device__ inline int colour_tiles(const Lattice &L, int ntz, int colour, int &zfirst) {
+  const int c2 = (colour >> 2) & 1;
+  const int zlo = max(L.win[4 * colour], 0), zhi = min(L.win[4 * colour + 1], ntz - 1);
+  zfirst = 0;
+  if (zlo > zhi) return 0;  // no tile of this colour holds candidates
+  zfirst = zlo + ((c2 - zlo) & 1);
+  const int nlayers = zfirst <= zhi ? (zhi - zfirst) / 2 + 1 : 0;
+  return nlayers * (L.ntx / 2) * (L.nty / 2);
+}
+// The wi-th of those tiles: layer by layer, y then x.
+__device__ inline int colour_tile(const Lattice &L, int colour, int zfirst, int wi) {
+  const int c0 = colour & 1, c1 = (colour >> 1) & 1;
+  const int hx = L.ntx / 2, per_layer = hx * (L.nty / 2);
+  const int layer = wi / per_layer, rr = wi - layer * per_layer;
+  return ((zfirst + 2 * layer) * L.nty + 2 * (rr / hx) + c1) * L.ntx + 2 * (rr % hx) + c0;
+}
+
 // One colour pass: the tiles of `colour` in the window of tile layers that
 // hold candidates, each workgroup (one wave) striding over them.  Three
 // waves per SIMD (168 VGPRs): 3072 tiles in flight on 256 CUs, against the
@@ -761,16 +786,10 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
   RunState *st = P.st;
   const double tau = st->tau;
   if (!(tau > 0.0) || st->err || st->stop) return;
-  const int c0 = P.colour & 1, c1 = (P.colour >> 1) & 1, c2 = (P.colour >> 2) & 1;
-  const int zlo = max(L.win[4 * P.colour], 0), zhi = min(L.win[4 * P.colour + 1], P.ntz - 1);
-  if (zlo > zhi) return;  // no tile of this colour holds candidates
-  const int zfirst = zlo + ((c2 - zlo) & 1);
-  const int nlayers = zfirst <= zhi ? (zhi - zfirst) / 2 + 1 : 0;
-  const int hx = L.ntx / 2, per_layer = hx * (L.nty / 2);
-  const int nwin = nlayers * per_layer;
+  int zfirst;
+  const int nwin = colour_tiles(L, P.ntz, P.colour, zfirst);
   for (int wi = blockIdx.x; wi < nwin; wi += gridDim.x) {
-    const int layer = wi / per_layer, rr = wi - layer * per_layer;
-    const int t = ((zfirst + 2 * layer) * L.nty + 2 * (rr / hx) + c1) * L.ntx + 2 * (rr % hx) + c0;
+    const int t = colour_tile(L, P.colour, zfirst, wi);
     if (pass_tile<LA, COV>(P, L, st, tau, t, P.colour, S, lk, lr, lc, lane)) return;
   }
 }
@@ -915,10 +934,8 @@ __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)
 
 // The fixed-order sum of the tile totals: block b sums its chunk of tiles
 // (each thread a fixed stride, then a tree in LDS) into rpart[b].
-__global__ void __launch_bounds__(256) k_rsum(const double *tile_r, int ntiles, RunState *st) {
-  __shared__ double s[256];
-  const int chunk = (ntiles + NPART - 1) / NPART;
-  const int lo = blockIdx.x * chunk, hi = min(lo + chunk, ntiles);
+__device__ inline void rsum_block(const double *tile_r, int ntiles, int chunk, RunState *st, int b, double *s) {
+  const int lo = b * chunk, hi = min(lo + chunk, ntiles);
   double acc = 0.0;
   for (int t = lo + (int)threadIdx.x; t < hi; t += 256) acc += tile_r[t];
   s[threadIdx.x] = acc;
@@ -927,17 +944,26 @@ __global__ void __launch_bounds__(256) k_rsum(const double *tile_r, int ntiles, 
     if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) st->rpart[blockIdx.x] = s[0];
+  if (threadIdx.x == 0) st->rpart[b] = s[0];
+}
+__global__ void __launch_bounds__(256) k_rsum(const double *tile_r, int ntiles, int chunk, RunState *st) {
+  __shared__ double s[256];
+  rsum_block(tile_r, ntiles, chunk, st, blockIdx.x, s);
 }
 
 // After the eight colour passes: the clock advances by the window just used
 // (operator splitting of the master equation over the colours), the next
 // window is events_per_pass / (largest tile rate), and the reference's
 // early stop applies when at most 20 MC particles remain (main.f:307-310).
-__global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, const int *list_err,
-                                                     unsigned int *sched) {
-  __shared__ double srsum[NSLOT];
-  __shared__ unsigned long long srmax[NSLOT], sn[NSLOT], stl[NSLOT], sev[NSLOT], scnt[8][NSLOT];
+struct SweepEndLds {
+  double srsum[NSLOT];
+  unsigned long long srmax[NSLOT], sn[NSLOT], stl[NSLOT], sev[NSLOT], scnt[8][NSLOT];
+};
+__device__ inline void sweep_end(RunState *st, int *win, const int *list_err, unsigned int *sched,
+                                 SweepEndLds &E) {
+  double *srsum = E.srsum;
+  unsigned long long *srmax = E.srmax, *sn = E.sn, *stl = E.stl, *sev = E.sev;
+  unsigned long long(*scnt)[NSLOT] = E.scnt;
   const int i = threadIdx.x;
   Slot &sl = st->slot[i];
   srsum[i] = st->rpart[i] + st->rpart[i + NSLOT];
@@ -1017,6 +1043,137 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
     st->final_sweep = 1;
   }
 }
+__global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, const int *list_err,
+                                                     unsigned int *sched) {
+  __shared__ SweepEndLds E;
+  sweep_end(st, win, list_err, sched, E);
+}
+
+// ---- the batched ensemble: one sweep over many lattices ------------------------
+// Every replica keeps its own lattice, lists, windows, run state and random
+// stream; a colour pass is one launch over that colour's tiles of all the
+// replicas still running.  The plan, remade after every sweep, lays the
+// colour's tiles of the replicas end to end:
+//   plan[c * (n + 1) + r]      c = 0..7, r = 0..n: tiles of colour c in the
+//                              windows of replicas 0..r-1 (replicas that have
+//                              stopped hold none)
+//   plan[8 * (n + 1) + c]      queue head of colour c, in units of 64
+//   plan[8 * (n + 1) + 8]      replicas still running
+//   plan[8 * (n + 1) + 9 + r]  replica r runs the coming sweep
+// Persistent waves take the entries of a colour's queue one at a time, so
+// the tiles of large and small replicas share the GPU and the slowest tiles
+// of a pass overlap other tiles' work instead of idling the rest.
+constexpr int ENS_MAX = 1024;
+static inline int64_t ens_plan_words(int n) { return 8 * (int64_t)(n + 1) + 9 + n; }
+
+__global__ void __launch_bounds__(ENS_MAX) k_ens_plan(const PassParams *__restrict__ reps, int n, int *plan) {
+  __shared__ int sc[ENS_MAX];
+  __shared__ int nrun;
+  const int r = threadIdx.x;
+  int cnt[8];
+  for (int c = 0; c < 8; c++) cnt[c] = 0;
+  int run = 0;
+  if (r == 0) nrun = 0;
+  if (r < n) {
+    const PassParams &P = reps[r];
+    const RunState *st = P.st;
+    // k_pass's condition for a replica's tiles to run
+    run = st->tau > 0.0 && !st->err && !st->stop;
+    if (run)
+      for (int c = 0; c < 8; c++) {
+        int zf;
+        cnt[c] = colour_tiles(P.L, P.ntz, c, zf);
+      }
+    plan[8 * (n + 1) + 9 + r] = run;
+  }
+  __syncthreads();
+  if (run) atomicAdd(&nrun, 1);
+  for (int c = 0; c < 8; c++) {  // inclusive scan over the replicas, in replica order
+    sc[r] = cnt[c];
+    __syncthreads();
+    for (int off = 1; off < ENS_MAX; off <<= 1) {
+      const int v = r >= off ? sc[r - off] : 0;
+      __syncthreads();
+      sc[r] += v;
+      __syncthreads();
+    }
+    if (r < n) plan[c * (n + 1) + r + 1] = sc[r];
+    if (r == 0) plan[c * (n + 1)] = 0;
+    __syncthreads();
+  }
+  if (r < 8) plan[8 * (n + 1) + r] = 0;
+  if (r == 0) plan[8 * (n + 1) + 8] = nrun;
+}
+
+// The replica whose tiles hold entry q of colour pc's queue: the last r with
+// pc[r] <= q (pc is the colour's prefix, pc[n] > q), 64 prefixes at a time.
+__device__ inline int ens_replica(const int *pc, int n, int q, int lane) {
+  const int j1 = 64 * lane;
+  const int v1 = pc[min(j1, n)];  // every lane loads (an index clamped to the prefix)
+  const int b = __popcll(__ballot(j1 <= n && v1 <= q)) - 1;  // pc[0] = 0 <= q: b >= 0
+  const int j2 = 64 * b + lane;
+  const int v2 = pc[min(j2, n)];
+  return 64 * b + __popcll(__ballot(j2 <= n && v2 <= q)) - 1;
+}
+
+template <bool LA, bool COV>
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_pass_ens(const PassParams *__restrict__ reps, int n, int *plan, int colour) {
+  extern __shared__ unsigned char smem[];
+  __shared__ PassLds S;
+  const int lane = threadIdx.x;
+  const int *pc = plan + colour * (n + 1);
+  const int total = __builtin_amdgcn_readfirstlane(pc[n]);
+  unsigned int *const head = (unsigned int *)(plan + 8 * (n + 1) + colour);
+  int r = 0, lo = 0, hi = 0;  // the last replica found and its entries [lo, hi)
+  // Every decision of this loop is wave-uniform (readfirstlane / ballot)
+  // and no memory access sits under a lane-dependent branch.
+  for (;;) {
+    // the next entry: every lane adds 1 (one atomic of 64 for the wave)
+    const unsigned int took = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int q = __builtin_amdgcn_readfirstlane((int)(took >> 6));
+    if (q >= total) break;
+    if (q < lo || q >= hi) {
+      r = __builtin_amdgcn_readfirstlane(ens_replica(pc, n, q, lane));
+      lo = __builtin_amdgcn_readfirstlane(pc[r]);
+      hi = __builtin_amdgcn_readfirstlane(pc[r + 1]);
+    }
+    const PassParams &P = reps[r];
+    Lattice L = P.L;
+    RunState *st = P.st;
+    // a replica that failed in this sweep runs no further tiles (k_pass
+    // checks at the start of each colour pass)
+    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+      continue;
+    const double tau = st->tau;
+    int zfirst;
+    colour_tiles(L, P.ntz, colour, zfirst);
+    const int t = colour_tile(L, colour, zfirst, q - lo);
+    int32_t *lk = (int32_t *)smem;
+    float *lr = (float *)(lk + P.cap);
+    uint8_t *lc = (uint8_t *)(lr + P.cap);
+    pass_tile<LA, COV>(P, L, st, tau, t, colour, S, lk, lr, lc, lane);
+  }
+}
+
+// Tile sums and the sweep's end for every replica that ran the sweep (the
+// same per-replica work as k_rsum and k_sweep_end, so the replica's window,
+// clock and stop are those it gets alone).
+__global__ void __launch_bounds__(256) k_rsum_ens(const PassParams *__restrict__ reps, int n, const int *plan) {
+  __shared__ double s[256];
+  const int r = blockIdx.y;
+  if (!plan[8 * (n + 1) + 9 + r]) return;
+  const PassParams &P = reps[r];
+  rsum_block(P.L.tile_r, P.L.ntx * P.L.nty * P.ntz, P.rsum_chunk, P.st, blockIdx.x, s);
+}
+__global__ void __launch_bounds__(NSLOT) k_sweep_end_ens(const PassParams *__restrict__ reps, int n,
+                                                         const int *plan) {
+  __shared__ SweepEndLds E;
+  const int r = blockIdx.x;
+  if (!plan[8 * (n + 1) + 9 + r]) return;
+  const PassParams &P = reps[r];
+  sweep_end(P.st, P.L.win, P.L.list_err, P.sched, E);
+}
 
 // ---- device-side initial state ----------------------------------------------
 // Diamond sites of the reference grid: x, y, z multiples of 3 whose thirds
@@ -1039,7 +1196,7 @@ __constant__ int c_jump_dia[8][3] = {{3, 3, 3}, {3, -3, -3}, {-3, -3, 3}, {-3, 3
 // particles with PTransE(s,2,0,...), the empty sites above with
 // coordination 2 and their deposition rate.  Every point is written.
 __global__ void __launch_bounds__(256) k_build_slab(Lattice L, int len, int sic, double fraction, uint32_t key0,
-                                                   uint32_t key1) {
+                                                   uint32_t key1, int top_lz) {
   const Tables &t = *L.tab;
   const int64_t n = L.box.npts;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
@@ -1050,7 +1207,8 @@ __global__ void __launch_bounds__(256) k_build_slab(Lattice L, int len, int sic,
     s.kc = 0; s.valid = 0; s.temp = 0;
     int cls = CLS_NONE;
     double r = 0.0;
-    const bool wall = sic != 1 && ((z >= 0 && z <= 5) || (z >= L.box.lz - 6 && z <= L.box.lz - 1));
+    // top_lz: LenZ of the whole box (the top wall planes lie above the stored ones when only the bottom planes are stored)
+    const bool wall = sic != 1 && ((z >= 0 && z <= 5) || (z >= top_lz - 6 && z <= top_lz - 1));
     const int ty = dia_type(x, y, z);
     if (wall) {
       s.kc = KIND_WALL;
@@ -1915,7 +2073,9 @@ F pick_variant(bool la, bool cov, F tt, F tf, F ft, F ff) {
 struct mskg_engine {
   mskg_config cfg;
   bool cov_model = true;  // the kernels with the coverage-species branches
-  Box box;
+  Box box;               // the stored planes (all of them unless cfg.stored_len_z)
+  int logical_lz = 0;    // the reference's LenZ
+  bool windowed = false; // only the bottom planes are stored
   Geometry geo;
   Tables tab;
   int ncrystal, ncov, list_cov[3];
@@ -1954,6 +2114,7 @@ struct mskg_engine {
   int *d_err = nullptr;
   int device = 0;          // the HIP device the engine lives on
   int tile = 0, ntx = 0, nty = 0, ntz = 0, ntiles = 0, cap = 1024, capg = 512;
+  int rsum_chunk = 1;      // tiles per block of the fixed-order tile sum
   bool lists_dirty = true;
   int pass_grid = 4096;  // workgroups per colour pass (they stride over the window)
   hipGraph_t graph = nullptr;
@@ -2021,6 +2182,7 @@ static Lattice make_lattice(mskg_engine *e) {
   L.nty = e->nty;
   L.key = e->track ? e->d_key : nullptr;
   L.cl = e->tab.la_variant ? (const long long *)((const char *)e->d_st + offsetof(RunState, cl)) : nullptr;
+  L.ztop = e->windowed ? e->box.lz - TOP_MARGIN : 0x7fffffff;
   return L;
 }
 
@@ -2066,6 +2228,8 @@ static const char *err_name(int c) {
     case ERR_LIQUID: return "a species' CountLiquid is negative: its solidification rate would be negative";
     case ERR_ST_RANGE: return "a site's temperature lies outside the InitSt 'ST' reference temperatures";
     case ERR_SCHED: return "a tile waited more than 2 s for its neighbour tiles in a dependency-scheduled sweep";
+    case ERR_TOP:
+      return "an event site within 24 planes of the top of the stored planes (stored_len_z): store more planes";
     default: return "unknown";
   }
 }
@@ -2283,12 +2447,18 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
     return fail_(e, MSKG_EINVAL, "box %d x %d x %d: len_x and len_y must be positive multiples of 12", cfg->len_x,
                  cfg->len_y, cfg->len_z);
   if (cfg->len_z > 8190) return fail_(e, MSKG_EINVAL, "len_z must be <= 8190");
+  if (cfg->stored_len_z < 0 || cfg->stored_len_z > cfg->len_z + 1 ||
+      (cfg->stored_len_z > 0 && cfg->stored_len_z < 2 * TOP_MARGIN))
+    return fail_(e, MSKG_EINVAL, "stored_len_z %d: 0 (all planes) or %d..%d", cfg->stored_len_z, 2 * TOP_MARGIN,
+                 cfg->len_z + 1);
   if (cfg->n_crystal < 1 || cfg->n_crystal > 3 || cfg->n_cov < 0 || cfg->n_cov > 3)
     return fail_(e, MSKG_EINVAL, "n_crystal must be 1..3 and n_cov 0..3");
   if (!cfg->ptrans_e || !cfg->ptrans_d) return fail_(e, MSKG_EINVAL, "ptrans_e and ptrans_d are required");
   if (cfg->n_cov > 0 && (!cfg->ptrans_abs || !cfg->ptrans_des))
     return fail_(e, MSKG_EINVAL, "ptrans_abs and ptrans_des are required when n_cov > 0");
-  e->box = make_box(cfg->len_x, cfg->len_y, cfg->len_z);
+  e->logical_lz = cfg->len_z;
+  e->windowed = cfg->stored_len_z > 0 && cfg->stored_len_z < cfg->len_z + 1;
+  e->box = make_box(cfg->len_x, cfg->len_y, e->windowed ? cfg->stored_len_z - 1 : cfg->len_z);
   e->ncrystal = cfg->n_crystal;
   e->ncov = cfg->n_cov;
   for (int i = 0; i < 3; i++) e->list_cov[i] = cfg->list_cov[i];
@@ -2331,8 +2501,12 @@ int mskg_create(const mskg_config *cfg, mskg_engine **out) {
   e->capg = e->cap;
   e->ntx = cfg->len_x / e->tile;
   e->nty = cfg->len_y / e->tile;
-  e->ntz = (cfg->len_z + 1 + e->tile - 1) / e->tile;
+  e->ntz = (e->box.lz + 1 + e->tile - 1) / e->tile;
   e->ntiles = e->ntx * e->nty * e->ntz;
+  // the tile sums are taken in chunks of the full box's tiles, so that a
+  // box storing only its bottom planes sums the same values in the same
+  // order (the tiles it does not store hold no rates)
+  e->rsum_chunk = (e->ntx * e->nty * ((cfg->len_z + 1 + e->tile - 1) / e->tile) + NPART - 1) / NPART;
 
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail_(e, MSKG_EDEVICE, "no HIP device available");
@@ -2424,6 +2598,15 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt, int32_t n_ad, const int
         s.kc = (uint8_t)(kind | (((v >> 24) & 7) << 3));
         s.temp = (uint16_t)((v >> 9) & 0x1FFF);
       }
+  // planes above the stored ones: vacuum or walls only, which no event reaches
+  for (int z = b.lz + 1; z <= e->logical_lz; z++)
+    for (int y = 0; y < b.ly; y++)
+      for (int x = 0; x < b.lx; x++) {
+        const int32_t v = latt[(int64_t)x + (int64_t)b.lx * ((int64_t)y + (int64_t)b.ly * z)];
+        if (v != 0 && (v & 255) != 113)
+          return fail_(e, MSKG_ESTATE, "LattCoo(%d,%d,%d)=0x%08x lies above the %d stored planes", x, y, z, v,
+                       b.lz + 1);
+      }
   // walls the caller's LattCoo holds off the lattice (index 113)
   std::vector<uint32_t> offwall;
   for (int z = 0; z <= b.lz; z++)
@@ -2450,6 +2633,9 @@ int mskg_load_state(mskg_engine *e, const int32_t *latt, int32_t n_ad, const int
       if (p < 0)
         return fail_(e, MSKG_ESTATE, "list %d record %d: site (%d,%d,%d) is not a lattice point of the box", which, i,
                      s[0], s[1], s[2]);
+      if (which == 0 && e->windowed && s[2] > b.lz - TOP_MARGIN)
+        return fail_(e, MSKG_ESTATE, "MC record %d at (%d,%d,%d) lies within %d planes of the stored top (%d)", i,
+                     s[0], s[1], s[2], TOP_MARGIN, b.lz);
       NL l;
       for (int k = 0; k < 4; k++) {
         l.x[k] = nn[12 * i + 3 * k];
@@ -2538,15 +2724,25 @@ static int set_temperature_map(mskg_engine *e, const int32_t *map, int layout, i
   return MSKG_OK;
 }
 
+// The laser-annealing maps and forms cover the whole grid: they need every plane stored.
+static int whole_grid(mskg_engine *e, const char *what) {
+  if (e->windowed)
+    return fail_(e, MSKG_EINVAL, "%s needs every grid plane stored (stored_len_z %d of %d)", what, e->box.lz + 1,
+                 e->logical_lz + 1);
+  return MSKG_OK;
+}
+
 int mskg_set_temperature(mskg_engine *e, const int32_t *temp_k) {
   if (!e || !temp_k) return MSKG_EINVAL;
   if (!e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  if (int rc = whole_grid(e, "the temperature map")) return rc;
   return set_temperature_map(e, temp_k, FIELD_LATTCOO, 0);
 }
 
 int mskg_set_temperature_field(mskg_engine *e, const int32_t *field, int32_t layout, int32_t on_device) {
   if (!e || !field) return MSKG_EINVAL;
   if (!e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  if (int rc = whole_grid(e, "the temperature map")) return rc;
   if (layout != MSKG_FIELD_SOCKET && layout != MSKG_FIELD_PLANES)
     return fail_(e, MSKG_EINVAL, "field layout %d: 0 (socket) or 1 (planes)", layout);
   return set_temperature_map(e, field, layout, on_device);
@@ -2555,6 +2751,7 @@ int mskg_set_temperature_field(mskg_engine *e, const int32_t *field, int32_t lay
 int mskg_phases(mskg_engine *e, int32_t *coor, int32_t *species, int32_t layout, int32_t on_device) {
   if (!e) return MSKG_EINVAL;
   if (!e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
+  if (int rc = whole_grid(e, "the phase maps")) return rc;
   if (layout != MSKG_FIELD_SOCKET && layout != MSKG_FIELD_PLANES)
     return fail_(e, MSKG_EINVAL, "phase map layout %d: 0 (socket) or 1 (planes)", layout);
   if (!coor && !species) return MSKG_OK;
@@ -2591,6 +2788,7 @@ int mskg_set_la_variant(mskg_engine *e, const double p0[3], const double tm[3], 
   if (!e || !p0 || !tm || !sigma || !tflex || !init_count || !count) return MSKG_EINVAL;
   if (!e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
   if (!e->tab.la_mode) return fail_(e, MSKG_EINVAL, "the laser-annealing build's form needs la_mode");
+  if (int rc = whole_grid(e, "the laser-annealing build's form")) return rc;
   for (int k = 0; k < 3; k++)
     if (k < e->ncrystal && !(sigma[k] != 0.0))
       return fail_(e, MSKG_EINVAL, "sigma of species %d is zero", k + 1);
@@ -2774,16 +2972,17 @@ int mskg_replay(mskg_engine *e, const mskg_event *log, int64_t n, int32_t clock_
   return MSKG_OK;
 }
 
-int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *events_done, double *kmc_time,
-             int32_t *stop_reason) {
+// The start of a synchronous-sublattice run, on the engine's stream: tile
+// lists current, the run's limits in the run state, the tile totals of the
+// current rates and the first window (k_sweep_end with tau = 0).  st: the
+// run state before it.
+static int run_begin(mskg_engine *e, int64_t max_events, double max_time, RunState &st) {
   if (!e || !e->loaded) return fail_(e, MSKG_EINVAL, "no state loaded");
   if (max_events < 0) return fail_(e, MSKG_EINVAL, "max_events < 0");
   Lattice L = make_lattice(e);
-  RunState st;
   int rc = read_state(e, st);
   if (rc) return rc;
   if (st.err) return fail_(e, MSKG_ESTATE, "engine is in an error state: %s", err_name(st.err));
-  const unsigned long long ev_start = st.events;
   const int tblocks = (e->ntiles + 3) / 4;
   {
     static int w0[32];
@@ -2817,26 +3016,43 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
   // tile totals of the current rates, then the first window
   hipLaunchKernelGGL(k_tile_rates, dim3(tblocks), dim3(256), 0, e->stream, L, e->ntiles, e->d_st);
   HIPCHK(e, hipGetLastError());
-  hipLaunchKernelGGL(k_rsum, dim3(NPART), dim3(256), 0, e->stream, e->d_tile_r, e->ntiles, e->d_st);
+  hipLaunchKernelGGL(k_rsum, dim3(NPART), dim3(256), 0, e->stream, e->d_tile_r, e->ntiles, e->rsum_chunk,
+                       e->d_st);
   hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err,
                        e->d_done + e->ntiles);
   HIPCHK(e, hipGetLastError());
+  return MSKG_OK;
+}
+
+// The pass kernels' parameters of an engine (colour 0).
+static PassParams pass_params(mskg_engine *e) {
+  PassParams P;
+  P.L = make_lattice(e);
+  P.st = e->d_st;
+  P.log = e->d_log;
+  P.log_cap = (unsigned long long)e->log_cap;
+  P.ntz = e->ntz;
+  P.cap = e->cap;
+  P.key0 = (uint32_t)e->cfg.seed;
+  P.key1 = (uint32_t)(e->cfg.seed >> 32);
+  P.colour = 0;
+  P.rsum_chunk = e->rsum_chunk;
+  P.done = e->d_done;
+  P.sched = e->d_done + e->ntiles;
+  return P;
+}
+
+int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *events_done, double *kmc_time,
+             int32_t *stop_reason) {
+  RunState st;
+  int rc = run_begin(e, max_events, max_time, st);
+  if (rc) return rc;
+  const unsigned long long ev_start = st.events;
   const size_t shm = (size_t)e->cap * (4 + 4 + 1) + 16;
   // one sweep = eight colour passes (or one dependency-scheduled launch) +
   // the window/clock update, captured once as a graph and replayed
   if (!e->graph_exec) {
-    PassParams P;
-    P.L = L;
-    P.st = e->d_st;
-    P.log = e->d_log;
-    P.log_cap = (unsigned long long)e->log_cap;
-    P.ntz = e->ntz;
-    P.cap = e->cap;
-    P.key0 = (uint32_t)e->cfg.seed;
-    P.key1 = (uint32_t)(e->cfg.seed >> 32);
-    P.colour = 0;
-    P.done = e->d_done;
-    P.sched = e->d_done + e->ntiles;
+    PassParams P = pass_params(e);
     const bool la = e->tab.la_mode || e->tab.st_n;
     int grid = std::max(1, std::min(e->ntiles / 8 + 1, e->pass_grid));
     if (e->schedule == MSKG_SCHEDULE_TILES) {
@@ -2857,7 +3073,8 @@ int mskg_run(mskg_engine *e, int64_t max_events, double max_time, int64_t *event
         hipLaunchKernelGGL(MSKG_VARIANT(k_pass, la, e->cov_model), dim3(grid), dim3(WAVE), shm, e->stream, P);
       }
     }
-    hipLaunchKernelGGL(k_rsum, dim3(NPART), dim3(256), 0, e->stream, e->d_tile_r, e->ntiles, e->d_st);
+    hipLaunchKernelGGL(k_rsum, dim3(NPART), dim3(256), 0, e->stream, e->d_tile_r, e->ntiles, e->rsum_chunk,
+                       e->d_st);
     hipLaunchKernelGGL(k_sweep_end, dim3(1), dim3(NSLOT), 0, e->stream, e->d_st, e->d_win, e->d_list_err,
                        e->d_done + e->ntiles);
     HIPCHK(e, hipStreamEndCapture(e->stream, &e->graph));
@@ -3097,12 +3314,15 @@ int mskg_build_alloy_slab(mskg_engine *e, int32_t len, double fraction) {
 }
 
 static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction) {
-  if (len < 6 || len > e->box.lz - (mode != 1 ? 7 : 1) || len % 3)
-    return fail_(e, MSKG_EINVAL, "slab height %d does not fit the box height %d", len, e->box.lz);
+  if (len < 6 || len > e->logical_lz - (mode != 1 ? 7 : 1) || len % 3)
+    return fail_(e, MSKG_EINVAL, "slab height %d does not fit the box height %d", len, e->logical_lz);
+  if (e->windowed && len + 6 > e->box.lz - TOP_MARGIN)
+    return fail_(e, MSKG_EINVAL, "slab height %d: the stored planes (stored_len_z %d) need %d more above it", len,
+                 e->box.lz + 1, TOP_MARGIN + 6);
   Lattice L = make_lattice(e);
   int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
   hipLaunchKernelGGL(k_build_slab, dim3(blocks), dim3(256), 0, e->stream, L, len, mode, fraction,
-                     (uint32_t)e->cfg.seed, (uint32_t)(e->cfg.seed >> 32));
+                     (uint32_t)e->cfg.seed, (uint32_t)(e->cfg.seed >> 32), e->logical_lz);
   HIPCHK(e, hipGetLastError());
   hipFree(e->d_offwall);  // the SetSi builders fill their wall planes at every grid point
   e->d_offwall = nullptr;
@@ -3157,6 +3377,7 @@ int mskg_copy_state(mskg_engine *dst, const mskg_engine *src) {
   if (!dst || !src || !src->loaded) return MSKG_EINVAL;
   if (dst == src) return MSKG_OK;
   if (dst->box.lx != src->box.lx || dst->box.ly != src->box.ly || dst->box.lz != src->box.lz ||
+      dst->logical_lz != src->logical_lz ||
       dst->tile != src->tile)
     return fail_(dst, MSKG_EINVAL, "copy_state: engines differ in box or tile");
   const int64_t np = dst->box.npts;
@@ -3349,6 +3570,119 @@ int mskg_run_serial(mskg_engine *const *engines, int32_t n, const int64_t *max_e
     if (e->d_log && st.log_n > (unsigned long long)e->log_cap && rc_all == MSKG_OK)
       rc_all = fail_(e, MSKG_ECAPACITY, "event log of replica %d overflowed", i);
     (void)log0;
+  }
+  return rc_all;
+}
+
+int mskg_run_ensemble(mskg_engine *const *engines, int32_t n, const int64_t *max_events, const double *max_time,
+                      int64_t *events_done, double *kmc_time, int32_t *stop_reason) {
+  if (!engines || n <= 0 || !max_events || !max_time) return MSKG_EINVAL;
+  mskg_engine *e0 = engines[0];
+  if (!e0) return MSKG_EINVAL;
+  if (n > ENS_MAX) return fail_(e0, MSKG_EINVAL, "ensemble of %d replicas: at most %d", n, ENS_MAX);
+  const bool la = e0->tab.la_mode || e0->tab.st_n;
+  const bool cov = e0->cov_model;
+  int cap = 0;
+  int64_t queue_bound = 0;
+  for (int i = 0; i < n; i++) {
+    mskg_engine *e = engines[i];
+    if (!e) return fail_(e0, MSKG_EINVAL, "ensemble: replica %d is null", i);
+    for (int j = 0; j < i; j++)
+      if (engines[j] == e) return fail_(e0, MSKG_EINVAL, "ensemble: replica %d appears twice", i);
+    if (e->device != e0->device) return fail_(e0, MSKG_EINVAL, "ensemble: replica %d lives on another device", i);
+    if ((bool)(e->tab.la_mode || e->tab.st_n) != la || e->cov_model != cov)
+      return fail_(e0, MSKG_EINVAL, "ensemble: replica %d needs another rate-model kernel than replica 0", i);
+    cap = std::max(cap, e->cap);
+    queue_bound += e->ntiles / 8 + 1;
+  }
+  // a colour's queue head counts entries in units of 64 in 32 bits
+  if (queue_bound > (1ll << 25)) return fail_(e0, MSKG_EINVAL, "ensemble: too many tiles per colour pass");
+  std::vector<PassParams> reps(n);
+  std::vector<unsigned long long> ev0(n);
+  for (int i = 0; i < n; i++) {
+    mskg_engine *e = engines[i];
+    RunState st;
+    if (int rc = run_begin(e, max_events[i], max_time[i], st)) {
+      if (e != e0) fail_(e0, rc, "ensemble replica %d: %s", i, e->err.c_str());
+      return rc;
+    }
+    ev0[i] = st.events;
+    reps[i] = pass_params(e);
+  }
+  for (int i = 0; i < n; i++) HIPCHK(engines[i], hipStreamSynchronize(engines[i]->stream));
+  hipStream_t sm = e0->stream;
+  PassParams *d_reps = nullptr;
+  int *d_plan = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const int64_t pw = ens_plan_words(n);
+  const size_t shm = (size_t)cap * (4 + 4 + 1) + 16;
+  int nrun = 0;
+  int64_t sweeps = 0;
+  float ms = 0;
+  auto kpass = MSKG_VARIANT(k_pass_ens, la, cov);
+  hipError_t ce = hipMalloc(&d_reps, n * sizeof(PassParams));
+  if (ce == hipSuccess) ce = hipMalloc(&d_plan, pw * sizeof(int));
+  if (ce == hipSuccess) ce = hipMemcpyAsync(d_reps, reps.data(), n * sizeof(PassParams), hipMemcpyHostToDevice, sm);
+  int grid = 0;
+  if (ce == hipSuccess) {
+    // persistent waves: as many as the device holds at once
+    int per_cu = 0, cus = 0;
+    ce = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kpass), WAVE, shm);
+    if (ce == hipSuccess) ce = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e0->device);
+    grid = std::max(1, per_cu * cus);
+  }
+  if (ce == hipSuccess) {
+    hipLaunchKernelGGL(k_ens_plan, dim3(1), dim3(ENS_MAX), 0, sm, d_reps, n, d_plan);
+    ce = hipGetLastError();
+  }
+  // one sweep of every running replica: the eight colour passes, the tile
+  // sums and sweep ends, the next sweep's plan
+  if (ce == hipSuccess) ce = hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal);
+  if (ce == hipSuccess) {
+    for (int c = 0; c < 8; c++) hipLaunchKernelGGL(kpass, dim3(grid), dim3(WAVE), shm, sm, d_reps, n, d_plan, c);
+    hipLaunchKernelGGL(k_rsum_ens, dim3(NPART, n), dim3(256), 0, sm, d_reps, n, d_plan);
+    hipLaunchKernelGGL(k_sweep_end_ens, dim3(n), dim3(NSLOT), 0, sm, d_reps, n, d_plan);
+    hipLaunchKernelGGL(k_ens_plan, dim3(1), dim3(ENS_MAX), 0, sm, d_reps, n, d_plan);
+    ce = hipStreamEndCapture(sm, &graph);
+  }
+  if (ce == hipSuccess) ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ce == hipSuccess) ce = hipEventRecord(e0->ev0, sm);
+  int batch = 1;
+  while (ce == hipSuccess) {
+    for (int b = 0; b < batch && ce == hipSuccess; b++) ce = hipGraphLaunch(exec, sm);
+    if (ce != hipSuccess) break;
+    sweeps += batch;
+    ce = hipMemcpyAsync(&nrun, d_plan + 8 * (n + 1) + 8, sizeof(int), hipMemcpyDeviceToHost, sm);
+    if (ce == hipSuccess) ce = hipStreamSynchronize(sm);
+    if (ce != hipSuccess || nrun == 0) break;
+    // a sweep queued after every replica has stopped still costs its
+    // launches: batches grow with the sweeps run, up to 32
+    batch = (int)std::max<int64_t>(1, std::min<int64_t>(sweeps / 4, 32));
+  }
+  if (ce == hipSuccess) ce = hipEventRecord(e0->ev1, sm);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(sm);
+  if (ce == hipSuccess) ce = hipEventElapsedTime(&ms, e0->ev0, e0->ev1);
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  hipFree(d_reps);
+  hipFree(d_plan);
+  if (ce != hipSuccess) return fail_(e0, MSKG_EDEVICE, "ensemble run: %s", hipGetErrorString(ce));
+  int rc_all = MSKG_OK;
+  for (int i = 0; i < n; i++) {
+    mskg_engine *e = engines[i];
+    e->run_ms = ms;
+    e->sweeps_done = sweeps;
+    RunState st;
+    if (int rc = read_state(e, st)) return rc;
+    if (events_done) events_done[i] = (int64_t)(st.events - ev0[i]);
+    if (kmc_time) kmc_time[i] = st.clock;
+    if (stop_reason) stop_reason[i] = st.stop == STOP_AT_LIMIT || !st.stop ? MSKG_STOP_LIMIT : st.stop;
+    if (st.err && rc_all == MSKG_OK) {
+      rc_all = fail_(e, MSKG_ESTATE, "run stopped in the tile at (%d,%d,%d): %s", st.err_site[0], st.err_site[1],
+                     st.err_site[2], err_name(st.err));
+      if (e != e0) fail_(e0, rc_all, "ensemble replica %d: %s", i, e->err.c_str());
+    }
   }
   return rc_all;
 }

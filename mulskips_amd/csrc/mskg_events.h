@@ -26,7 +26,8 @@ enum : int {
   ERR_SUMS = 8,        // serial selection: the rate sums disagree with the lists
   ERR_LIQUID = 9,      // negative CountLiquid (laser-annealing build's form)
   ERR_ST_RANGE = 10,   // a temperature outside the interpolation tables (InitSt 'ST')
-  ERR_SCHED = 11       // a tile waited too long for its neighbour tiles (dependency-scheduled sweep)
+  ERR_SCHED = 11,      // a tile waited too long for its neighbour tiles (dependency-scheduled sweep)
+  ERR_TOP = 12         // an event within 24 planes of the top of the stored planes (stored_len_z)
 };
 
 // Per-tile candidate lists (synchronous-sublattice mode): every MC particle
@@ -52,6 +53,9 @@ struct Lattice {
   float inv_tile;         // 1 / tile
   int32_t *key;           // may be null: rate-class key of every point (likelihood accounting)
   const long long *cl;    // may be null: CountLiquid(1..3) the laser-annealing build's rates read
+  int ztop;               // highest z of an event's site: an event reads at most 12 and writes at
+                          // most 6 planes from its site, so with only the bottom planes stored an
+                          // event above this would reach past them (INT_MAX: every plane stored)
 };
 
 // Rate-class keys for the trajectory likelihood (models without coverage
@@ -923,6 +927,10 @@ __device__ void execute(const Lattice &L, double rtot, const P3 &s, int itrans, 
   res.err = ERR_NONE;
   res.arm = -1;
   res.species = 0;
+  if (s.z > L.ztop) {  // before anything changes: the run stops with the state as it was
+    res.err = ERR_TOP;
+    return;
+  }
   const bool dep = itrans <= 3, evap = itrans == 4, absorb = itrans >= 6;
   if (lane < 12) {
     sc.sec[lane].i = -1;

@@ -50,13 +50,20 @@ class Engine:
     ptrans_abs (12), ptrans_des (12), ptrans_zig, n_crystal, n_cov,
     list_crystal, list_cov -- the quantities SetProbability.f reads from
     start.dat.
+
+    stored_planes: hold only the grid planes z < stored_planes in device
+    memory (mskg_config.stored_len_z; None: all LenZ+1).  The planes above
+    are vacuum no event reaches; a film that grows to within 24 planes of
+    the stored top stops the run with an error.  store_state() then returns
+    the stored planes of LattCoo.
     """
 
     def __init__(self, box, tables, seed=1, tile=0, tile_capacity=0, events_per_pass=0.0, tau_policy=0,
-                 frozen_below_z=0, la=None):
+                 frozen_below_z=0, la=None, stored_planes=None):
         self.L = _lib.lib()
         lx, ly, lz = box
         self.box = (int(lx), int(ly), int(lz))
+        self.stored_planes = int(lz) + 1 if stored_planes is None else min(int(stored_planes), int(lz) + 1)
         self._arrays = {k: np.ascontiguousarray(tables[k], np.float64)
                         for k in ("ptrans_e", "ptrans_d", "ptrans_abs", "ptrans_des")}
         cfg = Config()
@@ -80,6 +87,7 @@ class Engine:
         cfg.events_per_pass = float(events_per_pass)
         cfg.tau_policy = int(tau_policy)
         cfg.frozen_below_z = int(frozen_below_z)
+        cfg.stored_len_z = 0 if stored_planes is None else int(stored_planes)
         if tables.get("second_factor") is not None:
             self._arrays["second_factor"] = np.ascontiguousarray(tables["second_factor"], np.float64)
             if self._arrays["second_factor"].shape != (13,):
@@ -327,8 +335,14 @@ class Engine:
         self._check(self.L.mskg_replay(self.h, ptr(log), len(log), int(clock_mode)))
 
     def store_state(self, lattcoo=None, capacity=None):
+        """(LattCoo, MC list, bulk list, void list).  LattCoo is (LenZ+1,
+        LenY, LenX), or its stored planes for an engine holding only the
+        bottom ones; the lattice points are written, other grid points (and
+        planes above the stored ones) keep what lattcoo held."""
         lx, ly, lz = self.box
-        lat = np.zeros((lz + 1, ly, lx), np.int32) if lattcoo is None else np.ascontiguousarray(lattcoo, np.int32)
+        lat = (np.zeros((self.stored_planes, ly, lx), np.int32) if lattcoo is None
+               else np.ascontiguousarray(lattcoo, np.int32))
+        assert lat.shape[0] >= self.stored_planes and lat.shape[1:] == (ly, lx), lat.shape
         na, nb, nv = C.c_int32(), C.c_int32(), C.c_int32()
         z = None
         self.L.mskg_store_state(self.h, None, 0, C.byref(na), z, z, z, z, 0, C.byref(nb), z, z, 0, C.byref(nv), z, z)
@@ -412,6 +426,25 @@ def run_serial(engines, max_events, max_time=float("inf")):
         bad = [e for e in engines if L.mskg_last_error(e.h)]
         msg = "; ".join(L.mskg_last_error(e.h).decode() for e in bad) or "mskg_run_serial failed"
         raise EngineError(rc, msg)
+    return done, clock, stop
+
+
+def run_ensemble(engines, max_events, max_time=float("inf")):
+    """The batched ensemble (mskg_run_ensemble): the synchronous-sublattice
+    run of every engine at once, each colour pass one launch over the tiles
+    of all running replicas.  Each engine ends as mskg_run alone would leave
+    it.  Returns per-engine (events, clock, stop reason) arrays."""
+    L = _lib.lib()
+    n = len(engines)
+    me = np.ascontiguousarray(np.broadcast_to(np.asarray(max_events, np.int64), (n,)))
+    mt = np.ascontiguousarray(np.broadcast_to(np.asarray(max_time, np.float64), (n,)))
+    mt = np.where(np.isfinite(mt), mt, 1e300)
+    done = np.zeros(n, np.int64)
+    clock = np.zeros(n, np.float64)
+    stop = np.zeros(n, np.int32)
+    rc = L.mskg_run_ensemble(_engine_array(engines), n, ptr(me), ptr(mt), ptr(done), ptr(clock), ptr(stop))
+    if rc:
+        raise EngineError(rc, L.mskg_last_error(engines[0].h).decode() or "mskg_run_ensemble failed")
     return done, clock, stop
 
 

@@ -38,7 +38,8 @@ module mskg_bindings
     real(c_double) :: events_per_pass
     integer(c_int32_t) :: tau_policy, frozen_below_z
     type(c_ptr) :: second_factor = c_null_ptr
-    integer(c_int32_t) :: reserved(4)
+    integer(c_int32_t) :: stored_len_z = 0   ! planes held on the device (0: all)
+    integer(c_int32_t) :: reserved(3)
   end type mskg_config
 
   ! one logged event (48 bytes)
@@ -211,6 +212,19 @@ module mskg_bindings
     ! serial BKL on n engines in one launch: engines, max_events, max_time are arrays of n
     integer(c_int) function mskg_run_serial(engines, n, max_events, max_time, events_done, kmc_time, stop_reason) &
         bind(c, name="mskg_run_serial")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), intent(in) :: engines(*)
+      integer(c_int32_t), value :: n
+      integer(c_int64_t), intent(in) :: max_events(*)
+      real(c_double), intent(in) :: max_time(*)
+      integer(c_int64_t), intent(out) :: events_done(*)
+      real(c_double), intent(out) :: kmc_time(*)
+      integer(c_int32_t), intent(out) :: stop_reason(*)
+    end function
+
+    ! the batched synchronous-sublattice ensemble (same arguments as mskg_run_serial)
+    integer(c_int) function mskg_run_ensemble(engines, n, max_events, max_time, events_done, kmc_time, &
+        stop_reason) bind(c, name="mskg_run_ensemble")
       import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
       type(c_ptr), intent(in) :: engines(*)
       integer(c_int32_t), value :: n

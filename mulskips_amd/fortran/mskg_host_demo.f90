@@ -63,7 +63,7 @@ program mskg_host_demo
   cfg%ptrans_zig = zig
   cfg%la_mode = 0; cfg%tm = 0; cfg%p0 = 0; cfg%sigma = 0; cfg%tflex = 0
   cfg%seed = 2026; cfg%tile = 24; cfg%tile_capacity = 0; cfg%events_per_pass = 1.0d0; cfg%tau_policy = 1
-  cfg%frozen_below_z = 0; cfg%second_factor = c_null_ptr; cfg%reserved = 0
+  cfg%frozen_below_z = 0; cfg%second_factor = c_null_ptr; cfg%stored_len_z = 0; cfg%reserved = 0
   call check(mskg_create(cfg, eng), 'mskg_create')
   call check(mskg_load_state(eng, c_loc(latt), n_ad, c_loc(ad_xyz), c_loc(ad_nn), c_loc(ad_ev), c_loc(ad_prob), &
                              n_bulk, c_loc(bulk_xyz), c_loc(bulk_nn), n_void, c_loc(void_xyz), c_loc(void_nn), &
