@@ -1105,6 +1105,27 @@ __global__ void __launch_bounds__(ENS_MAX) k_ens_plan(const PassParams *__restri
   if (r == 0) plan[8 * (n + 1) + 8] = nrun;
 }
 
+// A pointer read from device memory is a generic (flat) pointer to the
+// compiler, and every access through it a flat instruction, which waits on
+// both the vector-memory and the LDS counters and takes 64-bit addresses in
+// vector registers.  Every pointer of a replica's parameters is to global
+// memory: a round trip through the global address space states that, and
+// the compiler then uses global instructions, as it does for the same
+// pointers passed as kernel arguments.
+template <typename T>
+__device__ inline T *as_global(T *p) {
+  // through an integer: a direct cast there and back folds away
+  __attribute__((address_space(1))) T *g = (__attribute__((address_space(1))) T *)(uintptr_t)p;
+  return (T *)g;
+}
+__device__ inline void to_global(Lattice &L) {
+  L.site = as_global(L.site); L.rate = as_global(L.rate); L.cls = as_global(L.cls);
+  L.inlist = as_global(L.inlist); L.tl_idx = as_global(L.tl_idx); L.tl_n = as_global(L.tl_n);
+  L.tl_own = as_global(L.tl_own); L.tile_r = as_global(L.tile_r); L.list_err = as_global(L.list_err);
+  L.win = as_global(L.win); L.geo = as_global(L.geo); L.tab = as_global(L.tab); L.key = as_global(L.key);
+  L.cl = as_global(L.cl);
+}
+
 // The replica whose tiles hold entry q of colour pc's queue: the last r with
 // pc[r] <= q (pc is the colour's prefix, pc[n] > q), 64 prefixes at a time.
 __device__ inline int ens_replica(const int *pc, int n, int q, int lane) {
@@ -1127,32 +1148,42 @@ k_pass_ens(const PassParams *__restrict__ reps, int n, int *plan, int colour) {
   unsigned int *const head = (unsigned int *)(plan + 8 * (n + 1) + colour);
   int r = 0, lo = 0, hi = 0;  // the last replica found and its entries [lo, hi)
   // Every decision of this loop is wave-uniform (readfirstlane / ballot)
-  // and no memory access sits under a lane-dependent branch.
+  // and no memory access sits under a lane-dependent branch.  A wave's
+  // first entry is its own (blockIdx.x), so that the waves do not all meet
+  // at the queue head when the pass starts (same-address atomics are
+  // served one after another), and an empty pass takes no atomic at all;
+  // the entries from gridDim.x on are taken from the queue.
+  int q = blockIdx.x;
   for (;;) {
-    // the next entry: every lane adds 1 (one atomic of 64 for the wave)
-    const unsigned int took = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int q = __builtin_amdgcn_readfirstlane((int)(took >> 6));
     if (q >= total) break;
     if (q < lo || q >= hi) {
       r = __builtin_amdgcn_readfirstlane(ens_replica(pc, n, q, lane));
       lo = __builtin_amdgcn_readfirstlane(pc[r]);
       hi = __builtin_amdgcn_readfirstlane(pc[r + 1]);
     }
-    const PassParams &P = reps[r];
-    Lattice L = P.L;
-    RunState *st = P.st;
-    // a replica that failed in this sweep runs no further tiles (k_pass
-    // checks at the start of each colour pass)
-    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-      continue;
-    const double tau = st->tau;
-    int zfirst;
-    colour_tiles(L, P.ntz, colour, zfirst);
-    const int t = colour_tile(L, colour, zfirst, q - lo);
-    int32_t *lk = (int32_t *)smem;
-    float *lr = (float *)(lk + P.cap);
-    uint8_t *lc = (uint8_t *)(lr + P.cap);
-    pass_tile<LA, COV>(P, L, st, tau, t, colour, S, lk, lr, lc, lane);
+    {
+      const PassParams &P = reps[r];
+      Lattice L = P.L;
+      to_global(L);
+      RunState *st = as_global(P.st);
+      // a replica that failed in this sweep runs no further tiles (k_pass
+      // checks at the start of each colour pass)
+      const int failed = __builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (!failed) {
+        const double tau = st->tau;
+        int zfirst;
+        colour_tiles(L, P.ntz, colour, zfirst);
+        const int t = colour_tile(L, colour, zfirst, q - lo);
+        int32_t *lk = (int32_t *)smem;
+        float *lr = (float *)(lk + P.cap);
+        uint8_t *lc = (uint8_t *)(lr + P.cap);
+        pass_tile<LA, COV>(P, L, st, tau, t, colour, S, lk, lr, lc, lane);
+      }
+    }
+    // the next entry: every lane adds 1 (one atomic of 64 for the wave)
+    const unsigned int took = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    q = (int)gridDim.x + __builtin_amdgcn_readfirstlane((int)(took >> 6));
   }
 }
 
