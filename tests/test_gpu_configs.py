@@ -11,6 +11,9 @@ configured sizes, checked against the oracle.
   the engine's.  The 256-cell replica (a 3072 x 3072 x 768 grid, beyond
   the oracle's host memory) gives the same log and frame records run alone
   as in the batch, and its event counts agree with the change of its atoms.
+* The same replicas through the batched ensemble (mskg_run_ensemble), each
+  storing its bottom planes, run to 20 ML deposited: the oracle replays the
+  logs of those it holds, and the 256-cell replica ends as mskg_run alone.
 * Laser annealing: Si0.7Ge0.3 on the 264 x 264 x 384 grid in LA mode, the
   tanh front T(z) = 1700 + 300 tanh((z - zm)/5 nm) exchanged twice in the
   FEM socket layout, 1e5 events of the synchronous-sublattice engine per
@@ -25,7 +28,7 @@ import pytest
 
 from mulskips_amd import deck as deckmod
 from mulskips_amd import models
-from mulskips_amd.engine import Engine, run_serial
+from mulskips_amd.engine import Engine, run_ensemble, run_serial
 from oracle.oracle import Oracle
 from tests.helpers import list_map, oracle_state, parity_mask
 
@@ -148,6 +151,87 @@ def test_c3_replicas_at_configured_sizes(tmp_path):
             np.testing.assert_array_equal(a, b)
         alone.close()
         print(f"[c3] {r['box'][0] // 12}-cell replica: batch and single runs agree", flush=True)
+    for e in engs:
+        if e is not None:
+            e.close()
+
+
+ENS_STORED = 192  # the planes the batched ensemble stores per replica (its film stays far below)
+
+
+@pytest.mark.timeout(900)
+def test_c3_ensemble_at_configured_sizes(tmp_path):
+    """The batched ensemble path on one replica per configured lateral
+    size at configured heights, each storing its bottom 192 planes, run to
+    20 ML deposited in one mskg_run_ensemble: the oracle replays the logs of
+    the replicas whose whole box it holds (lateral 32-128 cells) bit for
+    bit, and the 256-cell replica ends exactly as mskg_run leaves it alone."""
+    reps = c3_replicas()
+    engs, starts = [], []
+    for r in reps:
+        lc = r["box"][0] // 12
+        e = Engine(r["box"], r["tables"], seed=r["seed"], tile=24, events_per_pass=4.0, tau_policy=1,
+                   frozen_below_z=FROZEN, stored_planes=ENS_STORED)
+        e.build_alloy_slab(SLAB, r["x"])
+        e.refresh_rates()
+        if lc <= ORACLE_MAX_LATERAL:
+            e.set_event_log(4_000_000)
+            starts.append(e.store_state())
+        else:
+            starts.append(None)
+        engs.append(e)
+    done, clock, stop = run_ensemble(engs, [10 ** 12] * len(engs), [r["t_end"] for r in reps])
+    print(f"[c3 ensemble] events {done.tolist()}, stop {stop.tolist()}", flush=True)
+    assert (stop == 0).all() and all(c == r["t_end"] for c, r in zip(clock, reps))
+    for r, e, st, n in zip(reps, engs, starts, done):
+        lc = r["box"][0] // 12
+        # 20 ML over the lateral cells: at least 2 deposited atoms per cell and ML
+        assert n >= 40 * lc * lc, (lc, n)
+        if st is None:
+            continue
+        log = e.event_log()
+        assert len(log) == n
+        o = oracle_for(tmp_path, r["box"], r["tables"], "FS", SLAB, f"c3e-{lc}")
+        lat0 = o.lattcoo().copy()
+        lat_s, mc_s, bulk_s, void_s = st
+        k = lat_s.shape[0]
+        m = parity_mask(r["box"])[:k]
+        part = lat0[:k]
+        part[m] = lat_s[m]
+        o.load_state(lat0, mc_s, bulk_s, void_s)
+        o.set_frozen(FROZEN)
+        o.refresh_all()
+        print(f"[c3 ensemble] oracle replays {len(log)} events of the {lc}-cell replica", flush=True)
+        o.apply_log(log)
+        lat_o, mc_o, bulk_o, void_o = oracle_state(o)
+        lat_e, mc_e, bulk_e, void_e = e.store_state()
+        assert np.count_nonzero((lat_o[:k] != lat_e) & m) == 0, f"lateral {lc}: LattCoo differs"
+        for a, b in ((mc_o, mc_e), (bulk_o, bulk_e), (void_o, void_e)):
+            assert list_map(a) == list_map(b), f"lateral {lc}: lists differ"
+        rates_agree(e, mc_o, f"c3 ensemble lateral {lc}", z_min=FROZEN)
+        o.close()
+        del lat0, st, lat_o
+    for i, r in enumerate(reps):
+        if r["box"][0] // 12 <= ORACLE_MAX_LATERAL:
+            continue
+        lat_b, mc_b, _, _ = engs[i].store_state()
+        cnt_b = engs[i].counters()
+        engs[i].close()
+        engs[i] = None
+        alone = Engine(r["box"], r["tables"], seed=r["seed"], tile=24, events_per_pass=4.0, tau_policy=1,
+                       frozen_below_z=FROZEN, stored_planes=ENS_STORED)
+        alone.build_alloy_slab(SLAB, r["x"])
+        alone.refresh_rates()
+        n, t, s = alone.run(10 ** 12, r["t_end"])
+        assert n == done[i] and t == clock[i] and s == stop[i]
+        lat_a, mc_a, _, _ = alone.store_state()
+        np.testing.assert_array_equal(lat_a, lat_b)
+        assert list_map(mc_a) == list_map(mc_b)
+        cnt_a = alone.counters()
+        np.testing.assert_array_equal(cnt_a[0], cnt_b[0])
+        assert cnt_a[1:] == cnt_b[1:]
+        alone.close()
+        print(f"[c3 ensemble] {r['box'][0] // 12}-cell replica: ensemble and mskg_run agree", flush=True)
     for e in engs:
         if e is not None:
             e.close()
