@@ -133,82 +133,100 @@ def ensemble_engine_bytes(box, stored=ENS_STORED, tile=24, cap=1024):
     return lx * ly // 4 * planes * 22 + ntiles * (4 * cap + 24)
 
 
-def batched_leg(n, tile=24):
+def batched_leg(n, tiles=(16, 24, 32), sample=16):
     """The batched-ensemble configuration at its configured sizes: n
     Si(1-x)Ge(x) replicas drawn as models.ensemble_replicas draws them
     (lateral 32-256 cells, heights 64-256 cells, T, flux, Ge fraction), each
-    run to 20 ML deposited.  The replicas are packed in draw order into
-    groups that fit the free device memory; each group is one
-    mskg_run_ensemble (every colour pass one launch over the tiles of all of
-    the group's running replicas, graph-captured sweeps).  Against it: the
-    same replicas run one after another through mskg_run, with the same
-    engine settings, so every replica's trajectory is the same in both
-    (checked: events and clock)."""
+    run to 20 ML deposited.  The tile edge comes from the ensemble tuner
+    (autotune.tune_ensemble: the first `sample` replicas run together at
+    each candidate edge).  The replicas are packed in draw order into groups
+    that fit the free device memory; each group is one mskg_run_ensemble
+    (every colour pass one launch over the tiles of all of the group's
+    running replicas, graph-captured sweeps).  Against it: the same
+    replicas run one after another through mskg_run with the same engine
+    settings, so every replica's trajectory is the same in both (checked:
+    events and clock).  The same comparison at the tile edge of the main
+    leg (24) is reported beside it."""
     import numpy as np
     import torch
 
-    from mulskips_amd import models
+    from mulskips_amd import autotune, models
     from mulskips_amd.engine import Engine, run_ensemble
     reps = models.ensemble_replicas(n)
-    free, _ = torch.cuda.mem_get_info()
-    budget = 0.85 * free
-    groups, cur, used = [], [], 0
-    for i, r in enumerate(reps):
-        b = ensemble_engine_bytes(r["box"], tile=tile)
-        if cur and used + b > budget:
-            groups.append(cur)
-            cur, used = [], 0
-        cur.append(i)
-        used += b
-    groups.append(cur)
 
-    def make(r):
+    def make(r, tile):
         e = Engine(r["box"], r["tables"], seed=r["seed"], tile=tile, events_per_pass=4.0, tau_policy=1,
                    frozen_below_z=ENS_FROZEN, stored_planes=ENS_STORED)
         e.build_alloy_slab(ENS_SLAB, r["x"])
         e.refresh_rates()
         return e
 
-    b_events, b_clock = np.zeros(n, np.int64), np.zeros(n)
-    b_wall, b_device_ms = 0.0, 0.0
-    for g in groups:
-        engs = [make(reps[i]) for i in g]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        done, clock, _ = run_ensemble(engs, [10 ** 12] * len(g), [reps[i]["t_end"] for i in g])
-        b_wall += time.perf_counter() - t0
-        b_device_ms += engs[0].last_timing()["run_ms"]
-        b_events[g], b_clock[g] = done, clock
-        for e in engs:
+    head = reps[:sample]
+    tuned = autotune.tune_ensemble(make, head, tiles, [r["t_end"] for r in head],
+                                   setup_tag=f"ensemble_replicas({n})[:{sample}] slab {ENS_SLAB} stored {ENS_STORED}")
+    trial_txt = ", ".join("%d: %.1f" % (t["tile"], t.get("events_per_s", 0) / 1e6) for t in tuned["trials"])
+    print(f"# ensemble tuner: tile {tuned['tile']} ({trial_txt} M events/s)", file=sys.stderr, flush=True)
+
+    def compare(tile):
+        free, _ = torch.cuda.mem_get_info()
+        budget = 0.85 * free
+        groups, cur, used = [], [], 0
+        for i, r in enumerate(reps):
+            b = ensemble_engine_bytes(r["box"], tile=tile)
+            if cur and used + b > budget:
+                groups.append(cur)
+                cur, used = [], 0
+            cur.append(i)
+            used += b
+        groups.append(cur)
+        b_events, b_clock = np.zeros(n, np.int64), np.zeros(n)
+        b_wall, b_device_ms = 0.0, 0.0
+        for g in groups:
+            engs = [make(reps[i], tile) for i in g]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            done, clock, _ = run_ensemble(engs, [10 ** 12] * len(g), [reps[i]["t_end"] for i in g])
+            b_wall += time.perf_counter() - t0
+            b_device_ms += engs[0].last_timing()["run_ms"]
+            b_events[g], b_clock[g] = done, clock
+            for e in engs:
+                e.close()
+            del engs
+            print(f"# batched group of {len(g)} at tile {tile}: {int(done.sum())} events", file=sys.stderr, flush=True)
+        s_events, s_clock = np.zeros(n, np.int64), np.zeros(n)
+        s_wall = 0.0
+        for i, r in enumerate(reps):
+            e = make(r, tile)
+            t0 = time.perf_counter()
+            k, t, _ = e.run(10 ** 12, max_time=r["t_end"])
+            s_wall += time.perf_counter() - t0
+            s_events[i], s_clock[i] = k, t
             e.close()
-        del engs
-        print(f"# batched group of {len(g)}: {int(done.sum())} events", file=sys.stderr, flush=True)
-    s_events, s_clock = np.zeros(n, np.int64), np.zeros(n)
-    s_wall = 0.0
-    for i, r in enumerate(reps):
-        e = make(r)
-        t0 = time.perf_counter()
-        k, t, _ = e.run(10 ** 12, max_time=r["t_end"])
-        s_wall += time.perf_counter() - t0
-        s_events[i], s_clock[i] = k, t
-        e.close()
+        total = int(b_events.sum())
+        return {"tile": tile, "groups": [len(g) for g in groups], "events": total, "value": total / b_wall,
+                "wall_s": b_wall, "device_ms": b_device_ms,
+                "sequential_plain": {"events": int(s_events.sum()), "value": float(s_events.sum()) / s_wall,
+                                     "wall_s": s_wall},
+                "speedup": (total / b_wall) / (float(s_events.sum()) / s_wall),
+                "identical_to_sequential": bool((b_events == s_events).all() and (b_clock == s_clock).all()),
+                "device_bytes": int(sum(ensemble_engine_bytes(r["box"], tile=tile) for r in reps))}
+
+    main = compare(tuned["tile"])
+    other = compare(24 if tuned["tile"] != 24 else 16)
     lat = [r["box"][0] // 12 for r in reps]
     hts = [r["box"][2] // 12 for r in reps]
     count = lambda v: {str(k): int(c) for k, c in zip(*np.unique(v, return_counts=True))}  # noqa: E731
-    total = int(b_events.sum())
-    return {"replicas": n, "groups": [len(g) for g in groups], "events": total, "value": total / b_wall,
-            "unit": "events/s", "wall_s": b_wall, "device_ms": b_device_ms,
-            "sequential_plain": {"events": int(s_events.sum()), "value": float(s_events.sum()) / s_wall,
-                                 "wall_s": s_wall},
-            "speedup": (total / b_wall) / (float(s_events.sum()) / s_wall),
-            "identical_to_sequential": bool((b_events == s_events).all() and (b_clock == s_clock).all()),
-            "device_bytes": int(sum(ensemble_engine_bytes(r["box"], tile=tile) for r in reps)),
-            "workload": f"{n} Si(1-x)Ge(x) replicas as configured: lateral cells {count(lat)}, height cells "
-                        f"{count(hts)}, T 900-1500 K, flux 0.1-10 ML/s, Ge 0-0.5, 20 ML deposited each; "
-                        f"the bottom {ENS_STORED} planes of each box stored; tile {tile}, 4 events per tile and "
-                        f"pass; packed into {len(groups)} groups that fit HBM, each one mskg_run_ensemble "
-                        "(graph-captured sweeps), against the same replicas run one after another through "
-                        "mskg_run"}
+    out = dict(main, replicas=n, unit="events/s",
+               tuning={"tile": tuned["tile"], "sample": sample, "cached": tuned["cached"],
+                       "trials": tuned["trials"]},
+               workload=f"{n} Si(1-x)Ge(x) replicas as configured: lateral cells {count(lat)}, height cells "
+                        f"{count(hts)}, T 900-1500 K, flux 0.1-10 ML/s, Ge 0-0.5, 20 ML deposited each; the "
+                        f"bottom {ENS_STORED} planes of each box stored; 4 events per tile and pass; packed into "
+                        f"{len(main['groups'])} groups that fit HBM, each one mskg_run_ensemble (graph-captured "
+                        "sweeps), against the same replicas run one after another through mskg_run at the same "
+                        "tile edge")
+    out[f"at_tile_{other['tile']}"] = other
+    return out
 
 
 def la_leg(cycles, events_per_cycle=100_000, seed=2026):

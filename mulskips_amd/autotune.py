@@ -169,3 +169,54 @@ def tune(box, tables, setup, tiles=None, events_per_pass=(1.0, 2.0, 4.0), warmup
             json.dump(store, f, indent=1, sort_keys=True)
         os.replace(tmp, cache)
     return dict(out, cached=False)
+
+
+def tune_ensemble(make, replicas, tiles, max_time, max_events=None, cache=None, device=None, setup_tag=None):
+    """Pick the tile edge of a batched ensemble (mskg_run_ensemble): the
+    sample replicas run together once per candidate tile, each engine from
+    make(replica, tile) in its starting state, and the tile with the most
+    events per device second wins.  Every replica of an ensemble takes the
+    chosen edge (it must divide every box, as candidate_tiles() checks).
+    max_time / max_events: per-replica limits of the timed runs.  Cached
+    like tune(), keyed by the device, the candidates and setup_tag (which
+    must name the sample and its starting states)."""
+    from .engine import run_ensemble
+    device = device or device_name()
+    tiles = sorted(int(t) for t in tiles)
+    key = f"{device}|ensemble|{json.dumps({'tiles': tiles, 'setup': setup_tag}, sort_keys=True)}"
+    store = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            store = json.load(f)
+        if key in store:
+            return dict(store[key], cached=True)
+    n = len(replicas)
+    me = [10 ** 12] * n if max_events is None else list(max_events)
+    trials = []
+    for t in tiles:
+        tr = {"tile": t}
+        engs = []
+        try:
+            engs = [make(r, t) for r in replicas]
+            done, _, _ = run_ensemble(engs, me, list(max_time))
+            ms = engs[0].last_timing()["run_ms"]
+            tr["events"] = int(done.sum())
+            tr["events_per_s"] = float(done.sum()) / (ms * 1e-3) if ms > 0 else 0.0
+        except EngineError as exc:
+            tr["error"] = str(exc)
+        finally:
+            for e in engs:
+                e.close()
+        trials.append(tr)
+    ok = [tr for tr in trials if "events_per_s" in tr]
+    if not ok:
+        raise RuntimeError(f"every candidate failed: {trials}")
+    best = max(ok, key=lambda tr: tr["events_per_s"])
+    out = {"tile": best["tile"], "events_per_s": best["events_per_s"], "trials": trials, "key": key}
+    if cache:
+        store[key] = out
+        tmp = cache + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(store, f, indent=1, sort_keys=True)
+        os.replace(tmp, cache)
+    return dict(out, cached=False)
