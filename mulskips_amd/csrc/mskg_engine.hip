@@ -1146,7 +1146,12 @@ k_pass_ens(const PassParams *__restrict__ reps, int n, int *plan, int colour) {
   const int *pc = plan + colour * (n + 1);
   const int total = __builtin_amdgcn_readfirstlane(pc[n]);
   unsigned int *const head = (unsigned int *)(plan + 8 * (n + 1) + colour);
-  int r = 0, lo = 0, hi = 0;  // the last replica found and its entries [lo, hi)
+  // the replica of the last entry: its entries [lo, hi) of the queue, its
+  // lattice and run state, the window's first layer and the window tau
+  int r = -1, lo = 0, hi = 0, zfirst = 0, failed = 0;
+  double tau = 0.0;
+  Lattice L = reps[0].L;
+  RunState *st = nullptr;
   // Every decision of this loop is wave-uniform (readfirstlane / ballot)
   // and no memory access sits under a lane-dependent branch.  A wave's
   // first entry is its own (blockIdx.x), so that the waves do not all meet
@@ -1160,26 +1165,24 @@ k_pass_ens(const PassParams *__restrict__ reps, int n, int *plan, int colour) {
       r = __builtin_amdgcn_readfirstlane(ens_replica(pc, n, q, lane));
       lo = __builtin_amdgcn_readfirstlane(pc[r]);
       hi = __builtin_amdgcn_readfirstlane(pc[r + 1]);
-    }
-    {
-      const PassParams &P = reps[r];
-      Lattice L = P.L;
+      L = reps[r].L;
       to_global(L);
-      RunState *st = as_global(P.st);
-      // a replica that failed in this sweep runs no further tiles (k_pass
-      // checks at the start of each colour pass)
-      const int failed = __builtin_amdgcn_readfirstlane(
-          __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      if (!failed) {
-        const double tau = st->tau;
-        int zfirst;
-        colour_tiles(L, P.ntz, colour, zfirst);
-        const int t = colour_tile(L, colour, zfirst, q - lo);
-        int32_t *lk = (int32_t *)smem;
-        float *lr = (float *)(lk + P.cap);
-        uint8_t *lc = (uint8_t *)(lr + P.cap);
-        pass_tile<LA, COV>(P, L, st, tau, t, colour, S, lk, lr, lc, lane);
-      }
+      st = as_global(reps[r].st);
+      // a replica that has failed in this sweep runs no further tiles here
+      // (k_pass checks at the start of each colour pass)
+      failed = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&st->err, __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT));
+      tau = st->tau;
+      colour_tiles(L, reps[r].ntz, colour, zfirst);
+    }
+    if (!failed) {
+      const PassParams &P = reps[r];
+      const int t = colour_tile(L, colour, zfirst, q - lo);
+      int32_t *lk = (int32_t *)smem;
+      float *lr = (float *)(lk + P.cap);
+      uint8_t *lc = (uint8_t *)(lr + P.cap);
+      // a wave stops its tiles of a replica whose tile failed (as k_pass's wave returns)
+      failed = __builtin_amdgcn_readfirstlane(pass_tile<LA, COV>(P, L, st, tau, t, colour, S, lk, lr, lc, lane));
     }
     // the next entry: every lane adds 1 (one atomic of 64 for the wave)
     const unsigned int took = __hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
