@@ -1239,7 +1239,12 @@ static void coord_pass(orc_sim *s, int x0, int x1, int y0, int y1, int z0, int z
         }
       }
 }
+static void prob_pass_v(orc_sim *s, int x0, int x1, int y0, int y1, int z0, int z1, int walls, int voids);
 static void prob_pass(orc_sim *s, int x0, int x1, int y0, int y1, int z0, int z1, int walls) {
+  prob_pass_v(s, x0, x1, y0, y1, z0, z1, walls, 0);
+}
+/* voids=1: an empty site of coordination 4 joins the void list (SetCAD.f:330-332) */
+static void prob_pass_v(orc_sim *s, int x0, int x1, int y0, int y1, int z0, int z1, int walls, int voids) {
   int32_t site[3], nn[4][3], nbuff[3] = {0, 0, 0};
   for (int z = z0; z <= z1; z++)
     for (int y = y0; y <= y1; y++)
@@ -1265,8 +1270,12 @@ static void prob_pass(orc_sim *s, int x0, int x1, int y0, int y1, int z0, int z1
               }
               cp3(nn[0], nbuff);
             }
-            get_prob_ini(s, 0, 0, coor, site, nn, &iev, &prob);
-            add_mc(s, site, nn, iev, prob);
+            if (voids && coor == 4) {
+              add_void(s, site, nn);
+            } else {
+              get_prob_ini(s, 0, 0, coor, site, nn, &iev, &prob);
+              add_mc(s, site, nn, iev, prob);
+            }
           }
         } else {
           if (coor <= 3) {
@@ -1370,6 +1379,35 @@ static void set_si_finfet(orc_sim *s, int len, int lenww, int lenwwtop, int lap)
       }
   coord_pass(s, 0, s->lx - 1, 0, s->ly - 1, 6, lenwwtop + 9, 1);
   prob_pass(s, 0, s->lx - 1, 0, s->ly - 1, 6, lenwwtop + 9, 1);
+}
+
+/* SetCAD.f:84-418 for InitSt 'IN': wall planes z = 0..3 and LenZ-4..LenZ-1
+ * on every grid point (:84-104); the geometry ids of planes z = 4..LenZ-5
+ * (:106-190), ids[x + LenX*(y + LenY*z)]: 10 a wall on that grid point, 1 an
+ * atom on a diamond site, of species 1, or of species 2 where species2[] is
+ * set (the reference draws RAND() < alloyfraction there; NULL: all species
+ * 1, alloyfraction 0); then the coordination pass over the whole box
+ * (:194-303, isolated atoms removed) and the probability pass (:309-416),
+ * which puts an empty site of coordination 4 in the void list. */
+static void set_cad(orc_sim *s, const int8_t *ids, const int8_t *species2) {
+  set_wall(s, 0, 3);
+  set_wall(s, s->lz - 4, s->lz - 1);
+  for (int z = 4; z <= s->lz - 5; z++)
+    for (int y = 0; y < s->ly; y++)
+      for (int x = 0; x < s->lx; x++) {
+        const int64_t g = gidx(s, x, y, z);
+        const int id = ids[g];
+        int32_t *w = &s->latt[g];
+        if (id == 10) {
+          *w |= (1 << POS_OCC);
+          mvbits(WALL, LEN_INDEX, w, POS_INDEX);
+        } else if (id == 1 && dia_type(x, y, z)) {
+          *w |= (1 << POS_OCC);
+          *w |= (1 << ((species2 && species2[g]) ? POS_C : POS_SI));
+        }
+      }
+  coord_pass(s, 0, s->lx - 1, 0, s->ly - 1, 0, s->lz - 1, 1);
+  prob_pass_v(s, 0, s->lx - 1, 0, s->ly - 1, 0, s->lz - 1, 1, 1);
 }
 
 /* ---- start.dat (main.f:83-212, SetProbability.f:21-143) -------------- */
@@ -1943,6 +1981,25 @@ int orc_load_state(orc_sim *s, const int32_t *latt, int32_t n_ad, const int32_t 
     add_void(s, void_xyz + 3 * (int64_t)i, nn);
   }
   s->time = time;
+  s->iter = 1;
+  s->have_jb = 0;
+  return 0;
+}
+
+int orc_build_cad(orc_sim *s, const int8_t *ids, const int8_t *species2) {
+  if (setjmp(s->jb)) { s->have_jb = 0; return 2; }
+  s->have_jb = 1;
+  if (!s->tree) allocate_tree(s);
+  memset(s->latt, 0, (size_t)s->npts * sizeof(int32_t));
+  memset(s->lind, 0, (size_t)s->npts * sizeof(int32_t));
+  memset(s->tree, 0, ((size_t)s->size_tree + 2) * sizeof(double));
+  memset(s->ad, 0, (size_t)(NUM_MC_MAX + 1) * sizeof(mc_rec));
+  memset(s->vd, 0, (size_t)(NUM_VD_MAX + 1) * sizeof(at_rec));
+  if (s->at) memset(s->at, 0, (size_t)s->cap_at * sizeof(at_rec));
+  s->n_ad = s->n_at = s->n_vd = 0;
+  s->last_key = -1;
+  set_cad(s, ids, species2);
+  s->time = 0.0;
   s->iter = 1;
   s->have_jb = 0;
   return 0;

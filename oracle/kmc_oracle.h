@@ -90,6 +90,15 @@ int orc_set_numberslist(orc_sim *s, const double *values, int n);
  * Returns 0 on success. */
 int orc_load_deck(orc_sim *s, const char *start_dat_path);
 
+/* Replace the state by SetCAD's (initial/SetCAD.f:84-418, InitSt 'IN'):
+ * ids holds one geometry id per grid point, ids[x + LenX*(y + LenY*z)] for
+ * z = 0..LenZ-1 (the planes of the geometry file; 10 wall, 1 atom on a
+ * diamond site, else empty; planes 0..3 and LenZ-4..LenZ-1 are walls
+ * whatever they hold).  species2: NULL, or nonzero where the atom is of
+ * species 2 (the reference's RAND() < alloyfraction draw).  Tables and
+ * options stay those of the loaded deck.  Returns 0 on success. */
+int orc_build_cad(orc_sim *s, const int8_t *ids, const int8_t *species2);
+
 /* The whole program main.f:296-379: frame 0 output, the KMC loop with its
  * output cadence and the final frame.  When outdir is non-NULL the
  * I????????.xyz (and _d.xyz) frames are written there.  When log is

@@ -40,6 +40,7 @@ def _load():
         L.orc_last_error.argtypes = [P]
         L.orc_set_numberslist.argtypes = [P, P, I]
         L.orc_load_deck.argtypes = [P, C.c_char_p]
+        L.orc_build_cad.argtypes = [P, P, P]
         L.orc_main.argtypes = [P, C.c_char_p, P, I64, P, P]
         L.orc_step.argtypes = [P, P]
         L.orc_apply.argtypes = [P, P, I, I]
@@ -136,6 +137,17 @@ class Oracle:
                                    len(bulk["xyz"]), _p(keep[4]), _p(keep[5]), len(void["xyz"]), _p(keep[6]),
                                    _p(keep[7]), float(kmc_time))
         self._check(rc, "orc_load_state")
+
+    def build_cad(self, ids, species2=None):
+        """SetCAD (InitSt 'IN'): ids int8 (LenZ, LenY, LenX) geometry ids
+        (10 wall, 1 atom on a diamond site); species2: None or a bool array
+        of the same shape, species 2 where set."""
+        lx, ly, lz = self.dims
+        a = np.ascontiguousarray(ids, np.int8)
+        assert a.shape == (lz, ly, lx), a.shape
+        b = None if species2 is None else np.ascontiguousarray(species2, np.int8)
+        assert b is None or b.shape == a.shape
+        self._check(self.L.orc_build_cad(self.h, _p(a), None if b is None else _p(b)), "orc_build_cad")
 
     def run_main(self, outdir=None, log_cap=0):
         log = np.zeros(max(log_cap, 1), EVENT_DTYPE)
