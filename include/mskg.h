@@ -138,6 +138,30 @@ int mskg_build_slab(mskg_engine *e, int32_t mode, int32_t len);
  * RAND().  Needs n_crystal >= 2.  The atoms' rates come from Get_Prob. */
 int mskg_build_alloy_slab(mskg_engine *e, int32_t len, double fraction);
 
+/* The shaped initial states of main.f:224-294, built on the device, with
+ * InitSt and Len1..Len4 as start.dat gives them (main.f's reductions to
+ * multiples of 24 applied where main.f applies them): 'C' the box of
+ * initial/SetNC.f:25-235 and 'S' the sphere of SetNCSp.f:25-237 (Si on fcc
+ * sublattice A, C on B, centred in the box), 'I' the inverted pyramids of
+ * SetInvPOff.f:21-286, 'FF' the fin between walls of SetSiFINFET.f:25-312,
+ * and 'F' / 'FS' the slabs of mskg_build_slab.  As the reference: atoms and
+ * walls placed, atoms with no occupied neighbour removed, coordination and
+ * lists of every site, rates from Get_Prob_Ini (placeholder 1e-6).  The
+ * result equals the reference builder followed by mskg_load_state, except
+ * that a deposition's species is drawn when it happens.  A temperature set
+ * before (laser-annealing mode) stays.  Needs every plane stored. */
+int mskg_build_shape(mskg_engine *e, const char *init_st, const int32_t lens[4]);
+
+/* InitSt 'IN' (initial/SetCAD.f:84-418) on the device: ids holds the
+ * geometry file, one id per grid point, ids[x + len_x*(y + len_y*z)] for
+ * z = 0..len_z-1 (10 a wall on that grid point, 1 an atom on a diamond site,
+ * anything else empty; planes 0..3 and len_z-4..len_z-1 are walls whatever
+ * they hold), in host memory or, with on_device, device memory.  Atoms are
+ * species 2 with probability alloy_fraction (the reference's RAND() <
+ * alloyfraction, drawn here from the engine's Philox stream), else 1.
+ * Then as mskg_build_shape; an empty site of coordination 4 is a void. */
+int mskg_build_cad(mskg_engine *e, const int8_t *ids, int32_t on_device, double alloy_fraction);
+
 /* Per-site temperature in K for laser-annealing mode, (len_x, len_y,
  * len_z+1) like LattCoo's T bits (SetT.f:25-50).  Every value at a lattice
  * point must lie in 0..8191 (else MSKG_EINVAL and nothing changes).  Rates

@@ -24,6 +24,7 @@ EXPORTS = [
     "mskg_set_la_variant", "mskg_la_counts", "mskg_set_st_tables", "mskg_species_counts",
     "mskg_run_diagnostics", "mskg_frame_defects", "mskg_frame_device", "mskg_set_schedule",
     "mskg_refresh_active", "mskg_adjoint_checkpointed", "mskg_run_ensemble",
+    "mskg_build_shape", "mskg_build_cad",
 ]
 MSKG_FIELD_SOCKET, MSKG_FIELD_PLANES = 0, 1  # layouts of the laser-annealing maps
 
@@ -87,6 +88,8 @@ def lib():
     L.mskg_run_serial.argtypes = [P, I32, P, P, P, P, P]
     L.mskg_run_ensemble.argtypes = [P, I32, P, P, P, P, P]
     L.mskg_build_alloy_slab.argtypes = [P, I32, D]
+    L.mskg_build_shape.argtypes = [P, C.c_char_p, P]
+    L.mskg_build_cad.argtypes = [P, P, I32, D]
     L.mskg_track_classes.argtypes = [P, I32]
     L.mskg_copy_state.argtypes = [P, P]
     L.mskg_frame.argtypes = [P, P, I64, P, I64, P, I64, P]

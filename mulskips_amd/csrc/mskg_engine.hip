@@ -1285,6 +1285,210 @@ __global__ void __launch_bounds__(256) k_build_slab(Lattice L, int len, int sic,
 }
 
 
+// ---- the shaped initial states (main.f:224-294) ----------------------------------
+// SetNC / SetNCSp / SetInvPOff / SetSiFINFET / SetCAD all place atoms and
+// walls first, then run one coordination pass (an atom with no occupied
+// neighbour is removed) and one probability pass that fills the lists.  On
+// the device: k_shape_place writes every stored point's kind from the
+// builder's geometry, k_shape_isolated removes the isolated atoms,
+// k_shape_records gives every site its coordination and neighbour list,
+// and the rate sweep evaluates the MC particles.  An isolated atom's
+// neighbours are all empty, so no removal changes a coordination another
+// removal reads, and the result does not depend on the order of the
+// reference's sequential pass.
+enum : int { SHAPE_BOX = 0, SHAPE_SPHERE = 1, SHAPE_INVP = 2, SHAPE_FINFET = 3, SHAPE_CAD = 4 };
+struct ShapeSpec {
+  int shape;
+  int l1, l2, l3, l4;   // Len1..Len4 as the builder takes them
+  int cx, cy, cz;       // SetNC / SetNCSp centre (LenX/2, LenY/2, LenZ/2)
+  int lz;               // LenZ
+  int xa, xb;           // SetSiFINFET: the fin lies in xa <= x < xb
+  int zfix;             // SetInvPOff: atoms at z <= zfix are bulk of coordination 4 (-1: none)
+  int walls;            // an empty site's coordination leaves walls out
+  const int8_t *ids;    // SetCAD geometry ids, [z][y][x] for z < LenZ
+  double fraction;      // SetCAD: species 2 with this probability (Philox)
+  uint32_t key0, key1;
+};
+
+// The builder's content of grid point (x, y, z): KIND_WALL, a species, or
+// KIND_EMPTY.  Walls lie on every grid point of their region, atoms on the
+// diamond sites of theirs (SetNC.f, SetNCSp.f, SetInvPOff.f:21-286,
+// SetSiFINFET.f:25-312, SetCAD.f:84-190).
+__device__ inline int shape_content(const ShapeSpec &S, int x, int y, int z, int lx, int ly, int64_t p) {
+  const int t = dia_type(x, y, z);
+  switch (S.shape) {
+    case SHAPE_BOX:
+    case SHAPE_SPHERE: {
+      const int l2 = S.shape == SHAPE_BOX ? S.l2 : S.l1, l3 = S.shape == SHAPE_BOX ? S.l3 : S.l1;
+      if (!t || x < S.cx - S.l1 / 2 || x > S.cx + S.l1 / 2 - 1 || y < S.cy - l2 / 2 || y > S.cy + l2 / 2 - 1 ||
+          z < S.cz - l3 / 2 || z > S.cz + l3 / 2 - 1)
+        return KIND_EMPTY;
+      if (S.shape == SHAPE_SPHERE) {
+        const int d2 = (x - S.cx) * (x - S.cx) + (y - S.cy) * (y - S.cy) + (z - S.cz) * (z - S.cz);
+        if (!(d2 <= S.l1 * S.l1 / 4)) return KIND_EMPTY;
+      }
+      return t == 1 ? 1 : 2;
+    }
+    case SHAPE_INVP: {  // l1: Len (pyramid size), l2: Len3 (substrate top)
+      if (!t) return KIND_EMPTY;
+      if (z <= 20) return t == 1 ? 1 : 2;
+      if (z > S.l2 - 1) return KIND_EMPTY;
+      // the reference's single-precision test: integer sums minus
+      // 1.10*(z-Len3) in REAL(4) against REAL(4) Len
+      const int lenh = lx / 2 - 1;
+      const float dz = 1.10f * (float)(z - S.l2), fl = (float)S.l1;
+      const bool hole = ((float)(abs(x - lenh) + abs(y - lenh)) - dz <= fl) ||
+                        ((float)(abs(x) + abs(y)) - dz <= fl) || ((float)(abs(x - lx) + abs(y)) - dz <= fl) ||
+                        ((float)(abs(x) + abs(y - ly)) - dz <= fl) ||
+                        ((float)(abs(x - lx) + abs(y - ly)) - dz <= fl);
+      return hole ? KIND_EMPTY : (t == 1 ? 1 : 2);
+    }
+    case SHAPE_FINFET: {  // l1: Len (Si top), l2: wall bottom, l3: wall top
+      const bool wall = z <= 5 || (z >= S.lz - 6 && z <= S.lz - 1) || (z >= S.l3 && z <= S.lz - 7) ||
+                        (z >= S.l2 && z <= S.lz - 7 && (x < S.xa || x >= S.xb));
+      if (wall) return KIND_WALL;
+      return (t && z >= 6 && z <= S.l1 - 1) ? 1 : KIND_EMPTY;
+    }
+    default: {  // SHAPE_CAD
+      if (z <= 3 || (z >= S.lz - 4 && z <= S.lz - 1)) return KIND_WALL;
+      if (z > S.lz - 5) return KIND_EMPTY;
+      const int id = S.ids[(int64_t)x + (int64_t)lx * ((int64_t)y + (int64_t)ly * z)];
+      if (id == 10) return KIND_WALL;
+      if (id != 1 || !t) return KIND_EMPTY;
+      if (S.fraction > 0.0) {  // the reference's RAND() < alloyfraction, from the engine's stream
+        U4 c;
+        c.x = (uint32_t)p; c.y = (uint32_t)(p >> 32); c.z = 0x5e6e0002u; c.w = 0;
+        const U4 d = philox4x32_10(c, S.key0, S.key1);
+        if (u01_open(d.x, d.y) < S.fraction) return 2;
+      }
+      return 1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_shape_place(Lattice L, ShapeSpec S) {
+  const int64_t n = L.box.npts;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    pxyz(L.box, p, x, y, z);
+    Site s;
+    for (int a = 0; a < 4; a++) s.d[a][0] = s.d[a][1] = s.d[a][2] = 0;
+    s.valid = 0;
+    s.temp = L.site[p].temp;  // a temperature set before the build stays (SetCAD.f:62-65)
+    s.kc = (uint8_t)shape_content(S, x, y, z, L.box.lx, L.box.ly, p);
+    L.site[p] = s;
+    L.rate[p] = 0.0f;
+    L.cls[p] = 0;
+    L.inlist[p] = 0;
+  }
+}
+
+// The four neighbour sites of diamond site (x, y, z) of type t, x and y
+// wrapped once (the builders' JumpDia lists).
+__device__ inline NL dia_list(const Box &b, int t, int x, int y, int z) {
+  NL l;
+  for (int i = 0; i < 4; i++) {
+    const int *j = c_jump_dia[(t == 1 ? 0 : 4) + i];
+    l.x[i] = x + j[0]; l.y[i] = y + j[1]; l.z[i] = z + j[2];
+    if (l.x[i] >= b.lx) l.x[i] -= b.lx;
+    if (l.x[i] < 0) l.x[i] += b.lx;
+    if (l.y[i] >= b.ly) l.y[i] -= b.ly;
+    if (l.y[i] < 0) l.y[i] += b.ly;
+  }
+  return l;
+}
+__device__ inline int kind_at(const Lattice &L, int x, int y, int z) {
+  const int64_t q = pidx_checked(L.box, x, y, z);
+  return q < 0 ? KIND_EMPTY : s_kind(L.site[q]);
+}
+
+// An atom none of whose four neighbours is occupied (an atom or a wall) is
+// removed (SetCAD.f:229-246, SetNC.f's same pass); atoms of a fixed bulk
+// region are left alone.
+__global__ void __launch_bounds__(256) k_shape_isolated(Lattice L, ShapeSpec S) {
+  const int64_t n = L.box.npts;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const Site sp = L.site[p];
+    if (!kind_species(s_kind(sp))) continue;
+    int x, y, z;
+    pxyz(L.box, p, x, y, z);
+    if (z <= S.zfix) continue;
+    const int t = dia_type(x, y, z);
+    if (!t) continue;
+    const NL l = dia_list(L.box, t, x, y, z);
+    int occ = 0;
+    for (int i = 0; i < 4; i++) occ += kind_occupied(kind_at(L, l.x[i], l.y[i], l.z[i]));
+    if (!occ) {
+      Site e = sp;
+      e.kc = KIND_EMPTY;  // LattCoo = 0, the temperature kept
+      L.site[p] = e;
+    }
+  }
+}
+
+// Coordination and neighbour list of every diamond site (SetCAD.f:194-416):
+// an atom counts its occupied neighbours (walls included), an empty site
+// its atoms (with walls = 1; every occupied neighbour otherwise); an empty
+// site of coordination 1 keeps only that neighbour, in entry 1.  Atoms and
+// empty sites of coordination >= 1 get their list; walls and the empty
+// sites of coordination 0 keep a bare record.
+__global__ void __launch_bounds__(256) k_shape_records(Lattice L, ShapeSpec S, int *err) {
+  const int64_t n = L.box.npts;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    Site s = L.site[p];
+    const int kind = s_kind(s);
+    if (kind == KIND_WALL) continue;
+    int x, y, z;
+    pxyz(L.box, p, x, y, z);
+    const int t = dia_type(x, y, z);
+    if (!t) continue;
+    NL l = dia_list(L.box, t, x, y, z);
+    int coor = 0, last = -1;
+    for (int i = 0; i < 4; i++) {
+      const int k = kind_at(L, l.x[i], l.y[i], l.z[i]);
+      if (kind_species(kind) ? kind_occupied(k) : (S.walls ? kind_species(k) : kind_occupied(k))) {
+        coor++;
+        last = i;
+      }
+    }
+    if (kind_species(kind) && z <= S.zfix) coor = 4;
+    if (!kind_species(kind) && coor == 0) continue;
+    if (!kind_species(kind) && coor == 1) {
+      // the NBuff compaction of SetCAD.f:316-327: entry 1 the one atom, the rest unfixed
+      const int lx0 = l.x[last], ly0 = l.y[last], lz0 = l.z[last];
+      for (int i = 0; i < 4; i++) l.x[i] = l.y[i] = l.z[i] = 0;
+      l.x[0] = lx0; l.y[0] = ly0; l.z[0] = lz0;
+    }
+    s_set_coor(s, coor);
+    if (!set_list(L.box, s, x, y, z, l)) atomicExch(err, ERR_GEOM);
+    L.site[p] = s;
+  }
+}
+
+// The placeholder deposition rate of an initial state is Get_Prob_Ini's
+// REAL(4) 1.e-6 (Get_Prob_Ini.f:104), not Get_Prob's 1.e-8.
+__global__ void __launch_bounds__(256) k_initial_placeholders(Lattice L) {
+  const int64_t n = L.box.npts;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+    if (L.cls[p] == CLS_PLACEHOLDER) L.rate[p] = (float)(double)1.e-6f;
+}
+
+// Walls on the grid points off the stored lattice, one bit per grid point
+// (what the phase maps report): thread w sets word w.
+__global__ void __launch_bounds__(256) k_shape_offwall(Lattice L, ShapeSpec S, uint32_t *bits, int64_t words) {
+  const int lx = L.box.lx, ly = L.box.ly;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = 0;
+    for (int b = 0; b < 32; b++) {
+      const int64_t g = w * 32 + b;
+      const int x = (int)(g % lx), y = (int)((g / lx) % ly), z = (int)(g / ((int64_t)lx * ly));
+      if (z > L.box.lz || valid_parity(x, y, z)) continue;
+      if (shape_content(S, x, y, z, lx, ly, 0) == KIND_WALL) v |= 1u << b;
+    }
+    bits[w] = v;
+  }
+}
+
 // ---- serial BKL over a batch of independent lattices --------------------------
 // One workgroup (one wave) per replica runs the reference's serial algorithm:
 // every event picks one MC particle of the whole lattice with probability
@@ -3382,6 +3586,124 @@ static int build_slab(mskg_engine *e, int32_t mode, int32_t len, double fraction
   e->lists_dirty = true;
   e->keys_valid = false;
   return MSKG_OK;
+}
+
+// A shaped initial state built on the device (k_shape_place ->
+// k_shape_isolated -> k_shape_records -> rate sweep), and the off-lattice
+// wall bits of the builders that fill wall regions.
+static int build_shape(mskg_engine *e, ShapeSpec S, bool walls_off_lattice) {
+  if (e->windowed)
+    return fail_(e, MSKG_EINVAL, "the shaped builders fill the whole box: every plane must be stored");
+  S.lz = e->logical_lz;
+  S.key0 = (uint32_t)e->cfg.seed;
+  S.key1 = (uint32_t)(e->cfg.seed >> 32);
+  S.cx = e->box.lx / 2;
+  S.cy = e->box.ly / 2;
+  S.cz = e->logical_lz / 2;
+  Lattice L = make_lattice(e);
+  L.key = nullptr;
+  const int blocks = (int)std::min<int64_t>((e->box.npts + 255) / 256, 256 * 64);
+  HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
+  hipLaunchKernelGGL(k_shape_place, dim3(blocks), dim3(256), 0, e->stream, L, S);
+  hipLaunchKernelGGL(k_shape_isolated, dim3(blocks), dim3(256), 0, e->stream, L, S);
+  hipLaunchKernelGGL(k_shape_records, dim3(blocks), dim3(256), 0, e->stream, L, S, e->d_err);
+  hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(blocks),
+                     dim3(256), 0, e->stream, L, 0.0, e->d_err, 0);
+  hipLaunchKernelGGL(k_initial_placeholders, dim3(blocks), dim3(256), 0, e->stream, L);
+  HIPCHK(e, hipGetLastError());
+  hipFree(e->d_offwall);
+  e->d_offwall = nullptr;
+  e->offwall_planes = false;
+  if (walls_off_lattice) {
+    const int64_t words = (e->box.lx * (int64_t)e->box.ly * (e->box.lz + 1) + 31) / 32;
+    HIPCHK(e, hipMalloc(&e->d_offwall, words * sizeof(uint32_t)));
+    const int wb = (int)std::min<int64_t>((words + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_shape_offwall, dim3(wb), dim3(256), 0, e->stream, L, S, e->d_offwall, words);
+    HIPCHK(e, hipGetLastError());
+  }
+  int err = 0;
+  HIPCHK(e, hipMemcpyAsync(&err, e->d_err, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  RunState st;
+  memset(&st, 0, sizeof st);
+  st.events_per_pass = e->cfg.events_per_pass > 0 ? e->cfg.events_per_pass : 1.0;
+  HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(e, hipStreamSynchronize(e->stream));
+  e->loaded = true;  // the state is the builder's, whatever follows
+  e->lists_dirty = true;
+  e->keys_valid = false;
+  if (err) {
+    e->loaded = false;
+    return fail_(e, MSKG_ESTATE, "shaped builder: %s", err_name(err));
+  }
+  if (int rc_la = la_variant_off(e)) return rc_la;
+  if (int rc_c = census(e, e->census0)) return rc_c;
+  return MSKG_OK;
+}
+
+int mskg_build_shape(mskg_engine *e, const char *init_st, const int32_t lens[4]) {
+  if (!e || !e->d_site || !init_st || !lens) return MSKG_EINVAL;
+  const std::string is(init_st);
+  if (is == "F" || is == "FS") return mskg_build_slab(e, is == "F" ? 1 : 0, lens[0] - lens[0] % 24);
+  ShapeSpec S;
+  memset(&S, 0, sizeof S);
+  S.zfix = -1;
+  S.l1 = lens[0]; S.l2 = lens[1]; S.l3 = lens[2]; S.l4 = lens[3];
+  if (is == "C" || is == "S") {  // SetNC.f:25-235, SetNCSp.f:25-237
+    S.shape = is == "C" ? SHAPE_BOX : SHAPE_SPHERE;
+    if (S.l1 <= 0 || (is == "C" && (S.l2 <= 0 || S.l3 <= 0)))
+      return fail_(e, MSKG_EINVAL, "InitSt %s: the shape's lengths must be positive", init_st);
+    return build_shape(e, S, false);
+  }
+  if (is == "I") {  // SetInvPOff.f:21-286: Len1 the pyramid, Len2 the substrate top
+    S.shape = SHAPE_INVP;
+    S.zfix = 20;
+    if (S.l2 <= 21 || S.l2 + 10 > e->logical_lz)
+      return fail_(e, MSKG_EINVAL, "InitSt I: Len2 %d must lie in 22..LenZ-10", S.l2);
+    return build_shape(e, S, false);
+  }
+  if (is == "FF") {  // SetSiFINFET.f:25-312 with main.f:281-285's reductions
+    S.shape = SHAPE_FINFET;
+    S.l1 -= S.l1 % 24; S.l2 -= S.l2 % 24; S.l3 -= S.l3 % 24; S.l4 -= S.l4 % 24;
+    if (S.l1 < 6 || S.l3 + 10 > e->logical_lz)
+      return fail_(e, MSKG_EINVAL, "InitSt FF: lengths %d %d %d %d do not fit the box", S.l1, S.l2, S.l3, S.l4);
+    S.xa = (int)(0.5f * (float)(e->box.lx - S.l4));
+    S.xb = (int)(0.5f * (float)(e->box.lx + S.l4));
+    S.walls = 1;
+    return build_shape(e, S, true);
+  }
+  return fail_(e, MSKG_EINVAL, "InitSt '%s': the device builds F, FS, C, S, I, FF (and IN with mskg_build_cad)",
+               init_st);
+}
+
+int mskg_build_cad(mskg_engine *e, const int8_t *ids, int32_t on_device, double alloy_fraction) {
+  if (!e || !e->d_site || !ids) return MSKG_EINVAL;
+  if (!(alloy_fraction >= 0.0 && alloy_fraction <= 1.0))
+    return fail_(e, MSKG_EINVAL, "alloy fraction must be in [0, 1]");
+  if (alloy_fraction > 0.0 && e->ncrystal < 2) return fail_(e, MSKG_EINVAL, "an alloy needs n_crystal >= 2");
+  if (e->logical_lz < 10) return fail_(e, MSKG_EINVAL, "SetCAD needs LenZ >= 10");
+  ShapeSpec S;
+  memset(&S, 0, sizeof S);
+  S.shape = SHAPE_CAD;
+  S.zfix = -1;
+  S.walls = 1;
+  S.fraction = alloy_fraction;
+  int8_t *d_ids = nullptr;
+  const size_t nb = (size_t)e->box.lx * e->box.ly * e->logical_lz;
+  if (on_device) {
+    S.ids = ids;
+  } else {
+    HIPCHK(e, hipMalloc(&d_ids, nb));
+    hipError_t ce = hipMemcpyAsync(d_ids, ids, nb, hipMemcpyHostToDevice, e->stream);
+    if (ce != hipSuccess) {
+      hipFree(d_ids);
+      return fail_(e, MSKG_EDEVICE, "geometry upload: %s", hipGetErrorString(ce));
+    }
+    S.ids = d_ids;
+  }
+  const int rc = build_shape(e, S, true);
+  hipStreamSynchronize(e->stream);
+  hipFree(d_ids);
+  return rc;
 }
 
 int mskg_track_classes(mskg_engine *e, int32_t on) {

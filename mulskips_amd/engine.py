@@ -151,6 +151,28 @@ class Engine:
         """The SetSi slab as a random alloy: species 2 with probability fraction."""
         self._check(self.L.mskg_build_alloy_slab(self.h, int(length), float(fraction)))
 
+    def build_shape(self, init_st, lens):
+        """A shaped initial state on the device (mskg_build_shape): InitSt
+        'C', 'S', 'I', 'FF', 'F' or 'FS' with start.dat's Len1..Len4."""
+        ln = np.zeros(4, np.int32)
+        ln[:len(lens)] = lens
+        self._check(self.L.mskg_build_shape(self.h, str(init_st).encode(), ptr(ln)))
+
+    def build_cad(self, ids, alloy_fraction=0.0):
+        """SetCAD (InitSt 'IN') on the device: ids int8 (LenZ, LenY, LenX)
+        geometry ids (10 wall, 1 atom on a diamond site), a numpy array or an
+        int8 torch tensor on the engine's device."""
+        lx, ly, lz = self.box
+        dev = _device_tensor(ids)
+        if dev is not None:
+            assert tuple(dev.shape) == (lz, ly, lx) and dev.dtype == _torch().int8 and dev.is_contiguous()
+            _torch().cuda.current_stream(dev.device).synchronize()
+            self._check(self.L.mskg_build_cad(self.h, C.c_void_p(dev.data_ptr()), 1, float(alloy_fraction)))
+            return
+        a = np.ascontiguousarray(ids, np.int8)
+        assert a.shape == (lz, ly, lx), a.shape
+        self._check(self.L.mskg_build_cad(self.h, ptr(a), 0, float(alloy_fraction)))
+
     def frame(self):
         """Device frame compaction: (main, voids, non_epitaxial) record arrays
         of (x, y, z, code) rows in grid order and the header counts

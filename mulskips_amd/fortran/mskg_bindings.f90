@@ -222,6 +222,23 @@ module mskg_bindings
       integer(c_int32_t), intent(out) :: stop_reason(*)
     end function
 
+    ! the shaped initial states of main.f:224-294 (init_st NUL-terminated, lens = Len1..Len4)
+    integer(c_int) function mskg_build_shape(eng, init_st, lens) bind(c, name="mskg_build_shape")
+      import :: c_int, c_int32_t, c_ptr, c_char
+      type(c_ptr), value :: eng
+      character(kind=c_char), intent(in) :: init_st(*)
+      integer(c_int32_t), intent(in) :: lens(4)
+    end function
+
+    ! SetCAD: the geometry ids (one int8 per grid point, x fastest, z = 0..LenZ-1)
+    integer(c_int) function mskg_build_cad(eng, ids, on_device, alloy_fraction) bind(c, name="mskg_build_cad")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: eng
+      type(c_ptr), value :: ids
+      integer(c_int32_t), value :: on_device
+      real(c_double), value :: alloy_fraction
+    end function
+
     ! the batched synchronous-sublattice ensemble (same arguments as mskg_run_serial)
     integer(c_int) function mskg_run_ensemble(engines, n, max_events, max_time, events_done, kmc_time, &
         stop_reason) bind(c, name="mskg_run_ensemble")
