@@ -101,3 +101,30 @@ def test_tile_schedule_log_is_a_serial_history(tmp_path):
     assert list_map(mc_o) == list_map(mc_e)
     o.close()
     eng.close()
+
+
+def test_schedules_agree_in_laser_annealing_mode():
+    """The laser-annealing kernels (the bench's melt-front leg runs the tile
+    schedule): SiGe x=0.3 on the 264 x 264 x 384 grid under the tanh front,
+    two temperature exchanges, the same state and clock from both orders."""
+    from mulskips_amd import models
+    tab, _ = models.la_sige_tables(2026)
+    box = (264, 264, 384)
+    out = {}
+    for sch in ("colours", "tiles"):
+        eng = Engine(box, tab, seed=2026, tau_policy=1, events_per_pass=4.0, la=models.LA_CONSTANTS)
+        eng.build_alloy_slab(240, 0.3)
+        eng.set_schedule(sch)
+        runs = []
+        for zm in (240.0, 243.0):
+            eng.set_temperature_field(models.la_front_field(box, zm), "socket")
+            eng.refresh_rates()
+            runs.append(eng.run(50000))
+        print(f"[LA {sch}] runs {runs}", flush=True)
+        out[sch] = (snapshot(eng), runs)
+        eng.close()
+    (la, ma, ca, ta, na), ra = out["colours"]
+    (lb, mb, cb, tb, nb), rb = out["tiles"]
+    assert ra == rb and ra[0][0] >= 50000
+    assert np.array_equal(la, lb) and ma == mb
+    assert np.array_equal(ca, cb) and ta == tb and na == nb
