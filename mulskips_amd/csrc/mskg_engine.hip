@@ -3132,7 +3132,11 @@ static int refresh_range(mskg_engine *e, int64_t first, int64_t end, bool lists_
                        e->d_err, 0);
     HIPCHK(e, hipGetLastError());
   } else if (end > first) {
-    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((end - first + 1023) / 1024, 256 * 64));
+    // one group of four points per thread up to 65536 blocks: the range of
+    // the active layers (~57 M points at 512^3) runs in a single round of
+    // groups (0.671 of HBM peak, against 0.647-0.650 at 16384 or 4096
+    // blocks with several groups per thread)
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((end - first + 1023) / 1024, 65536));
     hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep_range, la, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L,
                        st.rtot, e->d_err, first, end);
     HIPCHK(e, hipGetLastError());
