@@ -3685,6 +3685,8 @@ int mskg_build_cad(mskg_engine *e, const int8_t *ids, int32_t on_device, double 
     return fail_(e, MSKG_EINVAL, "alloy fraction must be in [0, 1]");
   if (alloy_fraction > 0.0 && e->ncrystal < 2) return fail_(e, MSKG_EINVAL, "an alloy needs n_crystal >= 2");
   if (e->logical_lz < 10) return fail_(e, MSKG_EINVAL, "SetCAD needs LenZ >= 10");
+  if (e->windowed)
+    return fail_(e, MSKG_EINVAL, "the shaped builders fill the whole box: every plane must be stored");
   ShapeSpec S;
   memset(&S, 0, sizeof S);
   S.shape = SHAPE_CAD;
@@ -4021,7 +4023,10 @@ int mskg_run_ensemble(mskg_engine *const *engines, int32_t n, const int64_t *max
     batch = (int)std::max<int64_t>(1, std::min<int64_t>(sweeps / 4, 32));
   }
   if (ce == hipSuccess) ce = hipEventRecord(e0->ev1, sm);
-  if (ce == hipSuccess) ce = hipStreamSynchronize(sm);
+  // whatever happened, nothing queued may still use the plan or the
+  // parameters when they are freed
+  const hipError_t cs = hipStreamSynchronize(sm);
+  if (ce == hipSuccess) ce = cs;
   if (ce == hipSuccess) ce = hipEventElapsedTime(&ms, e0->ev0, e0->ev1);
   if (exec) hipGraphExecDestroy(exec);
   if (graph) hipGraphDestroy(graph);
