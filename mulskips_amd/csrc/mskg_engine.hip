@@ -1052,57 +1052,64 @@ __global__ void __launch_bounds__(NSLOT) k_sweep_end(RunState *st, int *win, con
 // ---- the batched ensemble: one sweep over many lattices ------------------------
 // Every replica keeps its own lattice, lists, windows, run state and random
 // stream; a colour pass is one launch over that colour's tiles of all the
-// replicas still running.  The plan, remade after every sweep, lays the
-// colour's tiles of the replicas end to end:
+// replicas still running.  Before each colour pass a plan lays that
+// colour's tiles of the replicas end to end -- made just before the pass,
+// as k_pass reads its window when it starts, because the passes of the
+// earlier colours widen the windows of the later ones when they enlist
+// points in a new tile layer:
 //   plan[c * (n + 1) + r]      c = 0..7, r = 0..n: tiles of colour c in the
-//                              windows of replicas 0..r-1 (replicas that have
-//                              stopped hold none)
+//                              windows of replicas 0..r-1 (replicas that do
+//                              not run hold none)
 //   plan[8 * (n + 1) + c]      queue head of colour c, in units of 64
-//   plan[8 * (n + 1) + 8]      replicas still running
+//   plan[8 * (n + 1) + 8]      replicas running the coming sweep
 //   plan[8 * (n + 1) + 9 + r]  replica r runs the coming sweep
-// Persistent waves take the entries of a colour's queue one at a time, so
-// the tiles of large and small replicas share the GPU and the slowest tiles
-// of a pass overlap other tiles' work instead of idling the rest.
+// The plan of colour 0 (start = 1) comes after the sweep's end and also
+// decides which replicas run the next sweep.  Persistent waves take the
+// entries of a colour's queue one at a time, so the tiles of large and
+// small replicas share the GPU and the slowest tiles of a pass overlap
+// other tiles' work instead of idling the rest.
 constexpr int ENS_MAX = 1024;
 static inline int64_t ens_plan_words(int n) { return 8 * (int64_t)(n + 1) + 9 + n; }
 
-__global__ void __launch_bounds__(ENS_MAX) k_ens_plan(const PassParams *__restrict__ reps, int n, int *plan) {
+__global__ void __launch_bounds__(ENS_MAX) k_ens_plan(const PassParams *__restrict__ reps, int n, int *plan,
+                                                      int colour, int start) {
   __shared__ int sc[ENS_MAX];
   __shared__ int nrun;
   const int r = threadIdx.x;
-  int cnt[8];
-  for (int c = 0; c < 8; c++) cnt[c] = 0;
-  int run = 0;
+  int cnt = 0, run = 0;
   if (r == 0) nrun = 0;
   if (r < n) {
     const PassParams &P = reps[r];
     const RunState *st = P.st;
-    // k_pass's condition for a replica's tiles to run
-    run = st->tau > 0.0 && !st->err && !st->stop;
-    if (run)
-      for (int c = 0; c < 8; c++) {
-        int zf;
-        cnt[c] = colour_tiles(P.L, P.ntz, c, zf);
-      }
-    plan[8 * (n + 1) + 9 + r] = run;
+    if (start) {
+      run = st->tau > 0.0 && !st->err && !st->stop;
+      plan[8 * (n + 1) + 9 + r] = run;
+    } else {
+      run = plan[8 * (n + 1) + 9 + r];
+    }
+    // k_pass's condition, checked as its colour pass starts
+    if (run && !st->err) {
+      int zf;
+      cnt = colour_tiles(P.L, P.ntz, colour, zf);
+    }
   }
   __syncthreads();
-  if (run) atomicAdd(&nrun, 1);
-  for (int c = 0; c < 8; c++) {  // inclusive scan over the replicas, in replica order
-    sc[r] = cnt[c];
+  if (start && run) atomicAdd(&nrun, 1);
+  // inclusive scan over the replicas, in replica order
+  sc[r] = cnt;
+  __syncthreads();
+  for (int off = 1; off < ENS_MAX; off <<= 1) {
+    const int v = r >= off ? sc[r - off] : 0;
     __syncthreads();
-    for (int off = 1; off < ENS_MAX; off <<= 1) {
-      const int v = r >= off ? sc[r - off] : 0;
-      __syncthreads();
-      sc[r] += v;
-      __syncthreads();
-    }
-    if (r < n) plan[c * (n + 1) + r + 1] = sc[r];
-    if (r == 0) plan[c * (n + 1)] = 0;
+    sc[r] += v;
     __syncthreads();
   }
-  if (r < 8) plan[8 * (n + 1) + r] = 0;
-  if (r == 0) plan[8 * (n + 1) + 8] = nrun;
+  if (r < n) plan[colour * (n + 1) + r + 1] = sc[r];
+  if (r == 0) {
+    plan[colour * (n + 1)] = 0;
+    plan[8 * (n + 1) + colour] = 0;
+    if (start) plan[8 * (n + 1) + 8] = nrun;
+  }
 }
 
 // A pointer read from device memory is a generic (flat) pointer to the
@@ -3995,17 +4002,20 @@ int mskg_run_ensemble(mskg_engine *const *engines, int32_t n, const int64_t *max
     grid = std::max(1, per_cu * cus);
   }
   if (ce == hipSuccess) {
-    hipLaunchKernelGGL(k_ens_plan, dim3(1), dim3(ENS_MAX), 0, sm, d_reps, n, d_plan);
+    hipLaunchKernelGGL(k_ens_plan, dim3(1), dim3(ENS_MAX), 0, sm, d_reps, n, d_plan, 0, 1);
     ce = hipGetLastError();
   }
   // one sweep of every running replica: the eight colour passes, the tile
   // sums and sweep ends, the next sweep's plan
   if (ce == hipSuccess) ce = hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal);
   if (ce == hipSuccess) {
-    for (int c = 0; c < 8; c++) hipLaunchKernelGGL(kpass, dim3(grid), dim3(WAVE), shm, sm, d_reps, n, d_plan, c);
+    for (int c = 0; c < 8; c++) {
+      if (c) hipLaunchKernelGGL(k_ens_plan, dim3(1), dim3(ENS_MAX), 0, sm, d_reps, n, d_plan, c, 0);
+      hipLaunchKernelGGL(kpass, dim3(grid), dim3(WAVE), shm, sm, d_reps, n, d_plan, c);
+    }
     hipLaunchKernelGGL(k_rsum_ens, dim3(NPART, n), dim3(256), 0, sm, d_reps, n, d_plan);
     hipLaunchKernelGGL(k_sweep_end_ens, dim3(n), dim3(NSLOT), 0, sm, d_reps, n, d_plan);
-    hipLaunchKernelGGL(k_ens_plan, dim3(1), dim3(ENS_MAX), 0, sm, d_reps, n, d_plan);
+    hipLaunchKernelGGL(k_ens_plan, dim3(1), dim3(ENS_MAX), 0, sm, d_reps, n, d_plan, 0, 1);
     ce = hipStreamEndCapture(sm, &graph);
   }
   if (ce == hipSuccess) ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);

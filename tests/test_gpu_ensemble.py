@@ -223,3 +223,54 @@ def test_ensemble_rejects_mixed_kernels():
         run_ensemble([a, a], [100, 100])
     a.close()
     b.close()
+
+
+def test_ensemble_of_coverage_and_laser_annealing_replicas(tmp_path):
+    """The other kernel instances of the ensemble's pass: replicas with
+    coverage species (the CVD-Si-SiH2 deck, H absorption and desorption)
+    and replicas in laser-annealing mode (SiGe under the tanh front) each
+    end as mskg_run leaves them alone.  The melting front carries MC
+    particles into new tile layers within a sweep, where the passes of the
+    later colours must visit them: the case that needs each colour's plan
+    made just before its pass (a plan per sweep parted from mskg_run after
+    ~2e4 events here)."""
+    from tests.helpers import SMALL, small_deck
+    base, box, lens = SMALL["CVD-Si-SiH2"]
+    path, d = small_deck(tmp_path, base, lens)
+    o = Oracle(*box)
+    o.load_deck(path)
+    start = oracle_state(o)
+    o.close()
+
+    def cvd(seed):
+        e = Engine(box, d.tables(), seed=seed, tile=24, events_per_pass=4.0, tau_policy=1)
+        e.load_state(*start)
+        return e
+
+    la_box = (96, 96, 240)
+    tab, _ = models.la_sige_tables(2026)
+
+    def la(seed, zm):
+        e = Engine(la_box, tab, seed=seed, tile=24, events_per_pass=4.0, tau_policy=1, la=models.LA_CONSTANTS)
+        e.build_alloy_slab(120, 0.3)
+        e.set_temperature_field(models.la_front_field(la_box, zm), "socket")
+        e.refresh_rates()
+        return e
+
+    for make, args, limit in ((cvd, [(3,), (4,)], 40000), (la, [(5, 120.0), (6, 126.0)], 30000)):
+        engs = [make(*a) for a in args]
+        done, clock, stop = run_ensemble(engs, [limit] * len(engs))
+        after = [state_of(e) for e in engs]
+        for e in engs:
+            e.close()
+        for i, a in enumerate(args):
+            e = make(*a)
+            k, t, s = e.run(limit)
+            alone = state_of(e)
+            e.close()
+            print(f"[{make.__name__} {a}] ensemble {done[i]} events, alone {k}", flush=True)
+            assert (k, t, s) == (done[i], clock[i], stop[i]) and k >= limit
+            np.testing.assert_array_equal(alone[0], after[i][0])
+            assert alone[1:4] == after[i][1:4]
+            np.testing.assert_array_equal(alone[4], after[i][4])
+            assert alone[5:] == after[i][5:]
