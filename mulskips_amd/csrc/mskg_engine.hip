@@ -3962,7 +3962,7 @@ int mskg_run_ensemble(mskg_engine *const *engines, int32_t n, const int64_t *max
     if ((bool)(e->tab.la_mode || e->tab.st_n) != la || e->cov_model != cov)
       return fail_(e0, MSKG_EINVAL, "ensemble: replica %d needs another rate-model kernel than replica 0", i);
     cap = std::max(cap, e->cap);
-    queue_bound += e->ntiles / 8 + 1;
+    queue_bound += (int64_t)(e->ntx / 2) * (e->nty / 2) * ((e->ntz + 1) / 2);  // a colour's tiles at most
   }
   // a colour's queue head counts entries in units of 64 in 32 bits
   if (queue_bound > (1ll << 25)) return fail_(e0, MSKG_EINVAL, "ensemble: too many tiles per colour pass");
