@@ -481,12 +481,6 @@ __device__ inline void sort_by_offset(int32_t *lk, float *lr, uint8_t *lc, int l
 
 // The LDS of one tile's pass besides its candidate list (which is dynamic).
 struct PassLds {
-  int stop, nt;
-  P3 tp[MAX_TOUCHED];
-  int tk[MAX_TOUCHED];
-  float tr[MAX_TOUCHED];
-  uint8_t tc[MAX_TOUCHED];
-  int found[MAX_TOUCHED];
   unsigned long long cnt[8];  // events by type (lane 0)
   long long dcc[3];           // crystal count changes (laser-annealing build's form)
   unsigned long long des[3];  // desorptions by coverage slot
@@ -500,7 +494,15 @@ template <bool LA, bool COV>
 __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, const double tau, const int t,
                                 const int colour, PassLds &S, int32_t *lk, float *lr, uint8_t *lc, const int lane) {
   const int cap = P.cap;
+  // The tile's first loads leave together, in one round trip: its list
+  // length and own count, the first WAVE list entries (read whatever the
+  // length; a lane uses its entry only below it, and the index stays in the
+  // tile's list storage), and the run state the window needs.
   const int n_in = L.tl_n[t];
+  const int own = L.tl_own[t];
+  const int k_first = L.tl_idx[(int64_t)t * L.capg + min(lane, L.capg - 1)];
+  const double rtot = st->rtot;
+  const unsigned long long sweeps = st->sweeps;
   if (n_in == 0) return 0;
   L.cur_tile = t;
   TileGeom g = tile_geom(t, L.tile, L.ntx, L.nty, L.box.lz);
@@ -509,7 +511,6 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
   // the entries other tiles appended since the tile's last pass (an order
   // that depends on timing) are then sorted by offset
   const int m_in = min(n_in, L.capg);
-  const int own = L.tl_own[t];
   int n = 0, nh = 0;
   for (int j0 = 0; j0 < m_in; j0 += WAVE) {
     int j = j0 + lane;
@@ -517,11 +518,12 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
     float r = 0.0f;
     int k = 0;
     if (j < m_in) {
-      k = L.tl_idx[(int64_t)t * L.capg + j];
+      k = j0 == 0 ? k_first : L.tl_idx[(int64_t)t * L.capg + j];
       P3 p = tile_point(L.box, g, L.tile, k);
+      // class and rate in one round trip (the rate is used only with a class)
       c = L.cls[p.i];
-      if (c) r = L.rate[p.i];
-      else L.inlist[p.i] = 0;
+      r = L.rate[p.i];
+      if (!c) L.inlist[p.i] = 0;
     }
     unsigned long long m = __ballot(c != 0);
     int slot = n + __popcll(m & ((1ull << lane) - 1ull));
@@ -545,7 +547,6 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
   sort_by_offset(lk, lr, lc, nh, n, lane);
 
   // 2. rejection-free KMC inside the tile for the window tau
-  const double rtot = st->rtot;
   double tloc = 0.0;
   unsigned long long nev = 0;
   if (lane < 8) S.cnt[lane] = 0;
@@ -553,8 +554,8 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
   if (COV && lane < 3) S.des[lane] = 0;
   U4 ctr;
   ctr.x = (uint32_t)t;
-  ctr.y = (uint32_t)st->sweeps;
-  ctr.z = (uint32_t)(st->sweeps >> 32) ^ ((uint32_t)colour << 29);
+  ctr.y = (uint32_t)sweeps;
+  ctr.z = (uint32_t)(sweeps >> 32) ^ ((uint32_t)colour << 29);
   ctr.w = 0;
   int err = 0;
   // every wave ends its pass: the window tau bounds the expected number of
@@ -614,8 +615,8 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
       EvResult res;
       execute<LA, COV>(L, rtot, s, itrans, ch, res, S.sc, lane);
       __syncthreads();  // neighbour updates by lanes 1..3 are visible to lane 0
+      err = __builtin_amdgcn_readfirstlane(res.err);  // res is complete on lane 0
       if (lane == 0) {
-        err = res.err;
         if (P.log) {
           const unsigned long long li = atomicAdd(&st->log_n, 1ull);
           if (li < P.log_cap) {
@@ -634,62 +635,60 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
         if (slot >= 0 && slot < 8) S.cnt[slot]++;
         if (LA && L.cl && res.species && itrans <= 4) S.dcc[res.species - 1] += itrans <= 3 ? 1 : -1;
         if (COV && itrans == 5 && res.species) S.des[res.species - 1]++;
-        for (int q = 0; q < S.sc.ntouched; q++) S.tp[q] = S.sc.touched[q];
-        S.nt = S.sc.ntouched;
-        S.stop = err;
       }
     }
     nev++;
-    __syncthreads();
-    if (S.stop) {
-      err = S.stop;
-      break;
-    }
+    if (err) break;
     // the touched points' new rates, one lane each, as execute() left them
-    // on chip; a point that got no new rate keeps its list entry as it is
-    const int nt = S.nt;
+    // on chip (S.sc holds them until the next event); a point that got no
+    // new rate keeps its list entry as it is
+    const int nt = __builtin_amdgcn_readfirstlane(S.sc.ntouched);
+    int kl = -1;  // tile-local offset of touched point `lane` (-1: none / outside the tile)
+    float tr = 0.0f;
+    uint8_t tc = 0;
     if (lane < nt) {
-      const P3 tp = S.tp[lane];
+      const P3 tp = S.sc.touched[lane];
       const int ts = S.sc.tset[lane];
-      const int kl = ts ? tile_local(g, L.tile, tp) : -1;
-      S.tk[lane] = kl;
-      S.found[lane] = -1;
+      kl = ts ? tile_local(g, L.tile, tp) : -1;
       if (kl >= 0) {
         if (ts == 1) {
-          S.tr[lane] = S.sc.trate[lane];
-          S.tc[lane] = S.sc.tcls[lane];
+          tr = S.sc.trate[lane];
+          tc = S.sc.tcls[lane];
         } else {
-          S.tr[lane] = L.rate[tp.i];
-          S.tc[lane] = L.cls[tp.i];
+          tr = L.rate[tp.i];
+          tc = L.cls[tp.i];
         }
       }
     }
-    __syncthreads();
-    // refresh the on-chip list: every lane matches its entries against the
-    // touched offsets (list entries are distinct, so each touched point has
-    // at most one slot; a point touched twice finds the same slot) ...
-    for (int j = lane; j < n; j += WAVE) {
-      const int kj = lk[j];
-      for (int q = 0; q < nt; q++)
-        if (S.tk[q] == kj) S.found[q] = j;
+    // refresh the on-chip list: the slot of each touched point, one ballot
+    // of the list against its offset (list entries are distinct, so each
+    // touched point has at most one slot; a point touched twice finds the
+    // same slot), and whether an earlier touched point has the same offset
+    int found = -1;
+    bool dup = false;
+    const int k_lane = lane < n ? lk[lane] : -1;  // the list's first WAVE entries, one a lane
+    for (int q = 0; q < nt; q++) {
+      const int kq = __builtin_amdgcn_readlane(kl, q);
+      if (q < lane && kq == kl) dup = true;
+      if (kq < 0) continue;
+      unsigned long long m = __ballot(k_lane == kq);
+      int f = m ? __ffsll((long long)m) - 1 : -1;
+      for (int j0 = WAVE; f < 0 && j0 < n; j0 += WAVE) {
+        const int kj = j0 + lane < n ? lk[j0 + lane] : -1;
+        m = __ballot(kj == kq);
+        if (m) f = j0 + __ffsll((long long)m) - 1;
+      }
+      if (lane == q) found = f;
     }
-    __syncthreads();
     // ... updates in place, and appends the new MC particles (first
     // occurrence of each) after the list in touched order
     bool app = false;
-    int kl = -1;
-    if (lane < nt) {
-      kl = S.tk[lane];
-      if (kl >= 0) {
-        const int f = S.found[lane];
-        if (f >= 0) {
-          lr[f] = S.tr[lane];
-          lc[f] = S.tc[lane];
-        } else if (S.tc[lane]) {
-          app = true;
-          for (int q = 0; q < lane; q++)
-            if (S.tk[q] == kl) app = false;
-        }
+    if (kl >= 0) {
+      if (found >= 0) {
+        lr[found] = tr;
+        lc[found] = tc;
+      } else if (tc) {
+        app = !dup;
       }
     }
     const unsigned long long am = __ballot(app);
@@ -697,8 +696,8 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
       const int slot = n + __popcll(am & ((1ull << lane) - 1ull));
       if (slot < cap) {
         lk[slot] = kl;
-        lr[slot] = S.tr[lane];
-        lc[slot] = S.tc[lane];
+        lr[slot] = tr;
+        lc[slot] = tc;
       }
     }
     if (n + __popcll(am) > cap) {  // the list keeps its first cap entries
