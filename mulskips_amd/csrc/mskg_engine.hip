@@ -129,6 +129,37 @@ __device__ inline double wave_incl_scan(double v, int lane) {
   return v;
 }
 
+// The value of v in another lane of the row (DPP, both halves of the
+// double moved by the same control); lanes the control gives no source, and
+// rows outside ROW_MASK, get 0.0.  Every lane of the wave must be active.
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_take(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// Inclusive prefix sum over the wave's lanes in registers: prefix sums
+// within each row of 16 lanes (row_shr 1, 2, 4, 8), then row 0's total into
+// row 1 and row 2's into row 3 (row_bcast:15), then rows 0-1's total into
+// rows 2 and 3 (row_bcast:31).  Every lane of the wave must be active.
+__device__ inline double wave_incl_scan_dpp(double v) {
+  v += dpp_take<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_take<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_take<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_take<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_take<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_take<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+// v of lane l (l the same in every lane).
+__device__ inline double read_lane(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ inline void slot_add(RunState *st, int b, double r, unsigned long long n, unsigned long long nev,
                                 const unsigned long long *cnt) {
   Slot &sl = st->slot[b & (NSLOT - 1)];
@@ -567,8 +598,8 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
     // the lanes' partial sums gives the total and the lane that holds the pick
     double part = 0.0;
     for (int j = lane; j < n; j += WAVE) part += (double)lr[j];
-    const double inc = wave_incl_scan(part, lane);
-    const double R = __shfl(inc, WAVE - 1, WAVE);
+    const double inc = wave_incl_scan_dpp(part);
+    const double R = read_lane(inc, WAVE - 1);
     if (!(R > 0.0)) break;
     // one Philox block per event: waiting time, selection, event type and
     // stacking choice, 32 random bits each (the reference's ran3 has fewer)
@@ -602,7 +633,7 @@ __device__ inline int pass_tile(const PassParams &P, Lattice &L, RunState *st, c
       }
       if (sel < 0) sel = last;
     }
-    sel = __shfl(sel, owner, WAVE);
+    sel = __builtin_amdgcn_readlane(sel, owner);
     if (sel < 0) break;
     {
       const P3 s = tile_point(L.box, g, L.tile, lk[sel]);
