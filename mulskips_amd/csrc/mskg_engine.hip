@@ -82,8 +82,6 @@ struct RunState {
   unsigned long long log_n;
   double max_time;                  // the run's limits: the sweep that reaches one stops the run
   unsigned long long max_events;
-  Slot slot[NSLOT];
-  double rpart[NPART];  // partial sums of the tile totals (k_rsum)
   int err;
   int err_site[3];
   int stop;
@@ -96,7 +94,12 @@ struct RunState {
   unsigned long long desorbed[3];  // desorptions by coverage slot (CountCov)
   int final_sweep;                 // the window was shortened to end at max_time
   unsigned long long guard_exits;  // tile windows cut short by the per-window event cap
+  // the sweep's working sums, last: the host reads and writes only the
+  // fields before them (RS_HEAD bytes)
+  Slot slot[NSLOT];
+  double rpart[NPART];  // partial sums of the tile totals (k_rsum)
 };
+constexpr size_t RS_HEAD = offsetof(RunState, slot);
 
 struct PassParams {
   Lattice L;
@@ -2534,9 +2537,16 @@ static int word_kind(const mskg_engine *e, int32_t v, int *kind) {
 
 static int la_variant_off(mskg_engine *e);
 static int census(mskg_engine *e, int64_t out[6]);
+// The run state's fields before the sweep sums (slot, rpart: left as they
+// are in st).
 static int read_state(mskg_engine *e, RunState &st) {
-  HIPCHK(e, hipMemcpyAsync(&st, e->d_st, sizeof st, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(e, hipMemcpyAsync(&st, e->d_st, RS_HEAD, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(e, hipStreamSynchronize(e->stream));
+  return MSKG_OK;
+}
+// ... and back (the device's sweep sums kept).
+static int write_state_head(mskg_engine *e, const RunState &st) {
+  HIPCHK(e, hipMemcpyAsync(e->d_st, &st, RS_HEAD, hipMemcpyHostToDevice, e->stream));
   return MSKG_OK;
 }
 
@@ -3090,7 +3100,7 @@ int mskg_set_la_variant(mskg_engine *e, const double p0[3], const double tm[3], 
     st.dcc[k] = 0ull;
     st.cl[k] = st.cinit[k] - st.cc[k];
   }
-  HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+  if (int rc2 = write_state_head(e, st)) return rc2;
   HIPCHK(e, hipStreamSynchronize(e->stream));
   e->lists_dirty = true;
   return MSKG_OK;
@@ -3288,12 +3298,12 @@ static int run_begin(mskg_engine *e, int64_t max_events, double max_time, RunSta
     s0.tau = 0.0;  // k_sweep_end with tau = 0 only sets the next window
     s0.stop = 0;
     s0.final_sweep = 0;
-    memset(s0.slot, 0, sizeof s0.slot);
     s0.tau_policy = e->cfg.tau_policy;
     s0.events_per_pass = e->cfg.events_per_pass > 0 ? e->cfg.events_per_pass : 1.0;
     s0.max_time = max_time;
     s0.max_events = st.events + (unsigned long long)max_events;
-    HIPCHK(e, hipMemcpyAsync(e->d_st, &s0, sizeof s0, hipMemcpyHostToDevice, e->stream));
+    if (int rc2 = write_state_head(e, s0)) return rc2;
+    HIPCHK(e, hipMemsetAsync((char *)e->d_st + RS_HEAD, 0, sizeof s0.slot, e->stream));
   }
   // tile totals of the current rates, then the first window
   hipLaunchKernelGGL(k_tile_rates, dim3(tblocks), dim3(256), 0, e->stream, L, e->ntiles, e->d_st);
@@ -3837,7 +3847,7 @@ int mskg_set_event_log(mskg_engine *e, int64_t capacity) {
     int rc = read_state(e, st);
     if (rc) return rc;
     st.log_n = 0;
-    HIPCHK(e, hipMemcpyAsync(e->d_st, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+    if (int rc2 = write_state_head(e, st)) return rc2;
     HIPCHK(e, hipStreamSynchronize(e->stream));
   }
   return MSKG_OK;
