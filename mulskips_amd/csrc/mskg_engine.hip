@@ -1781,8 +1781,9 @@ struct AdjDesc {
   const mskg_event *log;
   const double *dt;
   long long n;
-  const int *cnt0;           // [KEY_N] initial class populations
-  double iso0;               // initial sum of isolated-atom rates
+  const int *cnt0;           // [KEY_N + 1] initial class populations, then the unsupported points
+  const double *iso0;        // initial sum of isolated-atom rates
+  const int *sweep_err;      // the rate sweep's error flag (the engine's)
   double *S;                 // [ADJ_NS]
   double *A;                 // [KEY_N]
   double *scal;              // [4]: sum dt, integral of isolated rates, sum of their log rates, events
@@ -1811,7 +1812,7 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
   }
   for (int c = lane; c < ADJ_NS; c += WAVE) D.S[c] = 0.0;
   __syncthreads();
-  double t = 0.0, ttot = 0.0, r_iso = D.iso0, tau_iso = 0.0, iso_int = 0.0, iso_log = 0.0;
+  double t = 0.0, ttot = 0.0, r_iso = *D.iso0, tau_iso = 0.0, iso_int = 0.0, iso_log = 0.0;
   int err = 0;
   long long k = 0;
   // the log is read one event ahead: its loads overlap the event before
@@ -1926,6 +1927,19 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
     D.scal[3] = (double)k;
     if (err) D.st->err = err;
   }
+}
+
+// Before the replay: each segment's rate-sweep error and unsupported-point
+// count (out[2i], out[2i + 1]); after it: each replica's run error (out[i]).
+__global__ void __launch_bounds__(256) k_adj_check(const AdjDesc *descs, int n, int *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[2 * i] = descs[i].sweep_err[0];
+  out[2 * i + 1] = descs[i].cnt0[KEY_N];
+}
+__global__ void __launch_bounds__(256) k_adj_errs(const AdjDesc *descs, int n, int *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = descs[i].st->err;
 }
 
 // Initial class populations and isolated-rate sum of one lattice.
@@ -4117,134 +4131,168 @@ int mskg_adjoint_checkpointed(mskg_engine *const *engines, int32_t n, const mskg
   if (every < 0 || (every > 0 && !hist) || (every == 0 && hist)) return fail_(engines[0], MSKG_EINVAL, "checkpoints: every %lld", (long long)every);
   mskg_engine *e0 = engines[0];
   if (!e0) return MSKG_EINVAL;
-  std::vector<AdjDesc> descs(n);
-  std::vector<void *> scratch;
-  auto cleanup = [&]() {
-    for (void *q : scratch) hipFree(q);
-  };
-  auto dev = [&](size_t bytes) -> void * {
-    void *q = nullptr;
-    if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
-    scratch.push_back(q);
-    return q;
-  };
+  // every replica's state is checked and its key arrays exist before anything is queued
+  std::vector<char> fresh(n);
+  std::vector<double> rtot(n);
+  std::vector<int64_t> rows(n);
   for (int i = 0; i < n; i++) {
     mskg_engine *e = engines[i];
-    if (!e || !e->loaded) { cleanup(); return fail_(e0, MSKG_EINVAL, "adjoint: replica %d has no state loaded", i); }
+    if (!e || !e->loaded) return fail_(e0, MSKG_EINVAL, "adjoint: replica %d has no state loaded", i);
     for (int j = 0; j < i; j++)
-      if (engines[j] == e) { cleanup(); return fail_(e0, MSKG_EINVAL, "adjoint: replica %d appears twice", i); }
-    if (e->tab.ncov || e->tab.la_mode) {
-      cleanup();
+      if (engines[j] == e) return fail_(e0, MSKG_EINVAL, "adjoint: replica %d appears twice", i);
+    if (e->tab.ncov || e->tab.la_mode)
       return fail_(e, MSKG_EINVAL, "the likelihood covers models without coverage species and without LA rates");
-    }
-    if (n_events[i] < 0 || (n_events[i] > 0 && (!logs[i] || !dts[i]))) { cleanup(); return fail_(e, MSKG_EINVAL, "bad log"); }
+    if (n_events[i] < 0 || (n_events[i] > 0 && (!logs[i] || !dts[i]))) return fail_(e, MSKG_EINVAL, "bad log");
     const int64_t np = e->box.npts;
     if (!e->d_key) {
       if (hipMalloc(&e->d_key, np * sizeof(int32_t)) != hipSuccess ||
           hipMalloc(&e->d_key_seen, np * sizeof(int32_t)) != hipSuccess ||
-          hipMalloc(&e->d_rate_seen, np * sizeof(float)) != hipSuccess) {
-        cleanup();
+          hipMalloc(&e->d_rate_seen, np * sizeof(float)) != hipSuccess)
         return fail_(e, MSKG_ENOMEM, "adjoint: no device memory for the class keys");
-      }
     }
     RunState st;
-    int rc = read_state(e, st);
-    if (rc) { cleanup(); return rc; }
-    if (st.err) { cleanup(); return fail_(e, MSKG_ESTATE, "engine is in an error state: %s", err_name(st.err)); }
+    if (int rc = read_state(e, st)) return rc;  // also: nothing of e's own stream is still running
+    if (st.err) return fail_(e, MSKG_ESTATE, "engine is in an error state: %s", err_name(st.err));
     // the trajectory starts from the engine's rates: a tracked engine's keys
     // are current; otherwise a sweep refreshes the rates and assigns the keys
-    const bool fresh = !(e->track && e->keys_valid);
+    fresh[i] = !(e->track && e->keys_valid);
+    rtot[i] = st.rtot;
+    rows[i] = every > 0 ? n_events[i] / every : 0;
+  }
+  // One device block for every segment, in regions of one kind each so that
+  // the results come back in one copy per kind: the logs and waiting times,
+  // the initial class counts (+ the unsupported points) and isolated-rate
+  // sums, S, A and the scalars in the caller's layouts, the checkpoints, the
+  // check words, the descriptors.
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  std::vector<size_t> log_off(n), dt_off(n), hist_off(n);
+  size_t lb = 0, db = 0, hb = 0;
+  for (int i = 0; i < n; i++) {
+    const size_t ne = (size_t)std::max<int64_t>(1, n_events[i]);
+    log_off[i] = lb; lb += up(ne * sizeof(mskg_event));
+    dt_off[i] = db; db += up(ne * sizeof(double));
+    hist_off[i] = hb; hb += (size_t)rows[i] * KEY_N * sizeof(int32_t);
+  }
+  const size_t cnt_b = up((size_t)n * (KEY_N + 1) * sizeof(int)), iso_b = up((size_t)n * sizeof(double)),
+               s_b = up((size_t)n * ADJ_NS * sizeof(double)), a_b = up((size_t)n * KEY_N * sizeof(double)),
+               sc_b = up((size_t)n * 4 * sizeof(double)), h_b = up(std::max<size_t>(hb, 1)),
+               chk_b = up((size_t)n * 2 * sizeof(int)), d_b = up((size_t)n * sizeof(AdjDesc));
+  const size_t o_dt = lb, o_cnt = o_dt + db, o_iso = o_cnt + cnt_b, o_s = o_iso + iso_b, o_a = o_s + s_b,
+               o_sc = o_a + a_b, o_h = o_sc + sc_b, o_chk = o_h + h_b, o_d = o_chk + chk_b, total = o_d + d_b;
+  char *arena = nullptr;
+  if (hipMalloc(&arena, total) != hipSuccess) return fail_(e0, MSKG_ENOMEM, "adjoint: no device memory for the logs");
+  int *const d_cnt = (int *)(arena + o_cnt);
+  double *const d_iso = (double *)(arena + o_iso);
+  int *const d_chk = (int *)(arena + o_chk);
+  AdjDesc *const d_descs = (AdjDesc *)(arena + o_d);
+  std::vector<AdjDesc> descs(n);
+  for (int i = 0; i < n; i++) {
+    mskg_engine *e = engines[i];
     Lattice L = make_lattice(e);
     L.inlist = nullptr;
     L.key = e->d_key;
-    e->lists_dirty = true;
-    int *d_cnt = (int *)dev((KEY_N + 1) * sizeof(int));
-    double *d_iso = (double *)dev(sizeof(double));
-    mskg_event *d_log = (mskg_event *)dev(std::max<int64_t>(1, n_events[i]) * sizeof(mskg_event));
-    double *d_dt = (double *)dev(std::max<int64_t>(1, n_events[i]) * sizeof(double));
-    double *d_S = (double *)dev(ADJ_NS * sizeof(double));
-    double *d_A = (double *)dev(KEY_N * sizeof(double));
-    double *d_sc = (double *)dev(4 * sizeof(double));
-    const int64_t rows = every > 0 ? n_events[i] / every : 0;
-    int32_t *d_hist = rows > 0 ? (int32_t *)dev((size_t)rows * KEY_N * sizeof(int32_t)) : nullptr;
-    if (!d_cnt || !d_iso || !d_log || !d_dt || !d_S || !d_A || !d_sc || (rows > 0 && !d_hist)) {
-      cleanup();
-      return fail_(e, MSKG_ENOMEM, "adjoint: no device memory for the log");
-    }
-    int blocks = (int)std::min<int64_t>((np + 255) / 256, 256 * 64);
-    HIPCHK(e, hipMemsetAsync(e->d_err, 0, sizeof(int), e->stream));
-    if (fresh) {
-      hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(blocks), dim3(256), 0, e->stream, L, st.rtot, e->d_err, 0);
-      HIPCHK(e, hipGetLastError());
-    }
-    HIPCHK(e, hipMemsetAsync(d_cnt, 0, (KEY_N + 1) * sizeof(int), e->stream));
-    HIPCHK(e, hipMemsetAsync(d_iso, 0, sizeof(double), e->stream));
-    hipLaunchKernelGGL(k_key_hist, dim3(blocks), dim3(256), 0, e->stream, L, d_cnt, d_iso);
-    HIPCHK(e, hipGetLastError());
-    HIPCHK(e, hipMemcpyAsync(e->d_key_seen, e->d_key, np * sizeof(int32_t), hipMemcpyDeviceToDevice, e->stream));
-    HIPCHK(e, hipMemcpyAsync(e->d_rate_seen, e->d_rate, np * sizeof(float), hipMemcpyDeviceToDevice, e->stream));
-    if (n_events[i] > 0) {
-      HIPCHK(e, hipMemcpyAsync(d_log, logs[i], n_events[i] * sizeof(mskg_event), hipMemcpyHostToDevice, e->stream));
-      HIPCHK(e, hipMemcpyAsync(d_dt, dts[i], n_events[i] * sizeof(double), hipMemcpyHostToDevice, e->stream));
-    }
-    int err = 0, unsup = 0;
-    HIPCHK(e, hipMemcpyAsync(&err, e->d_err, sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipMemcpyAsync(&unsup, d_cnt + KEY_N, sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    if (err) { cleanup(); return fail_(e, MSKG_ESTATE, "rate sweep: %s", err_name(err)); }
-    if (unsup) { cleanup(); return fail_(e, MSKG_EINVAL, "%d points have rates the likelihood does not cover", unsup); }
-    double iso0 = 0.0;
-    HIPCHK(e, hipMemcpy(&iso0, d_iso, sizeof(double), hipMemcpyDeviceToHost));
     AdjDesc &D = descs[i];
     D.L = L;
     D.key_seen = e->d_key_seen;
     D.rate_seen = e->d_rate_seen;
-    D.log = d_log;
-    D.dt = d_dt;
+    D.log = (const mskg_event *)(arena + log_off[i]);
+    D.dt = (const double *)(arena + o_dt + dt_off[i]);
     D.n = n_events[i];
-    D.cnt0 = d_cnt;
-    D.iso0 = iso0;
-    D.S = d_S;
-    D.A = d_A;
-    D.scal = d_sc;
+    D.cnt0 = d_cnt + (size_t)i * (KEY_N + 1);
+    D.iso0 = d_iso + i;
+    D.sweep_err = e->d_err;
+    D.S = (double *)(arena + o_s) + (size_t)i * ADJ_NS;
+    D.A = (double *)(arena + o_a) + (size_t)i * KEY_N;
+    D.scal = (double *)(arena + o_sc) + (size_t)i * 4;
     D.st = e->d_st;
-    D.hist = d_hist;
+    D.hist = rows[i] > 0 ? (int32_t *)(arena + o_h + hist_off[i]) : nullptr;
     D.every = every > 0 ? every : 1;
   }
-  AdjDesc *d_descs = (AdjDesc *)dev(n * sizeof(AdjDesc));
-  if (!d_descs) { cleanup(); return fail_(e0, MSKG_ENOMEM, "adjoint: descriptors"); }
-  hipError_t ce = hipMemcpyAsync(d_descs, descs.data(), n * sizeof(AdjDesc), hipMemcpyHostToDevice, e0->stream);
-  const size_t shm = (size_t)KEY_N * (8 + 8 + 4);
-  if (ce == hipSuccess) ce = hipEventRecord(e0->ev0, e0->stream);
+  hipStream_t sm = e0->stream;
+  auto done = [&](int rc) {
+    hipStreamSynchronize(sm);  // nothing queued may still use the block when it is freed
+    hipFree(arena);
+    return rc;
+  };
+  // the copies from host memory first, while the stream is still idle (a
+  // copy from pageable memory returns only when done)
+  hipError_t ce = hipMemcpyAsync(d_descs, descs.data(), n * sizeof(AdjDesc), hipMemcpyHostToDevice, sm);
+  for (int i = 0; i < n && ce == hipSuccess; i++)
+    if (n_events[i] > 0) {
+      ce = hipMemcpyAsync(arena + log_off[i], logs[i], n_events[i] * sizeof(mskg_event), hipMemcpyHostToDevice, sm);
+      if (ce == hipSuccess)
+        ce = hipMemcpyAsync(arena + o_dt + dt_off[i], dts[i], n_events[i] * sizeof(double), hipMemcpyHostToDevice, sm);
+    }
+  if (ce == hipSuccess) ce = hipMemsetAsync(d_cnt, 0, cnt_b, sm);
+  if (ce == hipSuccess) ce = hipMemsetAsync(d_iso, 0, iso_b, sm);
+  // each segment's initial classes: the keys (a rate sweep if they are not
+  // current), their populations, and the keys and rates as first seen
+  for (int i = 0; i < n && ce == hipSuccess; i++) {
+    mskg_engine *e = engines[i];
+    const int64_t np = e->box.npts;
+    const int blocks = (int)std::min<int64_t>((np + 255) / 256, 256 * 64);
+    ce = hipMemsetAsync(e->d_err, 0, sizeof(int), sm);
+    if (ce == hipSuccess && fresh[i]) {
+      hipLaunchKernelGGL(MSKG_VARIANT(k_rate_sweep, e->tab.la_mode || e->tab.st_n, e->cov_model), dim3(blocks),
+                         dim3(256), 0, sm, descs[i].L, rtot[i], e->d_err, 0);
+      ce = hipGetLastError();
+    }
+    if (ce == hipSuccess) {
+      hipLaunchKernelGGL(k_key_hist, dim3(blocks), dim3(256), 0, sm, descs[i].L, d_cnt + (size_t)i * (KEY_N + 1),
+                         d_iso + i);
+      ce = hipGetLastError();
+    }
+    if (ce == hipSuccess) ce = hipMemcpyAsync(e->d_key_seen, e->d_key, np * sizeof(int32_t), hipMemcpyDeviceToDevice, sm);
+    if (ce == hipSuccess) ce = hipMemcpyAsync(e->d_rate_seen, e->d_rate, np * sizeof(float), hipMemcpyDeviceToDevice, sm);
+    e->lists_dirty = true;
+  }
+  const int cblocks = (n + 255) / 256;
+  std::vector<int> chk(2 * (size_t)n);
   if (ce == hipSuccess) {
-    hipLaunchKernelGGL(k_adjoint, dim3(n), dim3(WAVE), shm, e0->stream, d_descs);
+    hipLaunchKernelGGL(k_adj_check, dim3(cblocks), dim3(256), 0, sm, d_descs, n, d_chk);
     ce = hipGetLastError();
   }
-  if (ce == hipSuccess) ce = hipEventRecord(e0->ev1, e0->stream);
-  if (ce == hipSuccess) ce = hipStreamSynchronize(e0->stream);
-  if (ce != hipSuccess) { cleanup(); return fail_(e0, MSKG_EDEVICE, "k_adjoint: %s", hipGetErrorString(ce)); }
+  if (ce == hipSuccess) ce = hipMemcpyAsync(chk.data(), d_chk, 2 * (size_t)n * sizeof(int), hipMemcpyDeviceToHost, sm);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(sm);
+  if (ce != hipSuccess) return done(fail_(e0, MSKG_EDEVICE, "adjoint setup: %s", hipGetErrorString(ce)));
+  for (int i = 0; i < n; i++) {
+    if (chk[2 * i]) return done(fail_(engines[i], MSKG_ESTATE, "rate sweep: %s", err_name(chk[2 * i])));
+    if (chk[2 * i + 1])
+      return done(fail_(engines[i], MSKG_EINVAL, "%d points have rates the likelihood does not cover", chk[2 * i + 1]));
+  }
+  const size_t shm = (size_t)KEY_N * (8 + 8 + 4);
+  ce = hipEventRecord(e0->ev0, sm);
+  if (ce == hipSuccess) {
+    hipLaunchKernelGGL(k_adjoint, dim3(n), dim3(WAVE), shm, sm, d_descs);
+    ce = hipGetLastError();
+  }
+  if (ce == hipSuccess) ce = hipEventRecord(e0->ev1, sm);
+  if (ce == hipSuccess) {
+    hipLaunchKernelGGL(k_adj_errs, dim3(cblocks), dim3(256), 0, sm, d_descs, n, d_chk);
+    ce = hipGetLastError();
+  }
+  // the results, one copy per kind (the checkpoints through a host buffer)
+  std::vector<int32_t> hall(hb / sizeof(int32_t));
+  if (ce == hipSuccess) ce = hipMemcpyAsync(S, arena + o_s, (size_t)n * ADJ_NS * sizeof(double), hipMemcpyDeviceToHost, sm);
+  if (ce == hipSuccess) ce = hipMemcpyAsync(A, arena + o_a, (size_t)n * KEY_N * sizeof(double), hipMemcpyDeviceToHost, sm);
+  if (ce == hipSuccess) ce = hipMemcpyAsync(scalars, arena + o_sc, (size_t)n * 4 * sizeof(double), hipMemcpyDeviceToHost, sm);
+  if (ce == hipSuccess) ce = hipMemcpyAsync(chk.data(), d_chk, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, sm);
+  if (ce == hipSuccess && hb) ce = hipMemcpyAsync(hall.data(), arena + o_h, hb, hipMemcpyDeviceToHost, sm);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(sm);
+  if (ce != hipSuccess) return done(fail_(e0, MSKG_EDEVICE, "k_adjoint: %s", hipGetErrorString(ce)));
   float ms = 0;
   hipEventElapsedTime(&ms, e0->ev0, e0->ev1);
   int rc_all = MSKG_OK;
   for (int i = 0; i < n; i++) {
     mskg_engine *e = engines[i];
     e->run_ms = ms;
-    ce = hipMemcpy(S + (size_t)i * ADJ_NS, descs[i].S, ADJ_NS * sizeof(double), hipMemcpyDeviceToHost);
-    if (ce == hipSuccess) ce = hipMemcpy(A + (size_t)i * KEY_N, descs[i].A, KEY_N * sizeof(double), hipMemcpyDeviceToHost);
-    if (ce == hipSuccess) ce = hipMemcpy(scalars + (size_t)i * 4, descs[i].scal, 4 * sizeof(double), hipMemcpyDeviceToHost);
-    if (ce == hipSuccess && descs[i].hist)
-      ce = hipMemcpy(hist[i], descs[i].hist, (size_t)(n_events[i] / every) * KEY_N * sizeof(int32_t),
-                     hipMemcpyDeviceToHost);
-    if (ce != hipSuccess) { cleanup(); return fail_(e, MSKG_EDEVICE, "adjoint download: %s", hipGetErrorString(ce)); }
-    RunState st;
-    int rc = read_state(e, st);
-    if (rc) { cleanup(); return rc; }
-    if (st.err && rc_all == MSKG_OK)
+    if (rows[i] > 0)
+      memcpy(hist[i], hall.data() + hist_off[i] / sizeof(int32_t), (size_t)rows[i] * KEY_N * sizeof(int32_t));
+    if (chk[i] && rc_all == MSKG_OK)
       rc_all = fail_(e, MSKG_ESTATE, "adjoint replay of replica %d stopped after %lld events: %s", i,
-                     (long long)scalars[(size_t)i * 4 + 3], err_name(st.err));
+                     (long long)scalars[(size_t)i * 4 + 3], err_name(chk[i]));
   }
-  cleanup();
-  return rc_all;
+  return done(rc_all);
 }
 
 int mskg_frame(mskg_engine *e, int32_t *main_rec, int64_t cap_main, int32_t *void_rec, int64_t cap_void,
