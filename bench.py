@@ -86,7 +86,8 @@ def parse():
     ap.add_argument("--trajectories", type=int, default=64, help="adjoint leg (0 = skip)")
     ap.add_argument("--trajectory-events", type=int, default=1000000)
     ap.add_argument("--la-cycles", type=int, default=10, help="laser-annealing coupling leg (0 = skip)")
-    ap.add_argument("--checkpoint-every", type=int, default=100000)
+    ap.add_argument("--checkpoint-every", type=int, default=40000,
+                    help="adjoint leg: events per replayed segment (a multiple of 10^4)")
     ap.add_argument("--tune", action="store_true",
                     help="launch-shape leg: every tile edge x events per pass 2/4/8 timed on the benchmark model")
     return ap.parse_args()

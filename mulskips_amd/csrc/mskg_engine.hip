@@ -1790,13 +1790,12 @@ struct AdjDesc {
   RunState *st;
   int32_t *hist;             // may be null: [n / every][KEY_N] class populations after every `every` events
   long long every;
+  double *acc;               // [2][KEY_N]: each class's population integral, then the time of its last change
 };
 
 __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
   extern __shared__ unsigned char smem[];
-  double *A = (double *)smem;             // [KEY_N]
-  double *tau = A + KEY_N;                // [KEY_N]
-  int *cnt = (int *)(tau + KEY_N);        // [KEY_N]
+  int *cnt = (int *)smem;                 // [KEY_N] class populations
   __shared__ EvScratch sc;
   __shared__ int s_bad;
   __shared__ int s_o[MAX_TOUCHED], s_nw[MAX_TOUCHED];
@@ -1805,6 +1804,9 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
   const AdjDesc D = descs[blockIdx.x];
   Lattice L = D.L;
   L.cur_tile = -1;
+  // the per-class integrals and change times in the segment's own device
+  // memory (LDS holds the populations only, so that more waves fit a CU)
+  double *const A = D.acc, *const tau = D.acc + KEY_N;
   for (int c = lane; c < KEY_N; c += WAVE) {
     A[c] = 0.0;
     tau[c] = 0.0;
@@ -1859,14 +1861,14 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
     // the touched points' keys and rates, before and after, one lane each
     // (a point listed twice is taken once); every class change of the event
     // happens at time t, so the accounting below does not depend on order
-    const int nt = sc.ntouched;
+    const int nt = __builtin_amdgcn_readfirstlane(sc.ntouched);
+    int o = -3, nw = -1;  // o = -3: a point an earlier lane holds (or none)
+    float ro = 0.0f, rn = 0.0f;
     if (lane < nt) {
       const int64_t p = sc.touched[lane].i;
       bool first = p >= 0;
       for (int q = 0; q < lane; q++)
         if (sc.touched[q].i == p) first = false;
-      int o = -1, nw = -1;
-      float ro = 0.0f, rn = 0.0f;
       if (first) {
         o = D.key_seen[p];
         nw = L.key[p];
@@ -1875,34 +1877,51 @@ __global__ void __launch_bounds__(WAVE) k_adjoint(const AdjDesc *descs) {
         D.key_seen[p] = nw;
         D.rate_seen[p] = rn;
       }
-      s_o[lane] = first ? o : -3;
+      s_o[lane] = o;
       s_nw[lane] = nw;
       s_ro[lane] = ro;
       s_rn[lane] = rn;
     }
-    __syncthreads();
-    if (lane == 0) {
-      int bad = res.err;
-      for (int q = 0; q < nt && !bad; q++) {
-        const int o = s_o[q], nw = s_nw[q];
-        if (o == -3) continue;
-        if (nw == KEY_UNSUPPORTED) { bad = ERR_COVERAGE; break; }
-        if (o != nw) {
-          if (o >= 0) { A[o] += cnt[o] * (t - tau[o]); tau[o] = t; cnt[o]--; }
-          if (nw >= 0) { A[nw] += cnt[nw] * (t - tau[nw]); tau[nw] = t; cnt[nw]++; }
-        }
-        const double ro = o == KEY_ISO ? (double)s_ro[q] : 0.0;
-        const double rn = nw == KEY_ISO ? (double)s_rn[q] : 0.0;
-        if (ro != rn) {
-          iso_int += r_iso * (t - tau_iso);
-          tau_iso = t;
-          r_iso += rn - ro;
+    int bad = __builtin_amdgcn_readfirstlane(res.err);  // res is complete on lane 0
+    if (!bad && __ballot(o != -3 && nw == KEY_UNSUPPORTED)) bad = ERR_COVERAGE;
+    if (!bad) {
+      // Every class change of the event happens at time t: a class's
+      // integral takes its population up to t once (a second change at t
+      // adds population x 0), so the earliest lane holding a class, as old
+      // or new class in touched order, updates it with the population before
+      // the event, and the populations then change by atomics.
+      const bool ch = o != -3 && o != nw;
+      const int co = ch && o >= 0 ? o : -1, cn = ch && nw >= 0 ? nw : -1;
+      bool do_o = co >= 0, do_n = cn >= 0;
+      for (int q = 0; q < nt; q++) {
+        const int qo = __builtin_amdgcn_readlane(co, q), qn = __builtin_amdgcn_readlane(cn, q);
+        if (q < lane) {
+          if (qo == co || qn == co) do_o = false;
+          if (qo == cn || qn == cn) do_n = false;
         }
       }
-      s_bad = bad;
+      if (do_o) { A[co] += cnt[co] * (t - tau[co]); tau[co] = t; }
+      if (do_n) { A[cn] += cnt[cn] * (t - tau[cn]); tau[cn] = t; }
+      __syncthreads();  // every first update has read its population
+      if (co >= 0) atomicAdd(&cnt[co], -1);
+      if (cn >= 0) atomicAdd(&cnt[cn], 1);
+      // the isolated-atom rate sum changes in touched order (lane 0; rare)
+      const bool iso_ch = o != -3 && (o == KEY_ISO ? (double)ro : 0.0) != (nw == KEY_ISO ? (double)rn : 0.0);
+      if (__ballot(iso_ch) && lane == 0)
+        for (int q = 0; q < nt; q++) {
+          const int oq = s_o[q], nq = s_nw[q];
+          if (oq == -3) continue;
+          const double roq = oq == KEY_ISO ? (double)s_ro[q] : 0.0;
+          const double rnq = nq == KEY_ISO ? (double)s_rn[q] : 0.0;
+          if (roq != rnq) {
+            iso_int += r_iso * (t - tau_iso);
+            tau_iso = t;
+            r_iso += rnq - roq;
+          }
+        }
     }
     __syncthreads();
-    if (s_bad) { err = s_bad; k++; break; }
+    if (bad) { err = bad; k++; break; }
     if (D.hist && (k + 1) % D.every == 0) {  // checkpoint: the class populations after event k
       int32_t *row = D.hist + ((k + 1) / D.every - 1) * KEY_N;
       for (int c = lane; c < KEY_N; c += WAVE) row[c] = cnt[c];
@@ -4163,7 +4182,7 @@ int mskg_adjoint_checkpointed(mskg_engine *const *engines, int32_t n, const mskg
   // the results come back in one copy per kind: the logs and waiting times,
   // the initial class counts (+ the unsupported points) and isolated-rate
   // sums, S, A and the scalars in the caller's layouts, the checkpoints, the
-  // check words, the descriptors.
+  // check words, the descriptors, the replay's per-class integrals.
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   std::vector<size_t> log_off(n), dt_off(n), hist_off(n);
   size_t lb = 0, db = 0, hb = 0;
@@ -4176,9 +4195,11 @@ int mskg_adjoint_checkpointed(mskg_engine *const *engines, int32_t n, const mskg
   const size_t cnt_b = up((size_t)n * (KEY_N + 1) * sizeof(int)), iso_b = up((size_t)n * sizeof(double)),
                s_b = up((size_t)n * ADJ_NS * sizeof(double)), a_b = up((size_t)n * KEY_N * sizeof(double)),
                sc_b = up((size_t)n * 4 * sizeof(double)), h_b = up(std::max<size_t>(hb, 1)),
-               chk_b = up((size_t)n * 2 * sizeof(int)), d_b = up((size_t)n * sizeof(AdjDesc));
+               chk_b = up((size_t)n * 2 * sizeof(int)), d_b = up((size_t)n * sizeof(AdjDesc)),
+               acc_b = up((size_t)n * 2 * KEY_N * sizeof(double));
   const size_t o_dt = lb, o_cnt = o_dt + db, o_iso = o_cnt + cnt_b, o_s = o_iso + iso_b, o_a = o_s + s_b,
-               o_sc = o_a + a_b, o_h = o_sc + sc_b, o_chk = o_h + h_b, o_d = o_chk + chk_b, total = o_d + d_b;
+               o_sc = o_a + a_b, o_h = o_sc + sc_b, o_chk = o_h + h_b, o_d = o_chk + chk_b, o_acc = o_d + d_b,
+               total = o_acc + acc_b;
   char *arena = nullptr;
   if (hipMalloc(&arena, total) != hipSuccess) return fail_(e0, MSKG_ENOMEM, "adjoint: no device memory for the logs");
   int *const d_cnt = (int *)(arena + o_cnt);
@@ -4207,6 +4228,7 @@ int mskg_adjoint_checkpointed(mskg_engine *const *engines, int32_t n, const mskg
     D.st = e->d_st;
     D.hist = rows[i] > 0 ? (int32_t *)(arena + o_h + hist_off[i]) : nullptr;
     D.every = every > 0 ? every : 1;
+    D.acc = (double *)(arena + o_acc) + (size_t)i * 2 * KEY_N;
   }
   hipStream_t sm = e0->stream;
   auto done = [&](int rc) {
@@ -4260,7 +4282,7 @@ int mskg_adjoint_checkpointed(mskg_engine *const *engines, int32_t n, const mskg
     if (chk[2 * i + 1])
       return done(fail_(engines[i], MSKG_EINVAL, "%d points have rates the likelihood does not cover", chk[2 * i + 1]));
   }
-  const size_t shm = (size_t)KEY_N * (8 + 8 + 4);
+  const size_t shm = (size_t)KEY_N * sizeof(int);
   ce = hipEventRecord(e0->ev0, sm);
   if (ce == hipSuccess) {
     hipLaunchKernelGGL(k_adjoint, dim3(n), dim3(WAVE), shm, sm, d_descs);
