@@ -171,6 +171,17 @@ def tune(box, tables, setup, tiles=None, events_per_pass=(1.0, 2.0, 4.0), warmup
     return dict(out, cached=False)
 
 
+def ensemble_key(device, tiles, setup_tag, max_time, max_events=None):
+    """Cache key of tune_ensemble: the device, the candidate tiles (in any
+    order), the sample (setup_tag, its size) and the timed runs' per-replica
+    limits, so that a choice is reused only for the same question."""
+    tiles = sorted(int(t) for t in tiles)
+    limits = {"max_time": [float(t) for t in max_time],
+              "max_events": None if max_events is None else [int(k) for k in max_events]}
+    body = {"tiles": tiles, "setup": setup_tag, "replicas": len(limits["max_time"]), "limits": limits}
+    return f"{device}|ensemble|{json.dumps(body, sort_keys=True)}"
+
+
 def tune_ensemble(make, replicas, tiles, max_time, max_events=None, cache=None, device=None, setup_tag=None):
     """Pick the tile edge of a batched ensemble (mskg_run_ensemble): the
     sample replicas run together once per candidate tile, each engine from
@@ -179,11 +190,14 @@ def tune_ensemble(make, replicas, tiles, max_time, max_events=None, cache=None, 
     chosen edge (it must divide every box, as candidate_tiles() checks).
     max_time / max_events: per-replica limits of the timed runs.  Cached
     like tune(), keyed by the device, the candidates and setup_tag (which
-    must name the sample and its starting states)."""
+    must name the sample and its starting states) and the limits
+    (ensemble_key)."""
     from .engine import run_ensemble
     device = device or device_name()
     tiles = sorted(int(t) for t in tiles)
-    key = f"{device}|ensemble|{json.dumps({'tiles': tiles, 'setup': setup_tag}, sort_keys=True)}"
+    if len(max_time) != len(replicas) or (max_events is not None and len(max_events) != len(replicas)):
+        raise ValueError("one max_time (and max_events, if given) per sample replica")
+    key = ensemble_key(device, tiles, setup_tag, max_time, max_events)
     store = {}
     if cache and os.path.exists(cache):
         with open(cache) as f:

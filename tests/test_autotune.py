@@ -120,3 +120,18 @@ def test_cache_key_holds_the_request(tmp_path):
     r5 = Runner({(24, 4.0): 1.0})
     assert not tune((96, 96, 192), tab, setup=None, tiles=[24], events_per_pass=(4.0,), cache=cache, device="dev",
                     runner=r5)["cached"]
+
+
+def test_ensemble_key_names_the_question():
+    """The ensemble tuner's cache key: the same for a reordered candidate
+    list, different for another sample size, limit or starting state."""
+    from mulskips_amd.autotune import ensemble_key
+    k = ensemble_key("dev", (16, 24, 32), "s", [1.0, 2.0])
+    assert k == ensemble_key("dev", (32, 16, 24), "s", [1.0, 2.0])
+    others = [ensemble_key("dev", (16, 24), "s", [1.0, 2.0]),
+              ensemble_key("dev", (16, 24, 32), "t", [1.0, 2.0]),
+              ensemble_key("dev", (16, 24, 32), "s", [1.0, 2.5]),
+              ensemble_key("dev", (16, 24, 32), "s", [1.0, 2.0, 3.0]),
+              ensemble_key("dev", (16, 24, 32), "s", [1.0, 2.0], [10, 10]),
+              ensemble_key("other", (16, 24, 32), "s", [1.0, 2.0])]
+    assert len(set(others)) == len(others) and k not in others
